@@ -1,0 +1,187 @@
+// Fused commutator + integrator-stage kernel (FP64).
+//
+// For a tile of TK consecutive k-points it forms, per k,
+//     H(k) = H0(k) + sum_d E_d xi_d(k) [+ Sigma_HF(k)] [+ Hartree diagonal]
+//     f(k) = D(k) - i [H(k), rho(k)]
+// and applies the integrator's stage update to every element in the same
+// pass.  The reference computes the same quantities as
+//     Calculate_TDHamiltonian (Simulation.cpp:296-349),
+//     Coulomb::EffectiveHamiltonian (Coulomb.cpp:134-180),
+//     commutator(Output, -i, H, Input, false) (BlockMatrix_definitions.hpp:203-227),
+//     SumWithProduct stage updates (DESolver.hpp:10-20,155-220)
+// in separate sweeps over the whole grid.  Here H and rho of the tile sit in
+// LDS and each thread produces output elements
+//     f_nm = D_nm - i sum_l (H_nl rho_lm - rho_nl H_lm).
+// LAYOUT = kFull keeps every element; kHermitian keeps the upper triangle and
+// expands the lower one by conjugation while reading LDS.
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace edus {
+
+namespace {
+
+template <int NB, int LAYOUT>
+__device__ inline void decode_plane(int p, int& n, int& m)
+{
+    if (LAYOUT == kFull) {
+        n = p / NB;
+        m = p - n * NB;
+        return;
+    }
+    int nn = 0;
+    while (p >= NB - nn) {
+        p -= NB - nn;
+        ++nn;
+    }
+    n = nn;
+    m = nn + p;
+}
+
+// LDS slot of element (a, b) and whether it must be conjugated
+template <int NB, int LAYOUT>
+__device__ inline int slot(int a, int b, bool& conj)
+{
+    if (LAYOUT == kFull) {
+        conj = false;
+        return a * NB + b;
+    }
+    conj = a > b;
+    const int lo = conj ? b : a, hi = conj ? a : b;
+    return tri_index(lo, hi, NB);
+}
+
+template <int NB, int LAYOUT>
+constexpr int tile_for()
+{
+    // keep the two LDS tiles (rho and H) of NPL*TK complex doubles under ~64 KB
+    const int npl = LAYOUT == kFull ? NB * NB : NB * (NB + 1) / 2;
+    int tk = 64;
+    while (tk > 4 && 2 * npl * tk * 16 > 65536) tk /= 2;
+    return tk;
+}
+
+template <int NB, int TK, int LAYOUT>
+__global__ void __launch_bounds__(256) comm_kernel(CommArgs a)
+{
+    constexpr int NPL = LAYOUT == kFull ? NB * NB : NB * (NB + 1) / 2;
+    __shared__ double2 sR[NPL * TK];
+    __shared__ double2 sH[NPL * TK];
+    const int k0 = blockIdx.x * TK;
+    const int Nk = a.Nk;
+
+    for (int idx = threadIdx.x; idx < NPL * TK; idx += blockDim.x) {
+        const int p = idx / TK;
+        const int kk = idx - p * TK;
+        const int k = k0 + kk;
+        double2 r = make_double2(0., 0.), h = make_double2(0., 0.);
+        if (k < Nk) {
+            const long o = long(p) * Nk + k;
+            r = a.rho[o];
+            h = a.H0[o];
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+                if (a.xi[d]) {
+                    const double2 x = a.xi[d][o];
+                    h.x += a.E[d] * x.x;
+                    h.y += a.E[d] * x.y;
+                }
+            if (a.sigma) h = cadd(h, a.sigma[o]);
+            if (a.hdiag) {
+                int n, m;
+                decode_plane<NB, LAYOUT>(p, n, m);
+                if (n == m) h = cadd(h, a.hdiag[n]);
+            }
+        }
+        sR[idx] = r;
+        sH[idx] = h;
+    }
+    __syncthreads();
+
+    const StageUpdate& u = a.up;
+    for (int o = threadIdx.x; o < NPL * TK; o += blockDim.x) {
+        const int p = o / TK;
+        const int kk = o - p * TK;
+        const int k = k0 + kk;
+        if (k >= Nk) continue;
+        int n, m;
+        decode_plane<NB, LAYOUT>(p, n, m);
+        double2 acc = make_double2(0., 0.);
+#pragma unroll
+        for (int l = 0; l < NB; ++l) {
+            bool cnl, clm;
+            const int inl = slot<NB, LAYOUT>(n, l, cnl);
+            const int ilm = slot<NB, LAYOUT>(l, m, clm);
+            double2 Hnl = sH[inl * TK + kk], Rnl = sR[inl * TK + kk];
+            double2 Rlm = sR[ilm * TK + kk], Hlm = sH[ilm * TK + kk];
+            if (cnl) { Hnl.y = -Hnl.y; Rnl.y = -Rnl.y; }
+            if (clm) { Rlm.y = -Rlm.y; Hlm.y = -Hlm.y; }
+            acc.x += Hnl.x * Rlm.x - Hnl.y * Rlm.y - (Rnl.x * Hlm.x - Rnl.y * Hlm.y);
+            acc.y += Hnl.x * Rlm.y + Hnl.y * Rlm.x - (Rnl.x * Hlm.y + Rnl.y * Hlm.x);
+        }
+        const long og = long(p) * Nk + k;
+        // f = D - i * acc
+        double2 f = make_double2(acc.y, -acc.x);
+        if (a.D) f = cadd(a.D[og], f);
+
+        if (u.out_f) u.out_f[og] = f;
+        if (u.out1) {
+            const double2 x = u.x1[og];
+            u.out1[og] = make_double2(x.x + u.b1 * f.x, x.y + u.b1 * f.y);
+        }
+        if (u.out2) {
+            const double2 x = u.x2[og];
+            u.out2[og] = make_double2(x.x + u.b2 * f.x, x.y + u.b2 * f.y);
+        }
+        if (u.ab_order) {
+            double2 yv = u.y[og];
+            yv.x += u.ab_c[0] * f.x;
+            yv.y += u.ab_c[0] * f.y;
+            for (int i = 1; i < u.ab_order; ++i) {
+                const double2 h = u.hist[i][og];
+                yv.x += u.ab_c[i] * h.x;
+                yv.y += u.ab_c[i] * h.y;
+            }
+            u.y[og] = yv;
+        }
+    }
+}
+
+template <int NB>
+void launch_nb(const CommArgs& a, int layout, hipStream_t s)
+{
+    if (layout == kFull) {
+        constexpr int TK = tile_for<NB, kFull>();
+        hipLaunchKernelGGL((comm_kernel<NB, TK, kFull>), dim3((a.Nk + TK - 1) / TK), dim3(256), 0, s, a);
+    } else {
+        constexpr int TK = tile_for<NB, kHermitian>();
+        hipLaunchKernelGGL((comm_kernel<NB, TK, kHermitian>), dim3((a.Nk + TK - 1) / TK), dim3(256), 0, s, a);
+    }
+}
+
+}  // namespace
+
+void launch_comm(int nb, int layout, const CommArgs& a, hipStream_t s)
+{
+    switch (nb) {
+        case 1: launch_nb<1>(a, layout, s); break;
+        case 2: launch_nb<2>(a, layout, s); break;
+        case 3: launch_nb<3>(a, layout, s); break;
+        case 4: launch_nb<4>(a, layout, s); break;
+        case 5: launch_nb<5>(a, layout, s); break;
+        case 6: launch_nb<6>(a, layout, s); break;
+        case 7: launch_nb<7>(a, layout, s); break;
+        case 8: launch_nb<8>(a, layout, s); break;
+        case 9: launch_nb<9>(a, layout, s); break;
+        case 10: launch_nb<10>(a, layout, s); break;
+        case 11: launch_nb<11>(a, layout, s); break;
+        case 12: launch_nb<12>(a, layout, s); break;
+        case 13: launch_nb<13>(a, layout, s); break;
+        case 14: launch_nb<14>(a, layout, s); break;
+        case 15: launch_nb<15>(a, layout, s); break;
+        case 16: launch_nb<16>(a, layout, s); break;
+        default: break;  // rejected at context creation
+    }
+}
+
+}  // namespace edus

@@ -1,0 +1,578 @@
+// Engine context and the low-level C ABI (include/edus_gpu.h).
+//
+// Device memory (FP64 planes, `npl` of them per field -- nb*nb in the FULL
+// layout, nb(nb+1)/2 in the HERMITIAN one; each plane a contiguous N0 x N1
+// grid, see common.hpp):
+//   y, acc, aux        integrator state: rho_n, RK4 accumulator, stage input
+//   G, D               axis-0 derivative, full E.grad rho
+//   f                  right-hand side output of edus_rhs
+//   T                  Hartree-Fock work planes (R-space convolution)
+//   H0                 H0(k) (+ the constant rho0 part of the HSEX term)
+//   xi[3]              position operator r_d(k)
+//   W                  screened potential W(R), real planes
+//   U                  eigenvectors, full planes [nb*nb][Nk]
+//   vel[3]             velocity operator
+//   hist[5]            Adams-Bashforth right-hand-side history
+//   stage              host transfer staging, [Nk][nb][nb]
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../../include/edus_gpu.h"
+#include "engine_internal.hpp"
+#include "kernels.hpp"
+#include "obs.hpp"
+
+#define HIP_CHECK EDUS_HIP_CHECK
+
+namespace edus {
+
+thread_local std::string g_last_error;
+
+void set_error(const std::string& s) { g_last_error = s; }
+
+namespace {
+
+FFTFactors factorize(int n)
+{
+    FFTFactors f{};
+    int x = n;
+    // radix 4 first (fewest passes), then 2, 3, 5, then other primes
+    while (x % 4 == 0) { f.radix[f.count++] = 4; x /= 4; }
+    for (int r : {2, 3, 5})
+        while (x % r == 0) { f.radix[f.count++] = r; x /= r; }
+    for (int r = 7; x > 1; r += 2)
+        while (x % r == 0) {
+            if (f.count >= kMaxFactors) throw EngineError(EDUS_ERR_UNSUPPORTED, "grid size has too many factors");
+            f.radix[f.count++] = r;
+            x /= r;
+        }
+    return f;
+}
+
+std::vector<double2> twiddles(int n)
+{
+    std::vector<double2> t(n);
+    for (int j = 0; j < n; ++j) {
+        // symmetric index so |angle| <= pi; long double for the rounding
+        const int jj = (2 * j > n) ? j - n : j;
+        const long double a = 2.0L * 3.14159265358979323846264338327950288L * jj / n;
+        t[j] = make_double2(double(cosl(a)), double(-sinl(a)));
+    }
+    return t;
+}
+
+// lines per work-group: ~2048 values in flight, LDS (twiddles + two buffers) <= 64 KB
+int lines_per_group(int n)
+{
+    n = std::max(n, 1);
+    const int by_lds = (4096 / n - 1) / 2;
+    return std::max(1, std::min({32, 2048 / n, by_lds}));
+}
+
+}  // namespace
+
+void Engine::alloc(void** p, size_t bytes)
+{
+    HIP_CHECK(hipMalloc(p, bytes));
+    owned.push_back(*p);
+    HIP_CHECK(hipMemsetAsync(*p, 0, bytes, s));
+}
+
+Engine::Engine(const edus_desc& d, const EngineOptions& opt)
+{
+    nb = d.nb;
+    N0 = d.N0;
+    N1 = d.N1;
+    Nk = N0 * N1;
+    layout = d.layout;
+    prec = d.precision;
+    nf = d.filledbands;
+    coulomb = d.coulomb != 0;
+    std::memcpy(A, d.A, sizeof(A));
+    if (nb < 1 || nb > kMaxNb) throw EngineError(EDUS_ERR_UNSUPPORTED, "nb must be in 1..16");
+    if (N0 < 1 || N1 < 1) throw EngineError(EDUS_ERR_ARG, "grid sizes must be positive");
+    if (layout != EDUS_LAYOUT_FULL && layout != EDUS_LAYOUT_HERMITIAN) throw EngineError(EDUS_ERR_ARG, "unknown layout");
+    if (!d.H0k) throw EngineError(EDUS_ERR_ARG, "H0k is required");
+    if (prec != EDUS_PREC_FP64 && prec != EDUS_PREC_MIXED) throw EngineError(EDUS_ERR_ARG, "unknown precision");
+    if (prec == EDUS_PREC_MIXED) throw EngineError(EDUS_ERR_UNSUPPORTED, "mixed precision not built in this library");
+    if (coulomb && (!d.W || !d.hartree || !d.rho0k))
+        throw EngineError(EDUS_ERR_ARG, "coulomb requires W, hartree and rho0k");
+    npl = num_planes(nb, layout);
+
+    HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    const size_t pb = plane_bytes();
+    const size_t full = size_t(nb) * nb * Nk * sizeof(double2);
+    for (double2** p : {&y, &acc, &aux, &G, &D, &f, &H0}) alloc(reinterpret_cast<void**>(p), pb);
+    alloc(reinterpret_cast<void**>(&stage), full);
+    for (int i = 0; i < 5; ++i) alloc(reinterpret_cast<void**>(&hist[i]), pb);
+    alloc(reinterpret_cast<void**>(&mon), 2 * sizeof(unsigned long long));
+    const int nv = 2 * nb + 6;
+    alloc(reinterpret_cast<void**>(&obs_partial), size_t(observables_blocks(Nk)) * nv * sizeof(double));
+    alloc(reinterpret_cast<void**>(&obs_out), size_t(nv) * sizeof(double));
+
+    upload(d.H0k, H0);
+    for (int k = 0; k < 3; ++k) {
+        if (d.xik[k]) {
+            alloc(reinterpret_cast<void**>(&xi[k]), pb);
+            upload(d.xik[k], xi[k]);
+        }
+        if (d.vel[k]) {
+            alloc(reinterpret_cast<void**>(&vel[k]), pb);
+            upload(d.vel[k], vel[k]);
+        }
+    }
+    if (d.U) {
+        alloc(reinterpret_cast<void**>(&U), full);
+        HIP_CHECK(hipMemcpyAsync(stage, d.U, full, hipMemcpyHostToDevice, s));
+        launch_full_to_planes(stage, U, nb, Nk, s);
+        HIP_CHECK(hipStreamSynchronize(s));
+    }
+
+    fac[0] = factorize(N0);
+    fac[1] = factorize(N1);
+    for (int ax = 0; ax < 2; ++ax) {
+        const int n = ax == 0 ? N0 : N1;
+        auto t = twiddles(n);
+        alloc(reinterpret_cast<void**>(&tw[ax]), n * sizeof(double2));
+        HIP_CHECK(hipMemcpyAsync(tw[ax], t.data(), n * sizeof(double2), hipMemcpyHostToDevice, s));
+        HIP_CHECK(hipStreamSynchronize(s));
+    }
+
+    if (opt.velocity_det > 0.0) build_velocity(opt.velocity_det);
+
+    if (coulomb) {
+        alloc(reinterpret_cast<void**>(&T), pb);
+        alloc(reinterpret_cast<void**>(&W), size_t(npl) * Nk * sizeof(double));
+        HIP_CHECK(hipMemcpyAsync(stage, d.W, full, hipMemcpyHostToDevice, s));
+        launch_to_planes_real(stage, W, nb, Nk, layout, s);
+        std::vector<double> hr(nb * nb);
+        for (int i = 0; i < nb * nb; ++i) hr[i] = d.hartree[2 * i];  // V is real (ModelCoulomb.cpp:168-176)
+        alloc(reinterpret_cast<void**>(&hart), nb * nb * sizeof(double));
+        alloc(reinterpret_cast<void**>(&hdiag), nb * sizeof(double2));
+        alloc(reinterpret_cast<void**>(&diag_partial), size_t(nb) * diag_blocks(Nk) * sizeof(double2));
+        HIP_CHECK(hipMemcpyAsync(hart, hr.data(), nb * nb * sizeof(double), hipMemcpyHostToDevice, s));
+        // H_ee = -W o (rho - rho0) + Hartree(rho - rho0) (Coulomb.cpp:150-171).  The rho0
+        // part does not change in time: fold it into H0 once, H0 <- H0 - HF[rho0].
+        upload(d.rho0k, aux);
+        fock(aux);  // T = Sigma[rho0], hdiag = Hartree[rho0]
+        launch_axpy(H0, T, -1.0, long(npl) * Nk, s);
+        launch_add_diag(H0, hdiag, -1.0, nb, Nk, layout, s);
+        HIP_CHECK(hipMemsetAsync(aux, 0, pb, s));
+    }
+    HIP_CHECK(hipStreamSynchronize(s));
+    HIP_CHECK(hipGetLastError());
+}
+
+// V_d = (-i[r_d, H0] + e_d . grad_k H0) / det, evaluated with the same
+// kernels as the right-hand side: f = D - i[H, in] with H = H0 (no field
+// terms), in = -r_d and D = e_d . grad_k H0 (Simulation.cpp:538-575).
+void Engine::build_velocity(double det)
+{
+    const size_t pb = plane_bytes();
+    for (int d = 0; d < 3; ++d) {
+        if (!vel[d]) alloc(reinterpret_cast<void**>(&vel[d]), pb);
+        HIP_CHECK(hipMemsetAsync(aux, 0, pb, s));
+        if (xi[d]) launch_axpy(aux, xi[d], -1.0, long(npl) * Nk, s);  // aux = -r_d
+        double e[3] = {0., 0., 0.};
+        e[d] = 1.0;
+        const double2* Dp = gradient(H0, e);
+        HIP_CHECK(hipMemsetAsync(f, 0, pb, s));  // zero base for the scaled store
+        CommArgs c{};
+        c.Nk = Nk;
+        c.P = npl;
+        c.rho = aux;
+        c.D = Dp;
+        c.H0 = H0;
+        c.up.x1 = f;
+        c.up.out1 = vel[d];
+        c.up.b1 = 1.0 / det;
+        launch_comm(nb, layout, c, s);
+    }
+    HIP_CHECK(hipMemsetAsync(aux, 0, pb, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+}
+
+void Engine::observables(double* pop_bloch, double* pop_wannier, double* vel_out)
+{
+    ObsArgs a{};
+    a.nb = nb;
+    a.Nk = Nk;
+    a.layout = layout;
+    a.rho = y;
+    a.U = U;
+    for (int d = 0; d < 3; ++d) a.vel[d] = vel[d];
+    a.partial = obs_partial;
+    launch_observables(a, obs_out, s);
+    const int nv = 2 * nb + 6;
+    std::vector<double> h(nv);
+    HIP_CHECK(hipMemcpyAsync(h.data(), obs_out, nv * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    for (int n = 0; n < nb; ++n) {
+        if (pop_bloch) pop_bloch[n] = U ? ((n < nf) ? 1. - h[n] : h[n]) : 0.0;  // Simulation.cpp:459-465
+        if (pop_wannier) pop_wannier[n] = h[nb + n];
+    }
+    if (vel_out)
+        for (int c = 0; c < 6; ++c) vel_out[c] = h[2 * nb + c];
+}
+
+Engine::~Engine()
+{
+    if (s) (void)hipStreamSynchronize(s);
+    for (auto& e : pending) {
+        (void)hipEventDestroy(e.a);
+        (void)hipEventDestroy(e.b);
+    }
+    for (void* p : owned) (void)hipFree(p);
+    if (s) (void)hipStreamDestroy(s);
+}
+
+void Engine::upload(const double* host_full, double2* dst)
+{
+    HIP_CHECK(hipMemcpyAsync(stage, host_full, size_t(nb) * nb * Nk * sizeof(double2), hipMemcpyHostToDevice, s));
+    launch_to_planes(stage, dst, nb, Nk, layout, s);
+    HIP_CHECK(hipStreamSynchronize(s));  // the staging buffer is reused right away
+}
+
+void Engine::download(const double2* src, double* host_full)
+{
+    launch_from_planes(src, stage, nb, Nk, layout, s);
+    HIP_CHECK(hipMemcpyAsync(host_full, stage, size_t(nb) * nb * Nk * sizeof(double2), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+}
+
+void Engine::tbegin(const char* cls, TimedLaunch& t)
+{
+    t.cls = nullptr;
+    if (!timing) return;
+    t.cls = cls;
+    HIP_CHECK(hipEventCreate(&t.a));
+    HIP_CHECK(hipEventCreate(&t.b));
+    HIP_CHECK(hipEventRecord(t.a, s));
+}
+
+void Engine::tend(TimedLaunch& t)
+{
+    if (!t.cls) return;
+    HIP_CHECK(hipEventRecord(t.b, s));
+    pending.push_back(t);
+}
+
+void Engine::collect_timing()
+{
+    HIP_CHECK(hipStreamSynchronize(s));
+    for (auto& t : pending) {
+        float ms = 0.f;
+        HIP_CHECK(hipEventElapsedTime(&ms, t.a, t.b));
+        auto& acc_ = tsum[t.cls];
+        acc_.first += ms;
+        acc_.second += 1;
+        (void)hipEventDestroy(t.a);
+        (void)hipEventDestroy(t.b);
+    }
+    pending.clear();
+}
+
+void Engine::axis_fft(int axis, int mode, const double2* in, const double2* addend, double2* out, double coef,
+                      double scale, const double* mult, const char* cls)
+{
+    AxisFFTArgs a{};
+    a.in = in;
+    a.addend = addend;
+    a.out = out;
+    a.P = npl;
+    a.N0 = N0;
+    a.N1 = N1;
+    a.axis = axis;
+    a.n = axis == 0 ? N0 : N1;
+    a.L = lines_per_group(a.n);
+    if (axis == 0) a.L = std::min(a.L, N1);
+    a.mode = mode;
+    a.coef = coef;
+    a.scale = scale;
+    a.mult = mult;
+    a.fac = fac[axis];
+    a.tw = tw[axis];
+    TimedLaunch t;
+    tbegin(cls, t);
+    launch_axis_fft(a, s);
+    tend(t);
+}
+
+// D = E . grad_k rho  (null when the field has no component along the grid):
+// c_j = a_j . E for lattice vector j, spectral derivative along grid axis j
+// (Functional_SourceTerm.hpp:20-24, kGradient_definitions.hpp:35-41).
+const double2* Engine::gradient(const double2* in, const double E[3])
+{
+    double c[2];
+    for (int j = 0; j < 2; ++j) c[j] = A[0][j] * E[0] + A[1][j] * E[1] + A[2][j] * E[2];
+    const bool use0 = (N0 > 1) && c[0] != 0.0;
+    const bool use1 = (N1 > 1) && c[1] != 0.0;
+    if (!use0 && !use1) return nullptr;
+    if (use0 && use1) {
+        axis_fft(0, 0, in, nullptr, G, c[0] / N0, 1.0, nullptr, "grad_col");
+        axis_fft(1, 0, in, G, D, c[1] / N1, 1.0, nullptr, "grad_row");
+    } else if (use0) {
+        axis_fft(0, 0, in, nullptr, D, c[0] / N0, 1.0, nullptr, "grad_col");
+    } else {
+        axis_fft(1, 0, in, nullptr, D, c[1] / N1, 1.0, nullptr, "grad_row");
+    }
+    return D;
+}
+
+// T = F^-1[ -W o (F rho)/N ]  (screened exchange in k),  hdiag = Hartree shift
+void Engine::fock(const double2* in)
+{
+    axis_fft(1, 1, in, nullptr, T, 0.0, 1.0, nullptr, "hf_row");
+    axis_fft(0, 3, T, nullptr, T, 0.0, -1.0 / double(Nk), W, "hf_col");
+    axis_fft(1, 2, T, nullptr, T, 0.0, 1.0, nullptr, "hf_row");
+    launch_hartree(in, nb, Nk, layout, hart, diag_partial, hdiag, s);
+}
+
+void Engine::evaluate(const double2* in, const double E[3], const StageUpdate& up)
+{
+    const double2* Dp = gradient(in, E);
+    if (coulomb) fock(in);
+    CommArgs c{};
+    c.Nk = Nk;
+    c.P = npl;
+    c.rho = in;
+    c.D = Dp;
+    c.H0 = H0;
+    for (int d = 0; d < 3; ++d) {
+        c.xi[d] = (xi[d] && E[d] != 0.0) ? xi[d] : nullptr;
+        c.E[d] = E[d];
+    }
+    c.sigma = coulomb ? T : nullptr;
+    c.hdiag = coulomb ? hdiag : nullptr;
+    c.up = up;
+    TimedLaunch t;
+    tbegin("comm", t);
+    launch_comm(nb, layout, c, s);
+    tend(t);
+}
+
+void Engine::monitor(const double2* rho)
+{
+    TimedLaunch t;
+    tbegin("monitor", t);
+    launch_monitor(rho, nb, Nk, layout, double(nf), mon, s);
+    tend(t);
+}
+
+void Engine::rk4_step(double dt, const double* E3x3)
+{
+    StageUpdate u{};
+    // k1 = f(t, y);   aux = y + dt/2 k1;  acc = y + dt/6 k1      (DESolver.hpp:166-171)
+    u.x1 = y; u.out1 = acc; u.b1 = dt / 6.;
+    u.x2 = y; u.out2 = aux; u.b2 = dt / 2.;
+    evaluate(y, E3x3 + 0, u);
+    // k2 = f(t+dt/2, aux); aux = y + dt/2 k2; acc += dt/3 k2     (:174-177)
+    u = StageUpdate{};
+    u.x1 = acc; u.out1 = acc; u.b1 = dt / 3.;
+    u.x2 = y; u.out2 = aux; u.b2 = dt / 2.;
+    evaluate(aux, E3x3 + 3, u);
+    // k3 = f(t+dt/2, aux); aux = y + dt k3; acc += dt/3 k3       (:179-182)
+    u = StageUpdate{};
+    u.x1 = acc; u.out1 = acc; u.b1 = dt / 3.;
+    u.x2 = y; u.out2 = aux; u.b2 = dt;
+    evaluate(aux, E3x3 + 3, u);
+    // k4 = f(t+dt, aux); y = acc + dt/6 k4                       (:184-185)
+    u = StageUpdate{};
+    u.x1 = acc; u.out1 = y; u.b1 = dt / 6.;
+    evaluate(aux, E3x3 + 6, u);
+    monitor(y);
+}
+
+void Engine::ab_step(double dt, int order, const double* E3)
+{
+    // DESolver.hpp:110-126 coefficients, 202-220 update
+    static const double b4[5] = {55. / 24., -59. / 24., 37. / 24., -3. / 8., 0.};
+    static const double b5[5] = {1901. / 720., -2774. / 720., 2616. / 720., -1274. / 720., 251. / 720.};
+    const double* beta = order == 4 ? b4 : b5;
+    StageUpdate u{};
+    u.out_f = hist[abidx[0]];
+    u.ab_order = order;
+    u.y = y;
+    for (int i = 0; i < order; ++i) {
+        u.ab_c[i] = dt * beta[i];
+        u.hist[i] = hist[abidx[i]];
+    }
+    evaluate(y, E3, u);
+    const int tmp = abidx[order - 1];
+    for (int i = order - 1; i > 0; --i) abidx[i] = abidx[i - 1];
+    abidx[0] = tmp;
+    monitor(y);
+}
+
+}  // namespace edus
+
+using edus::Engine;
+using edus::EngineError;
+
+template <class F>
+static int guarded(F&& fn)
+{
+    try {
+        fn();
+        return EDUS_OK;
+    } catch (EngineError& e) {
+        edus::set_error(e.what());
+        return e.code;
+    } catch (std::exception& e) {
+        edus::set_error(e.what());
+        return EDUS_ERR_STATE;
+    }
+}
+
+#define EDUS_REQUIRE(cond)                         \
+    do {                                           \
+        if (!(cond)) {                             \
+            edus::set_error("null argument");      \
+            return EDUS_ERR_ARG;                   \
+        }                                          \
+    } while (0)
+
+extern "C" {
+
+const char* edus_last_error(void) { return edus::g_last_error.c_str(); }
+
+edus_ctx* edus_create(const edus_desc* desc)
+{
+    if (!desc) {
+        edus::set_error("null descriptor");
+        return nullptr;
+    }
+    edus_ctx* c = nullptr;
+    int rc = guarded([&] { c = new edus_ctx{new Engine(*desc)}; });
+    return rc == EDUS_OK ? c : nullptr;
+}
+
+int edus_destroy(edus_ctx* ctx)
+{
+    if (!ctx) return EDUS_OK;
+    int rc = guarded([&] { delete ctx->e; });
+    delete ctx;
+    return rc;
+}
+
+int edus_set_state(edus_ctx* ctx, const double* rho_k)
+{
+    EDUS_REQUIRE(ctx && rho_k);
+    return guarded([&] { ctx->e->upload(rho_k, ctx->e->y); });
+}
+
+int edus_get_state(edus_ctx* ctx, double* rho_k)
+{
+    EDUS_REQUIRE(ctx && rho_k);
+    return guarded([&] { ctx->e->download(ctx->e->y, rho_k); });
+}
+
+int edus_rhs(edus_ctx* ctx, const double E[3], const double* in, double* out)
+{
+    EDUS_REQUIRE(ctx && E && in && out);
+    return guarded([&] {
+        Engine& e = *ctx->e;
+        e.upload(in, e.aux);
+        edus::StageUpdate u{};
+        u.out_f = e.f;
+        e.evaluate(e.aux, E, u);
+        e.download(e.f, out);
+    });
+}
+
+int edus_rk4_steps(edus_ctx* ctx, double dt, int nsteps, const double* E)
+{
+    EDUS_REQUIRE(ctx && (nsteps <= 0 || E));
+    if (nsteps < 0) { edus::set_error("nsteps < 0"); return EDUS_ERR_ARG; }
+    return guarded([&] {
+        for (int s = 0; s < nsteps; ++s) ctx->e->rk4_step(dt, E + 9 * size_t(s));
+        HIP_CHECK(hipGetLastError());
+    });
+}
+
+int edus_ab_steps(edus_ctx* ctx, double dt, int order, int nsteps, const double* E)
+{
+    EDUS_REQUIRE(ctx && (nsteps <= 0 || E));
+    if (order != 4 && order != 5) {
+        edus::set_error("order " + std::to_string(order) + "not implemented for AB");  // DESolver.hpp:120-124
+        return EDUS_ERR_UNSUPPORTED;
+    }
+    if (nsteps < 0) { edus::set_error("nsteps < 0"); return EDUS_ERR_ARG; }
+    return guarded([&] {
+        for (int s = 0; s < nsteps; ++s) ctx->e->ab_step(dt, order, E + 3 * size_t(s));
+        HIP_CHECK(hipGetLastError());
+    });
+}
+
+int edus_ab_reset(edus_ctx* ctx)
+{
+    EDUS_REQUIRE(ctx);
+    return guarded([&] {
+        Engine& e = *ctx->e;
+        for (int i = 0; i < 5; ++i) {
+            HIP_CHECK(hipMemsetAsync(e.hist[i], 0, e.plane_bytes(), e.s));
+            e.abidx[i] = i;
+        }
+        HIP_CHECK(hipStreamSynchronize(e.s));
+    });
+}
+
+int edus_observables(edus_ctx* ctx, double* pop_bloch, double* pop_wannier, double* vel)
+{
+    EDUS_REQUIRE(ctx);
+    return guarded([&] { ctx->e->observables(pop_bloch, pop_wannier, vel); });
+}
+
+int edus_check(edus_ctx* ctx, double* trace_dev, double* herm_dev)
+{
+    EDUS_REQUIRE(ctx);
+    return guarded([&] {
+        Engine& e = *ctx->e;
+        unsigned long long m[2];
+        HIP_CHECK(hipMemcpyAsync(m, e.mon, sizeof(m), hipMemcpyDeviceToHost, e.s));
+        HIP_CHECK(hipMemsetAsync(e.mon, 0, sizeof(m), e.s));
+        HIP_CHECK(hipStreamSynchronize(e.s));
+        double v[2];
+        std::memcpy(v, m, sizeof(v));
+        if (trace_dev) *trace_dev = v[0];
+        if (herm_dev) *herm_dev = v[1];
+    });
+}
+
+int edus_synchronize(edus_ctx* ctx)
+{
+    EDUS_REQUIRE(ctx);
+    return guarded([&] { HIP_CHECK(hipStreamSynchronize(ctx->e->s)); });
+}
+
+int edus_timing_enable(edus_ctx* ctx, int on)
+{
+    EDUS_REQUIRE(ctx);
+    return guarded([&] {
+        Engine& e = *ctx->e;
+        e.collect_timing();
+        e.tsum.clear();
+        e.timing = on != 0;
+    });
+}
+
+int edus_timing_get(edus_ctx* ctx, const char* cls, double* total_ms, long* count)
+{
+    EDUS_REQUIRE(ctx && cls);
+    return guarded([&] {
+        Engine& e = *ctx->e;
+        e.collect_timing();
+        auto it = e.tsum.find(cls);
+        if (total_ms) *total_ms = it == e.tsum.end() ? 0.0 : it->second.first;
+        if (count) *count = it == e.tsum.end() ? 0 : it->second.second;
+    });
+}
+
+}  // extern "C"

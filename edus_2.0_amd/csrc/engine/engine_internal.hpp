@@ -1,0 +1,99 @@
+// Engine class behind the C ABI.  Host-side only.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../../include/edus_gpu.h"
+#include "kernels.hpp"
+
+namespace edus {
+
+struct EngineError : std::runtime_error {
+    int code;
+    EngineError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+void set_error(const std::string& s);
+extern thread_local std::string g_last_error;
+
+#define EDUS_HIP_CHECK(x)                                                                              \
+    do {                                                                                               \
+        hipError_t e_ = (x);                                                                           \
+        if (e_ != hipSuccess)                                                                          \
+            throw ::edus::EngineError(EDUS_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_));  \
+    } while (0)
+
+struct TimedLaunch {
+    const char* cls = nullptr;
+    hipEvent_t a{}, b{};
+};
+
+// Options used by the in-library host set-up (not part of the C ABI).
+struct EngineOptions {
+    // > 0: build the velocity operator V_d = (-i[r_d, H0] + d_k_d H0) / det on
+    // the device from H0 and r (Simulation.cpp:523-576) instead of taking it
+    // from the descriptor.
+    double velocity_det = 0.0;
+};
+
+class Engine {
+  public:
+    explicit Engine(const edus_desc& d, const EngineOptions& opt = EngineOptions());
+    ~Engine();
+    Engine(const Engine&) = delete;
+    Engine& operator=(const Engine&) = delete;
+
+    int nb, N0, N1, Nk, npl, layout, prec, nf;
+    bool coulomb;
+    double A[3][3];
+    hipStream_t s = nullptr;
+
+    double2 *y = nullptr, *acc = nullptr, *aux = nullptr, *G = nullptr, *D = nullptr, *f = nullptr;
+    double2 *H0 = nullptr, *xi[3] = {nullptr, nullptr, nullptr}, *vel[3] = {nullptr, nullptr, nullptr};
+    double2 *U = nullptr, *T = nullptr, *stage = nullptr;
+    double *W = nullptr, *hart = nullptr;
+    double2 *hdiag = nullptr, *diag_partial = nullptr;
+    double *obs_partial = nullptr, *obs_out = nullptr;
+    unsigned long long* mon = nullptr;
+    double2* hist[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    int abidx[5] = {0, 1, 2, 3, 4};
+    double2* tw[2] = {nullptr, nullptr};
+    FFTFactors fac[2];
+
+    bool timing = false;
+    std::vector<TimedLaunch> pending;
+    std::map<std::string, std::pair<double, long>> tsum;
+
+    size_t plane_bytes() const { return size_t(npl) * Nk * sizeof(double2); }
+    void upload(const double* host_full, double2* dst);       // [Nk][nb][nb] -> planes
+    void download(const double2* src, double* host_full);     // planes -> [Nk][nb][nb]
+    void evaluate(const double2* in, const double E[3], const StageUpdate& up);
+    void rk4_step(double dt, const double* E3x3);
+    void ab_step(double dt, int order, const double* E3);
+    void monitor(const double2* rho);
+    void observables(double* pop_bloch, double* pop_wannier, double* vel);
+    void collect_timing();
+
+  private:
+    std::vector<void*> owned;
+    void alloc(void** p, size_t bytes);
+    void axis_fft(int axis, int mode, const double2* in, const double2* addend, double2* out, double coef,
+                  double scale, const double* mult, const char* cls);
+    const double2* gradient(const double2* in, const double E[3]);
+    void fock(const double2* in);
+    void build_velocity(double det);
+    void tbegin(const char* cls, TimedLaunch& t);
+    void tend(TimedLaunch& t);
+};
+
+}  // namespace edus
+
+// The opaque handle of the C ABI.
+struct edus_ctx {
+    edus::Engine* e;
+};

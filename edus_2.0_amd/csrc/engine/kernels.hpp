@@ -1,0 +1,61 @@
+// Kernel argument blocks and launch entry points of the EDUS engine.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "common.hpp"
+
+namespace edus {
+
+struct FFTFactors {
+    int count;
+    int radix[kMaxFactors];
+};
+
+struct AxisFFTArgs {
+    const double2* in;      // planes [P][N0][N1]
+    const double2* addend;  // optional, same layout
+    double2* out;
+    int P, N0, N1;
+    int axis;               // 0: columns (length N0), 1: rows (length N1)
+    int n;                  // line length
+    int L;                  // lines per work-group
+    int mode;               // 0 derivative, 1 forward, 2 inverse, 3 convolution
+    double coef;            // derivative: c_axis / n
+    double scale;           // transforms: output scale; convolution: factor on mult
+    const double* mult;     // convolution: real multiplier field, triangle planes [P][N0][N1]
+    FFTFactors fac;
+    const double2* tw;      // exp(-2 pi i j / n), j < n
+};
+
+// Per-stage update rule of the fused commutator kernel.  With f the
+// right-hand side at this k-point and element:
+//   out_f  = f                          (if out_f)
+//   out1   = x1 + b1 * f                (if out1)
+//   out2   = x2 + b2 * f                (if out2)
+//   AB:    y  = y + sum_i c_i h_i  with h_0 = f, h_i = hist[i]  (if ab_order)
+struct StageUpdate {
+    double2* out_f;
+    const double2* x1; double2* out1; double b1;
+    const double2* x2; double2* out2; double b2;
+    int ab_order;
+    const double2* hist[5];  // hist[0] unused (= f)
+    double ab_c[5];
+    double2* y;               // AB target
+};
+
+struct CommArgs {
+    int Nk, P;
+    const double2* rho;       // stage input, planes
+    const double2* D;         // E.grad rho part (may be null)
+    const double2* H0;
+    const double2* xi[3];     // null where E_d == 0 or operator absent
+    double E[3];
+    const double2* sigma;     // Hartree-Fock self energy in k (may be null)
+    const double2* hdiag;     // Hartree diagonal shift, nb complex on device (may be null)
+    StageUpdate up;
+};
+
+void launch_axis_fft(const AxisFFTArgs& a, hipStream_t s);
+void launch_comm(int nb, int layout, const CommArgs& a, hipStream_t s);
+
+}  // namespace edus

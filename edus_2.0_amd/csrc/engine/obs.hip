@@ -1,0 +1,303 @@
+// Layout conversion at the C-ABI boundary, observable reductions, invariant
+// monitors and the small element-wise kernels of the Hartree-Fock term.
+#include "common.hpp"
+#include "obs.hpp"
+
+namespace edus {
+
+namespace {
+
+__device__ inline void decode_tri(int p, int nb, int& n, int& m)
+{
+    int nn = 0;
+    while (p >= nb - nn) {
+        p -= nb - nn;
+        ++nn;
+    }
+    n = nn;
+    m = nn + p;
+}
+
+// [Nk][nb][nb] -> planes.  FULL: straight transpose.  HERMITIAN: upper
+// triangle of the Hermitian projection (X + X^dagger)/2.
+__global__ void to_planes_kernel(const double2* __restrict__ full, double2* __restrict__ planes, int nb, int Nk,
+                                 int layout)
+{
+    const int npl = num_planes(nb, layout);
+    const long idx = long(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (idx >= long(npl) * Nk) return;
+    const int p = int(idx / Nk);
+    const int k = int(idx - long(p) * Nk);
+    if (layout == kFull) {
+        planes[idx] = full[long(k) * nb * nb + p];
+        return;
+    }
+    int n, m;
+    decode_tri(p, nb, n, m);
+    const double2 a = full[(long(k) * nb + n) * nb + m];
+    const double2 b = full[(long(k) * nb + m) * nb + n];
+    planes[idx] = make_double2(0.5 * (a.x + b.x), 0.5 * (a.y - b.y));
+}
+
+// planes -> [Nk][nb][nb]
+__global__ void from_planes_kernel(const double2* __restrict__ planes, double2* __restrict__ full, int nb, int Nk,
+                                   int layout)
+{
+    const long idx = long(blockIdx.x) * blockDim.x + threadIdx.x;
+    const long nb2 = long(nb) * nb;
+    if (idx >= nb2 * Nk) return;
+    const int k = int(idx / nb2);
+    const int r = int(idx - long(k) * nb2);
+    const int n = r / nb, m = r - (r / nb) * nb;
+    full[idx] = element(planes, n, m, nb, layout, Nk, k);
+}
+
+// [Nk][nb][nb] complex -> real part as planes (real-valued W(R))
+__global__ void to_planes_real_kernel(const double2* __restrict__ full, double* __restrict__ planes, int nb, int Nk,
+                                      int layout)
+{
+    const int npl = num_planes(nb, layout);
+    const long idx = long(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (idx >= long(npl) * Nk) return;
+    const int p = int(idx / Nk);
+    const int k = int(idx - long(p) * Nk);
+    int n, m;
+    if (layout == kFull) {
+        n = p / nb;
+        m = p - n * nb;
+    } else {
+        decode_tri(p, nb, n, m);
+    }
+    planes[idx] = full[(long(k) * nb + n) * nb + m].x;
+}
+
+// [Nk][nb][nb] -> full planes [nb*nb][Nk] (non-Hermitian fields, e.g. U)
+__global__ void full_to_planes_kernel(const double2* __restrict__ full, double2* __restrict__ planes, int nb2,
+                                      int Nk)
+{
+    const long idx = long(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (idx >= long(nb2) * Nk) return;
+    const int q = int(idx / Nk);
+    const int k = int(idx - long(q) * Nk);
+    planes[idx] = full[long(k) * nb2 + q];
+}
+
+constexpr int kObsBlock = 128;  // keeps the (2nb+6) x block LDS tile of the observables kernel <= 38 KB
+
+// Per-block partial sums of the observables over k:
+//   vals[0..nb)       sum_k Re (U^dagger rho U)_nn      (Bloch population)
+//   vals[nb..2nb)     sum_k Re rho_nn                    (Wannier population)
+//   vals[2nb..2nb+6)  sum_k Tr(V_d rho)  (re, im per d)
+__global__ void __launch_bounds__(kObsBlock) observables_kernel(ObsArgs a)
+{
+    const int nb = a.nb, Nk = a.Nk, L = a.layout;
+    const int nv = 2 * nb + 6;
+    extern __shared__ double sh[];  // [nv][kObsBlock]
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    for (int v = 0; v < nv; ++v) sh[v * kObsBlock + threadIdx.x] = 0.;
+    if (k < Nk) {
+        for (int n = 0; n < nb; ++n) sh[(nb + n) * kObsBlock + threadIdx.x] = element(a.rho, n, n, nb, L, Nk, k).x;
+        if (a.U) {
+            for (int n = 0; n < nb; ++n) {
+                double pr = 0.;
+                for (int i = 0; i < nb; ++i) {
+                    double2 w = make_double2(0., 0.);
+                    for (int j = 0; j < nb; ++j)
+                        w = cadd(w, cmul(element(a.rho, i, j, nb, L, Nk, k), a.U[long(j * nb + n) * Nk + k]));
+                    const double2 ui = a.U[long(i * nb + n) * Nk + k];
+                    pr += ui.x * w.x + ui.y * w.y;  // Re(conj(u_i) w_i)
+                }
+                sh[n * kObsBlock + threadIdx.x] = pr;
+            }
+        }
+        for (int d = 0; d < 3; ++d) {
+            if (!a.vel[d]) continue;
+            double2 tr = make_double2(0., 0.);
+            for (int i = 0; i < nb; ++i)
+                for (int j = 0; j < nb; ++j)
+                    tr = cadd(tr, cmul(element(a.vel[d], i, j, nb, L, Nk, k), element(a.rho, j, i, nb, L, Nk, k)));
+            sh[(2 * nb + 2 * d) * kObsBlock + threadIdx.x] = tr.x;
+            sh[(2 * nb + 2 * d + 1) * kObsBlock + threadIdx.x] = tr.y;
+        }
+    }
+    __syncthreads();
+    for (int s = kObsBlock / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s)
+            for (int v = 0; v < nv; ++v) sh[v * kObsBlock + threadIdx.x] += sh[v * kObsBlock + threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int v = 0; v < nv; ++v) a.partial[long(blockIdx.x) * nv + v] = sh[v * kObsBlock];
+}
+
+// Deterministic final reduction of per-block partials (fixed order).
+__global__ void reduce_partials_kernel(const double* __restrict__ partial, int nblocks, int nv, double scale,
+                                       double* __restrict__ out)
+{
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nv) return;
+    double s = 0.;
+    for (int b = 0; b < nblocks; ++b) s += partial[long(b) * nv + v];
+    out[v] = s * scale;
+}
+
+// Per-block partial sums of the diagonal planes (for the Hartree k-mean).
+__global__ void __launch_bounds__(kObsBlock) diag_sum_kernel(const double2* __restrict__ rho, int nb, int Nk,
+                                                             int layout, double2* __restrict__ partial)
+{
+    __shared__ double2 sh[kObsBlock];
+    const int n = blockIdx.y;
+    const double2* plane = rho + long(diag_plane(n, nb, layout)) * Nk;
+    double2 s = make_double2(0., 0.);
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < Nk; k += gridDim.x * blockDim.x) s = cadd(s, plane[k]);
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int st = kObsBlock / 2; st > 0; st >>= 1) {
+        if (threadIdx.x < st) sh[threadIdx.x] = cadd(sh[threadIdx.x], sh[threadIdx.x + st]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[long(n) * gridDim.x + blockIdx.x] = sh[0];
+}
+
+// hdiag_n = sum_m Hartree_nm * rho_mm(R=0), rho_mm(R=0) = mean_k rho_mm(k)
+// (Coulomb.cpp:150-159; the rho0 part is folded into H0 at set-up).
+// One block; partials [nb][nblk].
+__global__ void hartree_kernel(const double2* __restrict__ partial, int nblk, const double* __restrict__ hart,
+                               int nb, double invNk, double2* __restrict__ hdiag)
+{
+    __shared__ double2 mean[kMaxNb];
+    const int t = threadIdx.x;
+    if (t < nb) {
+        double2 s = make_double2(0., 0.);
+        for (int b = 0; b < nblk; ++b) s = cadd(s, partial[long(t) * nblk + b]);
+        mean[t] = cscale(s, invNk);
+    }
+    __syncthreads();
+    if (t < nb) {
+        double2 h = make_double2(0., 0.);
+        for (int m = 0; m < nb; ++m) h = cadd(h, cscale(mean[m], hart[t * nb + m]));
+        hdiag[t] = h;
+    }
+}
+
+// Invariant monitors: max_k |Tr rho(k) - nf| and the Hermiticity defect
+// max_k max_{n<=m} |rho_nm - conj(rho_mn)| (HERMITIAN layout: the diagonal's
+// imaginary part, the only part storage does not fix), folded into two device
+// maxima.  Bit patterns of non-negative doubles order like the values, so an
+// integer atomicMax is exact and independent of order.
+__global__ void __launch_bounds__(kObsBlock) monitor_kernel(const double2* __restrict__ rho, int nb, int Nk,
+                                                            int layout, double nf, unsigned long long* __restrict__ mon)
+{
+    __shared__ double s_tr[kObsBlock], s_h[kObsBlock];
+    double mt = 0., mh = 0.;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < Nk; k += gridDim.x * blockDim.x) {
+        double2 tr = make_double2(-nf, 0.);
+        for (int n = 0; n < nb; ++n) {
+            const double2 v = rho[long(diag_plane(n, nb, layout)) * Nk + k];
+            tr = cadd(tr, v);
+            mh = fmax(mh, 2.0 * fabs(v.y));
+        }
+        if (layout == kFull)
+            for (int n = 0; n < nb; ++n)
+                for (int m = n + 1; m < nb; ++m) {
+                    const double2 a = rho[long(n * nb + m) * Nk + k];
+                    const double2 b = rho[long(m * nb + n) * Nk + k];
+                    mh = fmax(mh, sqrt((a.x - b.x) * (a.x - b.x) + (a.y + b.y) * (a.y + b.y)));
+                }
+        mt = fmax(mt, sqrt(tr.x * tr.x + tr.y * tr.y));
+    }
+    s_tr[threadIdx.x] = mt;
+    s_h[threadIdx.x] = mh;
+    __syncthreads();
+    for (int st = kObsBlock / 2; st > 0; st >>= 1) {
+        if (threadIdx.x < st) {
+            s_tr[threadIdx.x] = fmax(s_tr[threadIdx.x], s_tr[threadIdx.x + st]);
+            s_h[threadIdx.x] = fmax(s_h[threadIdx.x], s_h[threadIdx.x + st]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        double a = s_tr[0], b = s_h[0];
+        if (a != a) a = __builtin_huge_val();  // NaN trips the monitor
+        if (b != b) b = __builtin_huge_val();
+        atomicMax(&mon[0], static_cast<unsigned long long>(__double_as_longlong(a)));
+        atomicMax(&mon[1], static_cast<unsigned long long>(__double_as_longlong(b)));
+    }
+}
+
+__global__ void axpy_planes_kernel(double2* __restrict__ y, const double2* __restrict__ x, double a, long n)
+{
+    const long i = long(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = make_double2(y[i].x + a * x[i].x, y[i].y + a * x[i].y);
+}
+
+// H_nn(k) += scale * d_n for every k
+__global__ void add_diag_kernel(double2* __restrict__ H, const double2* __restrict__ d, double scale, int nb, int Nk,
+                                int layout)
+{
+    const long i = long(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= long(nb) * Nk) return;
+    const int n = int(i / Nk);
+    const int k = int(i - long(n) * Nk);
+    double2& h = H[long(diag_plane(n, nb, layout)) * Nk + k];
+    h = make_double2(h.x + scale * d[n].x, h.y + scale * d[n].y);
+}
+
+inline unsigned grid1d(long n, int bs) { return unsigned((n + bs - 1) / bs); }
+
+}  // namespace
+
+void launch_to_planes(const double2* full, double2* planes, int nb, int Nk, int layout, hipStream_t s)
+{
+    const long n = long(num_planes(nb, layout)) * Nk;
+    hipLaunchKernelGGL(to_planes_kernel, dim3(grid1d(n, 256)), dim3(256), 0, s, full, planes, nb, Nk, layout);
+}
+void launch_from_planes(const double2* planes, double2* full, int nb, int Nk, int layout, hipStream_t s)
+{
+    hipLaunchKernelGGL(from_planes_kernel, dim3(grid1d(long(nb) * nb * Nk, 256)), dim3(256), 0, s, planes, full, nb,
+                       Nk, layout);
+}
+void launch_to_planes_real(const double2* full, double* planes, int nb, int Nk, int layout, hipStream_t s)
+{
+    const long n = long(num_planes(nb, layout)) * Nk;
+    hipLaunchKernelGGL(to_planes_real_kernel, dim3(grid1d(n, 256)), dim3(256), 0, s, full, planes, nb, Nk, layout);
+}
+void launch_full_to_planes(const double2* full, double2* planes, int nb, int Nk, hipStream_t s)
+{
+    hipLaunchKernelGGL(full_to_planes_kernel, dim3(grid1d(long(nb) * nb * Nk, 256)), dim3(256), 0, s, full, planes,
+                       nb * nb, Nk);
+}
+int observables_blocks(int Nk) { return int(grid1d(Nk, kObsBlock)); }
+void launch_observables(const ObsArgs& a, double* out, hipStream_t s)
+{
+    const int nv = 2 * a.nb + 6;
+    const int blocks = observables_blocks(a.Nk);
+    hipLaunchKernelGGL(observables_kernel, dim3(blocks), dim3(kObsBlock), size_t(nv) * kObsBlock * sizeof(double), s,
+                       a);
+    hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(64), 0, s, a.partial, blocks, nv, 1.0 / double(a.Nk),
+                       out);
+}
+int diag_blocks(int Nk) { return int(std::min<long>(grid1d(Nk, kObsBlock), 64)); }
+void launch_hartree(const double2* rho, int nb, int Nk, int layout, const double* hart, double2* partial,
+                    double2* hdiag, hipStream_t s)
+{
+    const int nblk = diag_blocks(Nk);
+    hipLaunchKernelGGL(diag_sum_kernel, dim3(nblk, nb), dim3(kObsBlock), 0, s, rho, nb, Nk, layout, partial);
+    hipLaunchKernelGGL(hartree_kernel, dim3(1), dim3(64), 0, s, partial, nblk, hart, nb, 1.0 / double(Nk), hdiag);
+}
+void launch_monitor(const double2* rho, int nb, int Nk, int layout, double nf, unsigned long long* mon, hipStream_t s)
+{
+    const int blocks = int(std::min<long>(grid1d(Nk, kObsBlock), 512));
+    hipLaunchKernelGGL(monitor_kernel, dim3(blocks), dim3(kObsBlock), 0, s, rho, nb, Nk, layout, nf, mon);
+}
+void launch_axpy(double2* y, const double2* x, double a, long n, hipStream_t s)
+{
+    hipLaunchKernelGGL(axpy_planes_kernel, dim3(grid1d(n, 256)), dim3(256), 0, s, y, x, a, n);
+}
+void launch_add_diag(double2* H, const double2* d, double scale, int nb, int Nk, int layout, hipStream_t s)
+{
+    hipLaunchKernelGGL(add_diag_kernel, dim3(grid1d(long(nb) * Nk, 256)), dim3(256), 0, s, H, d, scale, nb, Nk,
+                       layout);
+}
+
+}  // namespace edus

@@ -1,0 +1,31 @@
+// Launchers of the conversion, observable and monitor kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+namespace edus {
+
+struct ObsArgs {
+    int nb, Nk, layout;
+    const double2* rho;      // planes in `layout`
+    const double2* U;        // full planes [nb*nb][Nk] or null
+    const double2* vel[3];   // planes in `layout` or null
+    double* partial;         // [blocks][2nb+6]
+};
+
+void launch_to_planes(const double2* full, double2* planes, int nb, int Nk, int layout, hipStream_t s);
+void launch_from_planes(const double2* planes, double2* full, int nb, int Nk, int layout, hipStream_t s);
+void launch_to_planes_real(const double2* full, double* planes, int nb, int Nk, int layout, hipStream_t s);
+void launch_full_to_planes(const double2* full, double2* planes, int nb, int Nk, hipStream_t s);
+int observables_blocks(int Nk);
+void launch_observables(const ObsArgs& a, double* out, hipStream_t s);
+int diag_blocks(int Nk);
+void launch_hartree(const double2* rho, int nb, int Nk, int layout, const double* hart, double2* partial,
+                    double2* hdiag, hipStream_t s);
+void launch_monitor(const double2* rho, int nb, int Nk, int layout, double nf, unsigned long long* mon,
+                    hipStream_t s);
+void launch_axpy(double2* y, const double2* x, double a, long n, hipStream_t s);
+void launch_add_diag(double2* H, const double2* d, double scale, int nb, int Nk, int layout, hipStream_t s);
+
+}  // namespace edus

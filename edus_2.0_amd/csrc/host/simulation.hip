@@ -1,0 +1,515 @@
+// Reference-format simulation on top of the engine: the host C++ mirror of
+// src/Simulation/Simulation.cpp and src/DESolver/DESolver.hpp.  Set-up runs
+// in FP64 on the host (explicit DFT, eigensystem, lasers, Coulomb model);
+// every per-step operation runs in the engine.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <functional>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../../include/edus_gpu.h"
+#include "../engine/engine_internal.hpp"
+#include "setup.hpp"
+
+namespace edus {
+namespace host {
+
+namespace {
+
+void parallel_for(int n, const std::function<void(int)>& body)
+{
+    const int nt = std::max(1, std::min<int>(16, int(std::thread::hardware_concurrency())));
+    if (n < 64 || nt == 1) {
+        for (int i = 0; i < n; ++i) body(i);
+        return;
+    }
+    std::vector<std::thread> pool;
+    for (int t = 0; t < nt; ++t)
+        pool.emplace_back([&, t] {
+            for (int i = t; i < n; i += nt) body(i);
+        });
+    for (auto& th : pool) th.join();
+}
+
+// MeshGrid.cpp:345-371 reduce(): fold x into [lo, up) with period `period`
+double fold(double x, double lo, double up, double period)
+{
+    if (period <= 0) return x;
+    while (x < lo && std::abs(x - lo) > kThreshold) x += period;
+    while (x > up || std::abs(x - up) < kThreshold) x -= period;
+    return x;
+}
+
+void matmul(int n, const cplx* A, const cplx* B, cplx* C)
+{
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) {
+            cplx s = 0.;
+            for (int l = 0; l < n; ++l) s += A[i * n + l] * B[l * n + j];
+            C[i * n + j] = s;
+        }
+}
+
+}  // namespace
+
+// DESolver mirror (DESolver.hpp:26-82): owns the time bookkeeping, the
+// right-hand side and stage updates live in the engine.
+class DESolver {
+  public:
+    enum SolverType { AB, RK };
+    void initialize(Engine* e, const SetOfLaser* l, SolverType t, int order)
+    {
+        engine_ = e;
+        lasers_ = l;
+        type_ = t;
+        order_ = order;
+        if (t == RK && order != 4) throw EngineError(EDUS_ERR_UNSUPPORTED, "RK order must be 4");  // DESolver.hpp:157
+        if (t == AB && order != 4 && order != 5)
+            throw EngineError(EDUS_ERR_UNSUPPORTED, "order " + std::to_string(order) + "not implemented for AB");
+    }
+    const double& get_CurrentTime() const { return CurrentTime; }
+    const double& get_ResolutionTime() const { return ResolutionTime; }
+    void set_InitialTime(double t) { InitialTime = t; }  // CurrentTime is left at 0, as the reference does
+    void set_ResolutionTime(double dt) { ResolutionTime = dt; }
+    void Propagate() { Propagate(1); }
+    void Propagate(int nstep)
+    {
+        if (nstep <= 0) return;
+        const double dt = ResolutionTime;
+        if (type_ == RK) {
+            std::vector<double> E(size_t(nstep) * 9);
+            double t = CurrentTime;
+            for (int s = 0; s < nstep; ++s) {
+                lasers_->field(t, &E[9 * size_t(s)]);
+                lasers_->field(t + dt / 2., &E[9 * size_t(s) + 3]);
+                lasers_->field(t + dt, &E[9 * size_t(s) + 6]);
+                t += dt;
+            }
+            for (int s = 0; s < nstep; ++s) {
+                engine_->rk4_step(dt, &E[9 * size_t(s)]);
+                CurrentTime += dt;
+            }
+        } else {
+            std::vector<double> E(size_t(nstep) * 3);
+            double t = CurrentTime;
+            for (int s = 0; s < nstep; ++s) {
+                lasers_->field(t, &E[3 * size_t(s)]);
+                t += dt;
+            }
+            for (int s = 0; s < nstep; ++s) {
+                engine_->ab_step(dt, order_, &E[3 * size_t(s)]);
+                CurrentTime += dt;
+            }
+        }
+    }
+
+  private:
+    Engine* engine_ = nullptr;
+    const SetOfLaser* lasers_ = nullptr;
+    SolverType type_ = RK;
+    int order_ = 4;
+    double InitialTime = 0., ResolutionTime = 0., CurrentTime = 0.;
+};
+
+class Simulation {
+  public:
+    explicit Simulation(const edus_sim_params& p);
+    int nb, N0, N1, Nk, nsteps, filled, print_stride;
+    double dt;
+    Material material;
+    SetOfLaser setoflaser;
+    std::vector<cplx> H0k, rk[3], U, rho0, W, Vbare, hartree;
+    std::vector<double> eps;
+    std::unique_ptr<Engine> engine;
+    DESolver desolver;
+
+    bool PrintObservables(double t) const { return int(std::round(t / dt)) % print_stride == 0; }
+    int propagate(int n, double* times, double* laser, double* popb, double* popw, double* vel);
+
+  private:
+    double jacobian() const;
+    void setup_coulomb(const edus_sim_params& p);
+};
+
+double Simulation::jacobian() const
+{
+    // Simulation.cpp:476-508: 2-D slab, perpendicular axis must be z
+    const double(*A)[3] = material.A;
+    return std::abs(A[0][0] * A[1][1] - A[1][0] * A[0][1]);
+}
+
+Simulation::Simulation(const edus_sim_params& p)
+{
+    // times in a.u. (Simulation.cpp:18-26); step count (:355)
+    const double ti = Convert(p.initialtime, unit(p.initialtime_units), AuTime);
+    const double tf = Convert(p.finaltime, unit(p.finaltime_units), AuTime);
+    dt = Convert(p.dt, unit(p.dt_units), AuTime);
+    if (!(dt > 0)) throw EngineError(EDUS_ERR_ARG, "dt must be positive (check dt_units)");
+    nsteps = int((tf - ti) / dt) + 2;
+    print_stride = p.printresolution_pulse != 0 ? p.printresolution_pulse : p.printresolution;
+    if (print_stride <= 0) throw EngineError(EDUS_ERR_ARG, "printresolution must be positive");
+    filled = p.filledbands;
+
+    material = read_material(p.tb_file);
+    nb = material.nb;
+    if (p.grid[2] != 1) throw EngineError(EDUS_ERR_UNSUPPORTED, "only 2-D k grids (grid[2] == 1) are supported");
+    int dim = (p.grid[0] > 1) + (p.grid[1] > 1);
+    if (dim != 2) throw EngineError(EDUS_ERR_UNSUPPORTED, "the velocity normalisation needs a 2-D grid");
+    N0 = p.grid[0];
+    N1 = p.grid[1];
+    Nk = N0 * N1;
+    const int M = nb * nb;
+    if (filled < 0 || filled > nb) throw EngineError(EDUS_ERR_ARG, "filledbands out of range");
+
+    // explicit DFT over the tb R list folded onto the Gamma-centred grid (Operator.hpp:331-381,
+    // fftPair.cpp:110-143), sign +1; r(k) made Hermitian (BlockMatrix_definitions.hpp:258-284)
+    std::vector<std::array<double, 3>> Rf(material.nR);
+    const int Nd[3] = {N0, N1, 1};
+    for (int i = 0; i < material.nR; ++i)
+        for (int d = 0; d < 3; ++d) Rf[i][d] = fold(material.R[i][d], -Nd[d] / 2., Nd[d] / 2., Nd[d]);
+    H0k.assign(size_t(Nk) * M, 0.);
+    for (auto& v : rk) v.assign(size_t(Nk) * M, 0.);
+    parallel_for(Nk, [&](int ik) {
+        const int i0 = ik / N1, i1 = ik % N1;
+        const double kc[2] = {double(i0) / N0, double(i1) / N1};
+        std::vector<cplx> ph(material.nR);
+        for (int iR = 0; iR < material.nR; ++iR)
+            ph[iR] = std::exp(cplx(0, 1) * 2. * kPi * (kc[0] * Rf[iR][0] + kc[1] * Rf[iR][1]));
+        for (int h = 0; h < M; ++h) {
+            cplx a = 0., b[3] = {0., 0., 0.};
+            for (int iR = 0; iR < material.nR; ++iR) {
+                a += ph[iR] * material.H[size_t(iR) * M + h];
+                for (int d = 0; d < 3; ++d) b[d] += ph[iR] * material.r[d][size_t(iR) * M + h];
+            }
+            H0k[size_t(ik) * M + h] = a;
+            for (int d = 0; d < 3; ++d) rk[d][size_t(ik) * M + h] = b[d];
+        }
+        for (int d = 0; d < 3; ++d) {
+            cplx* m = &rk[d][size_t(ik) * M];
+            for (int a = 0; a < nb; ++a)
+                for (int b2 = a; b2 < nb; ++b2) {
+                    const cplx t = (m[a * nb + b2] + std::conj(m[b2 * nb + a])) / 2.;
+                    m[a * nb + b2] = t;
+                    m[b2 * nb + a] = std::conj(t);
+                }
+        }
+    });
+
+    // lasers (Simulation.cpp:121-144)
+    for (int i = 0; i < p.nlasers; ++i) {
+        const edus_laser_params& lp = p.lasers[i];
+        Laser L;
+        L.t0 = Convert(lp.t0, unit(lp.t0_units), AuTime);
+        L.amplitude = std::sqrt(Convert(lp.intensity, unit(lp.intensity_units), AuIntensity));
+        if (std::abs(lp.frequency) > 1.e-07) {
+            L.omega = Convert(lp.frequency, unit(lp.frequency_units), AuEnergy);
+        } else {
+            if (!lp.wavelength) throw EngineError(EDUS_ERR_ARG, "You must specify (nonzero) frequency *xor* wavelength!");
+            L.omega = 2 * kPi * kSpeedOfLight / Convert(lp.wavelength, unit(lp.wavelength_units), AuLength);
+        }
+        L.duration = lp.cycles * (2. * kPi / L.omega);
+        L.phase = lp.phase;
+        double pol[3], n1 = 0.;
+        for (int d = 0; d < 3; ++d) n1 += lp.polarization[d] * lp.polarization[d];
+        n1 = std::sqrt(n1);
+        if (!(n1 > 0)) throw EngineError(EDUS_ERR_ARG, "laser polarization must be non-zero");
+        double n2 = 0.;
+        for (int d = 0; d < 3; ++d) { pol[d] = lp.polarization[d] / n1; n2 += pol[d] * pol[d]; }
+        n2 = std::sqrt(n2);
+        for (int d = 0; d < 3; ++d) L.pol[d] = pol[d] / n2;  // normalised twice (Simulation.cpp:141, Laser.cpp:199)
+        setoflaser.lasers.push_back(L);
+    }
+
+    // eigensystem (Simulation.cpp:263-286) and initial condition (Functional_InitialCondition.hpp:12-32)
+    U.assign(size_t(Nk) * M, 0.);
+    eps.assign(size_t(Nk) * nb, 0.);
+    rho0.assign(size_t(Nk) * M, 0.);
+    parallel_for(Nk, [&](int ik) {
+        hermitian_eigen(nb, &H0k[size_t(ik) * M], &eps[size_t(ik) * nb], &U[size_t(ik) * M]);
+        const cplx* u = &U[size_t(ik) * M];
+        cplx* r = &rho0[size_t(ik) * M];
+        for (int a = 0; a < nb; ++a)
+            for (int b = 0; b < nb; ++b) {
+                cplx s = 0.;
+                for (int n = 0; n < filled; ++n) s += u[a * nb + n] * std::conj(u[b * nb + n]);
+                r[a * nb + b] = s;
+            }
+    });
+
+    if (p.coulomb) setup_coulomb(p);
+
+    edus_desc d{};
+    d.nb = nb;
+    d.N0 = N0;
+    d.N1 = N1;
+    d.precision = p.precision;
+    d.layout = p.layout;
+    d.H0k = reinterpret_cast<const double*>(H0k.data());
+    for (int k = 0; k < 3; ++k) d.xik[k] = reinterpret_cast<const double*>(rk[k].data());
+    std::memcpy(d.A, material.A, sizeof(d.A));
+    d.coulomb = p.coulomb;
+    if (p.coulomb) {
+        d.W = reinterpret_cast<const double*>(W.data());
+        d.hartree = reinterpret_cast<const double*>(hartree.data());
+        d.rho0k = reinterpret_cast<const double*>(rho0.data());
+    }
+    d.U = reinterpret_cast<const double*>(U.data());
+    d.filledbands = filled;
+    EngineOptions opt;
+    opt.velocity_det = jacobian();  // velocity operator built on the device (Simulation.cpp:523-576)
+    engine = std::make_unique<Engine>(d, opt);
+    engine->upload(reinterpret_cast<const double*>(rho0.data()), engine->y);
+
+    const std::string solver = p.solver;
+    desolver.initialize(engine.get(), &setoflaser, solver == "AB" ? DESolver::AB : DESolver::RK, p.order);
+    if (solver != "AB" && solver != "RK") throw EngineError(EDUS_ERR_ARG, "solver must be RK or AB");
+    desolver.set_InitialTime(ti);
+    desolver.set_ResolutionTime(dt);
+}
+
+void Simulation::setup_coulomb(const edus_sim_params& p)
+{
+    // ModelCoulomb.cpp:89-134 and Coulomb.cpp:19-59
+    const int M = nb * nb;
+    std::vector<std::array<double, 3>> c(nb);
+    for (int n = 0; n < nb; ++n) c[n] = material.wannier_centre(n);
+    std::array<double, 3> mind = {100., 0., 0.};
+    double mindn = 100.;
+    for (int n = 0; n < nb; ++n)
+        for (int m = n + 1; m < nb; ++m) {
+            const double dx = c[n][0] - c[m][0], dy = c[n][1] - c[m][1], dz = c[n][2] - c[m][2];
+            const double dn = std::sqrt(dx * dx + dy * dy + dz * dz);
+            if (dn < mindn && dn > 1.e-05) { mind = {dx, dy, dz}; mindn = dn; }
+        }
+    double r0[3];
+    std::vector<double> r0in;
+    for (int i = 0; i < std::max(1, std::min(3, p.nr0)); ++i) r0in.push_back(Convert(p.r0[i], unit(p.r0_units), AuLength));
+    r0[0] = r0in[0];
+    if (r0in.size() == 1) { r0[1] = r0[2] = r0in[0]; }
+    else if (r0in.size() == 2) { r0[1] = r0in[1]; r0[2] = (r0in[0] + r0in[1]) / 2.; }
+    else { r0[1] = r0in[1]; r0[2] = r0in[2]; }
+    const double r0avg = (r0[0] + r0[1] + r0[2]) / 3.;
+    const double epsr = p.epsilon;
+    auto Wf = [&](const double* r) {
+        double q[3] = {r[0] / r0[0], r[1] / r0[1], r[2] / r0[2]};
+        double n = std::sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2]);
+        if (n < kThreshold) {
+            const double m[3] = {mind[0] / r0[0], mind[1] / r0[1], mind[2] / r0[2]};
+            n = std::sqrt(m[0] * m[0] + m[1] * m[1] + m[2] * m[2]);
+        }
+        return kPi / (2. * r0avg * epsr) * (struve0(n) - ::y0(n));
+    };
+    auto Vf = [&](const double* r) {
+        const double n = std::sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+        return (n < mindn) ? 2. / mindn : 2. / n;
+    };
+    W.assign(size_t(Nk) * M, 0.);
+    Vbare.assign(size_t(Nk) * M, 0.);
+    const double(*A)[3] = material.A;
+    parallel_for(Nk, [&](int iR) {
+        const int j0 = iR / N1, j1 = iR % N1;
+        const double R0 = fold(j0, -N0 / 2., N0 / 2., N0), R1 = fold(j1, -N1 / 2., N1 / 2., N1);
+        const double Rc[3] = {A[0][0] * R0 + A[0][1] * R1, A[1][0] * R0 + A[1][1] * R1, A[2][0] * R0 + A[2][1] * R1};
+        for (int n = 0; n < nb; ++n) {
+            double r[3] = {c[n][0] - Rc[0], c[n][1] - Rc[1], c[n][2] - Rc[2]};
+            for (int m = 0; m < nb; ++m) {
+                double rr[3];
+                for (int d = 0; d < 3; ++d) {
+                    if (p.coulomb_center_quirk) {
+                        r[d] -= c[m][d];  // ModelCoulomb.cpp:76-79: r accumulates over m
+                        rr[d] = r[d];
+                    } else {
+                        rr[d] = c[n][d] - Rc[d] - c[m][d];
+                    }
+                }
+                W[size_t(iR) * M + n * nb + m] = Wf(rr);
+                Vbare[size_t(iR) * M + n * nb + m] = Vf(rr);
+            }
+        }
+    });
+    hartree.assign(M, 0.);
+    for (int iR = 0; iR < Nk; ++iR)
+        for (int h = 0; h < M; ++h) hartree[h] += Vbare[size_t(iR) * M + h];
+    for (int a = 0; a < nb; ++a)
+        for (int b = a + 1; b < nb; ++b) {
+            const cplx v = (hartree[a * nb + b] + hartree[b * nb + a]) / 2.;
+            hartree[a * nb + b] = hartree[b * nb + a] = v;
+        }
+    // The engine stores Hermitian fields only.  The exchange term is Hermitian
+    // iff W_mn(R) = W_nm(-R); the reference's centre accumulation breaks this
+    // whenever more than one Wannier centre sits off the origin.
+    double asym = 0., scale = 0.;
+    for (int iR = 0; iR < Nk; ++iR) {
+        const int j0 = iR / N1, j1 = iR % N1;
+        const int mR = ((N0 - j0) % N0) * N1 + (N1 - j1) % N1;
+        for (int n = 0; n < nb; ++n)
+            for (int m = 0; m < nb; ++m) {
+                asym = std::max(asym, std::abs(W[size_t(iR) * M + m * nb + n] - W[size_t(mR) * M + n * nb + m]));
+                scale = std::max(scale, std::abs(W[size_t(iR) * M + n * nb + m]));
+            }
+    }
+    if (p.layout == EDUS_LAYOUT_HERMITIAN && asym > 1e-12 * std::max(scale, 1e-300))
+        throw EngineError(EDUS_ERR_UNSUPPORTED,
+                          "screened exchange W_mn(R) != W_nm(-R): the reference's Wannier-centre accumulation "
+                          "(coulomb_center_quirk) makes H non-Hermitian for this model; use the FULL layout "
+                          "or set coulomb_center_quirk=0");
+}
+
+int Simulation::propagate(int n, double* times, double* laser, double* popb, double* popw, double* vel)
+{
+    // Simulation::Propagate / do_onestep (Simulation.cpp:352-432): print before stepping
+    int printed = 0;
+    int it = 0;
+    while (it < n) {
+        const double t = desolver.get_CurrentTime();
+        if (PrintObservables(t)) {
+            if (times) times[printed] = t;
+            double E[3];
+            setoflaser.field(t, E);
+            if (laser)
+                for (int d = 0; d < 3; ++d) laser[3 * printed + d] = E[d];
+            engine->observables(popb ? popb + size_t(printed) * nb : nullptr, popw ? popw + size_t(printed) * nb : nullptr,
+                                vel ? vel + 6 * size_t(printed) : nullptr);
+            ++printed;
+        }
+        // advance to the next printed step in one batch
+        int batch = 1;
+        double tt = t + dt;
+        while (it + batch < n && !PrintObservables(tt)) {
+            ++batch;
+            tt += dt;
+        }
+        desolver.Propagate(batch);
+        it += batch;
+    }
+    return printed;
+}
+
+}  // namespace host
+}  // namespace edus
+
+using edus::EngineError;
+using edus::host::Simulation;
+
+struct edus_sim {
+    Simulation* s;
+    edus_ctx ctx;
+};
+
+template <class F>
+static int guarded_sim(F&& fn)
+{
+    try {
+        fn();
+        return EDUS_OK;
+    } catch (EngineError& e) {
+        edus::set_error(e.what());
+        return e.code;
+    } catch (std::exception& e) {
+        edus::set_error(e.what());
+        return EDUS_ERR_IO;
+    }
+}
+
+extern "C" {
+
+edus_sim* edus_sim_create(const edus_sim_params* p)
+{
+    if (!p) {
+        edus::set_error("null parameters");
+        return nullptr;
+    }
+    edus_sim* out = nullptr;
+    int rc = guarded_sim([&] {
+        auto* s = new Simulation(*p);
+        out = new edus_sim{s, edus_ctx{s->engine.get()}};
+    });
+    return rc == EDUS_OK ? out : nullptr;
+}
+
+int edus_sim_destroy(edus_sim* sim)
+{
+    if (!sim) return EDUS_OK;
+    int rc = guarded_sim([&] { delete sim->s; });
+    delete sim;
+    return rc;
+}
+
+int edus_sim_info(edus_sim* sim, int* nb, int* N0, int* N1, int* nsteps, double* dt)
+{
+    if (!sim) { edus::set_error("null simulation"); return EDUS_ERR_ARG; }
+    if (nb) *nb = sim->s->nb;
+    if (N0) *N0 = sim->s->N0;
+    if (N1) *N1 = sim->s->N1;
+    if (nsteps) *nsteps = sim->s->nsteps;
+    if (dt) *dt = sim->s->dt;
+    return EDUS_OK;
+}
+
+edus_ctx* edus_sim_engine(edus_sim* sim) { return sim ? &sim->ctx : nullptr; }
+
+int edus_sim_field(edus_sim* sim, double t, double E[3])
+{
+    if (!sim || !E) { edus::set_error("null argument"); return EDUS_ERR_ARG; }
+    sim->s->setoflaser.field(t, E);
+    return EDUS_OK;
+}
+
+double edus_sim_time(edus_sim* sim) { return sim ? sim->s->desolver.get_CurrentTime() : 0.0; }
+
+int edus_sim_propagate(edus_sim* sim, int nsteps, double* times, double* laser, double* pop_bloch,
+                       double* pop_wannier, double* vel, int* nprinted)
+{
+    if (!sim) { edus::set_error("null simulation"); return EDUS_ERR_ARG; }
+    return guarded_sim([&] {
+        const int n = nsteps < 0 ? sim->s->nsteps : nsteps;
+        const int k = sim->s->propagate(n, times, laser, pop_bloch, pop_wannier, vel);
+        if (nprinted) *nprinted = k;
+    });
+}
+
+int edus_sim_steps(edus_sim* sim, int nsteps)
+{
+    if (!sim) { edus::set_error("null simulation"); return EDUS_ERR_ARG; }
+    return guarded_sim([&] { sim->s->desolver.Propagate(nsteps); });
+}
+
+long edus_sim_get(edus_sim* sim, const char* name, double* out)
+{
+    if (!sim || !name) return -1;
+    Simulation& s = *sim->s;
+    const std::string n = name;
+    const std::vector<edus::host::cplx>* v = nullptr;
+    if (n == "H0k") v = &s.H0k;
+    else if (n == "rk0") v = &s.rk[0];
+    else if (n == "rk1") v = &s.rk[1];
+    else if (n == "rk2") v = &s.rk[2];
+    else if (n == "U") v = &s.U;
+    else if (n == "rho0") v = &s.rho0;
+    else if (n == "W") v = &s.W;
+    else if (n == "hartree") v = &s.hartree;
+    else if (n == "eps") {
+        if (out) std::memcpy(out, s.eps.data(), s.eps.size() * sizeof(double));
+        return long(s.eps.size());
+    } else if (n == "Vel0" || n == "Vel1" || n == "Vel2") {
+        const int d = n[3] - '0';
+        const long cnt = long(s.Nk) * s.nb * s.nb;
+        if (out) {
+            int rc = guarded_sim([&] { s.engine->download(s.engine->vel[d], out); });
+            if (rc) return -1;
+        }
+        return cnt;
+    }
+    if (!v) return -1;
+    if (out) std::memcpy(out, v->data(), v->size() * sizeof(edus::host::cplx));
+    return long(v->size());
+}
+
+}  // extern "C"

@@ -1,0 +1,331 @@
+"""ctypes binding of the C ABI in include/edus_gpu.h (libedus_gpu.so).
+
+The library is built in-tree (`make -C edus_2.0_amd`); nothing here falls back
+to another implementation: if the library or a GPU is missing the calls fail.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libedus_gpu.so")
+HEADER_PATH = os.path.join(os.path.dirname(os.path.dirname(_HERE)), "include", "edus_gpu.h")
+
+PREC_FP64 = 0
+PREC_MIXED = 1
+LAYOUT_FULL = 0
+LAYOUT_HERMITIAN = 1
+
+_LIB = None
+
+
+class EdusError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"edus error {code}: {msg}")
+        self.code = code
+
+
+class _Desc(C.Structure):
+    _fields_ = [
+        ("nb", C.c_int), ("N0", C.c_int), ("N1", C.c_int), ("precision", C.c_int), ("layout", C.c_int),
+        ("H0k", C.c_void_p), ("xik", C.c_void_p * 3),
+        ("A", (C.c_double * 3) * 3),
+        ("coulomb", C.c_int), ("W", C.c_void_p), ("hartree", C.c_void_p), ("rho0k", C.c_void_p),
+        ("U", C.c_void_p), ("vel", C.c_void_p * 3),
+        ("filledbands", C.c_int),
+    ]
+
+
+class _Laser(C.Structure):
+    _fields_ = [
+        ("intensity", C.c_double), ("intensity_units", C.c_char * 32),
+        ("frequency", C.c_double), ("frequency_units", C.c_char * 32),
+        ("wavelength", C.c_double), ("wavelength_units", C.c_char * 32),
+        ("polarization", C.c_double * 3),
+        ("cycles", C.c_double),
+        ("t0", C.c_double), ("t0_units", C.c_char * 32),
+        ("phase", C.c_double),
+    ]
+
+
+class _SimParams(C.Structure):
+    _fields_ = [
+        ("tb_file", C.c_char * 1024),
+        ("grid", C.c_int * 3),
+        ("filledbands", C.c_int),
+        ("dt", C.c_double), ("dt_units", C.c_char * 32),
+        ("initialtime", C.c_double), ("initialtime_units", C.c_char * 32),
+        ("finaltime", C.c_double), ("finaltime_units", C.c_char * 32),
+        ("printresolution", C.c_int), ("printresolution_pulse", C.c_int),
+        ("solver", C.c_char * 8), ("order", C.c_int),
+        ("coulomb", C.c_int), ("epsilon", C.c_double),
+        ("nr0", C.c_int), ("r0", C.c_double * 3), ("r0_units", C.c_char * 32),
+        ("coulomb_center_quirk", C.c_int),
+        ("nlasers", C.c_int),
+        ("lasers", _Laser * 10),
+        ("precision", C.c_int),
+        ("layout", C.c_int),
+    ]
+
+
+EXPORTS = [
+    "edus_create", "edus_destroy", "edus_last_error", "edus_set_state", "edus_get_state", "edus_rhs",
+    "edus_rk4_steps", "edus_ab_steps", "edus_ab_reset", "edus_observables", "edus_check", "edus_synchronize",
+    "edus_timing_enable", "edus_timing_get", "edus_sim_create", "edus_sim_destroy", "edus_sim_info",
+    "edus_sim_engine", "edus_sim_field", "edus_sim_time", "edus_sim_propagate", "edus_sim_steps", "edus_sim_get",
+]
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} not built: run `make -C edus_2.0_amd` (or __graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        vp, dp, ip, lp = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(C.c_long)
+        L.edus_create.restype = vp
+        L.edus_create.argtypes = [C.POINTER(_Desc)]
+        L.edus_destroy.argtypes = [vp]
+        L.edus_last_error.restype = C.c_char_p
+        for name in ("edus_set_state", "edus_get_state"):
+            getattr(L, name).argtypes = [vp, dp]
+        L.edus_rhs.argtypes = [vp, dp, dp, dp]
+        L.edus_rk4_steps.argtypes = [vp, C.c_double, C.c_int, dp]
+        L.edus_ab_steps.argtypes = [vp, C.c_double, C.c_int, C.c_int, dp]
+        L.edus_ab_reset.argtypes = [vp]
+        L.edus_observables.argtypes = [vp, dp, dp, dp]
+        L.edus_check.argtypes = [vp, dp, dp]
+        L.edus_synchronize.argtypes = [vp]
+        L.edus_timing_enable.argtypes = [vp, C.c_int]
+        L.edus_timing_get.argtypes = [vp, C.c_char_p, dp, lp]
+        L.edus_sim_create.restype = vp
+        L.edus_sim_create.argtypes = [C.POINTER(_SimParams)]
+        L.edus_sim_destroy.argtypes = [vp]
+        L.edus_sim_info.argtypes = [vp, ip, ip, ip, ip, dp]
+        L.edus_sim_engine.restype = vp
+        L.edus_sim_engine.argtypes = [vp]
+        L.edus_sim_field.argtypes = [vp, C.c_double, dp]
+        L.edus_sim_time.restype = C.c_double
+        L.edus_sim_time.argtypes = [vp]
+        L.edus_sim_propagate.argtypes = [vp, C.c_int, dp, dp, dp, dp, dp, ip]
+        L.edus_sim_steps.argtypes = [vp, C.c_int]
+        L.edus_sim_get.restype = C.c_long
+        L.edus_sim_get.argtypes = [vp, C.c_char_p, dp]
+        _LIB = L
+    return _LIB
+
+
+def _check(rc: int):
+    if rc != 0:
+        raise EdusError(rc, lib().edus_last_error().decode())
+
+
+def _dp(a: np.ndarray | None):
+    if a is None:
+        return None
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _c128(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.complex128)
+
+
+class Engine:
+    """Low-level engine context (edus_create) built from set-up arrays."""
+
+    def __init__(self, nb, N0, N1, H0k, xik=(None, None, None), A=None, filledbands=0, U=None,
+                 vel=(None, None, None), coulomb=False, W=None, hartree=None, rho0k=None,
+                 precision=PREC_FP64, layout=LAYOUT_FULL, _handle=None, _owner=None):
+        self.nb, self.N0, self.N1 = nb, N0, N1
+        self.Nk = N0 * N1
+        self._owner = _owner
+        if _handle is not None:
+            self.h = _handle
+            self._owned = False
+            return
+        self._keep = []
+        d = _Desc()
+        d.nb, d.N0, d.N1, d.precision, d.layout = nb, N0, N1, precision, layout
+
+        def ptr(x):
+            if x is None:
+                return None
+            a = _c128(x)
+            self._keep.append(a)
+            return a.ctypes.data
+
+        d.H0k = ptr(H0k)
+        for i in range(3):
+            d.xik[i] = ptr(xik[i])
+            d.vel[i] = ptr(vel[i])
+        A = np.eye(3) if A is None else np.asarray(A, dtype=float)
+        for i in range(3):
+            for j in range(3):
+                d.A[i][j] = A[i, j]
+        d.coulomb = int(bool(coulomb))
+        d.W, d.hartree, d.rho0k, d.U = ptr(W), ptr(hartree), ptr(rho0k), ptr(U)
+        d.filledbands = int(filledbands)
+        self.h = lib().edus_create(C.byref(d))
+        if not self.h:
+            raise EdusError(-1, lib().edus_last_error().decode())
+        self._owned = True
+
+    def close(self):
+        if getattr(self, "_owned", False) and self.h:
+            lib().edus_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def _shape(self):
+        return (self.Nk, self.nb, self.nb)
+
+    def set_state(self, rho):
+        a = _c128(rho).reshape(self._shape())
+        _check(lib().edus_set_state(self.h, _dp(a.view(np.float64))))
+
+    def get_state(self) -> np.ndarray:
+        out = np.empty(self._shape(), dtype=np.complex128)
+        _check(lib().edus_get_state(self.h, _dp(out.view(np.float64))))
+        return out
+
+    def rhs(self, E, rho) -> np.ndarray:
+        a = _c128(rho).reshape(self._shape())
+        e = np.ascontiguousarray(E, dtype=np.float64)
+        out = np.empty_like(a)
+        _check(lib().edus_rhs(self.h, _dp(e), _dp(a.view(np.float64)), _dp(out.view(np.float64))))
+        return out
+
+    def rk4_steps(self, dt: float, E) -> None:
+        e = np.ascontiguousarray(E, dtype=np.float64).reshape(-1, 3, 3)
+        _check(lib().edus_rk4_steps(self.h, float(dt), e.shape[0], _dp(e)))
+
+    def ab_steps(self, dt: float, order: int, E) -> None:
+        e = np.ascontiguousarray(E, dtype=np.float64).reshape(-1, 3)
+        _check(lib().edus_ab_steps(self.h, float(dt), int(order), e.shape[0], _dp(e)))
+
+    def ab_reset(self):
+        _check(lib().edus_ab_reset(self.h))
+
+    def observables(self):
+        pb, pw, v = np.empty(self.nb), np.empty(self.nb), np.empty(6)
+        _check(lib().edus_observables(self.h, _dp(pb), _dp(pw), _dp(v)))
+        return pb, pw, v
+
+    def check(self):
+        t, h = C.c_double(), C.c_double()
+        _check(lib().edus_check(self.h, C.byref(t), C.byref(h)))
+        return t.value, h.value
+
+    def synchronize(self):
+        _check(lib().edus_synchronize(self.h))
+
+    def timing_enable(self, on: bool = True):
+        _check(lib().edus_timing_enable(self.h, int(bool(on))))
+
+    def timing_get(self, cls: str):
+        ms, n = C.c_double(), C.c_long()
+        _check(lib().edus_timing_get(self.h, cls.encode(), C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+
+def _sim_params(p: dict, precision: int, layout: int) -> _SimParams:
+    s = _SimParams()
+    s.tb_file = p["tb_file"].encode()
+    for i in range(3):
+        s.grid[i] = int(p["grid"][i])
+    s.filledbands = int(p["filledbands"])
+    for key in ("dt", "initialtime", "finaltime"):
+        setattr(s, key, float(p[key]))
+        setattr(s, key + "_units", p[key + "_units"].encode())
+    s.printresolution = int(p["printresolution"])
+    s.printresolution_pulse = int(p["printresolution_pulse"])
+    s.solver = p["solver"].encode()
+    s.order = int(p["order"])
+    s.coulomb = int(bool(p["coulomb"]))
+    s.epsilon = float(p["epsilon"])
+    r0 = list(p["r0"])
+    s.nr0 = len(r0)
+    for i, v in enumerate(r0[:3]):
+        s.r0[i] = float(v)
+    s.r0_units = p["r0_units"].encode()
+    s.coulomb_center_quirk = int(bool(p.get("coulomb_center_quirk", True)))
+    s.nlasers = len(p["lasers"])
+    for i, las in enumerate(p["lasers"]):
+        L = s.lasers[i]
+        for key in ("intensity", "frequency", "wavelength", "t0"):
+            setattr(L, key, float(las[key]))
+            setattr(L, key + "_units", las[key + "_units"].encode())
+        for d in range(3):
+            L.polarization[d] = float(las["polarization"][d])
+        L.cycles = float(las["cycles"])
+        L.phase = float(las["phase"])
+    s.precision = int(precision)
+    s.layout = int(layout)
+    return s
+
+
+class Simulation:
+    """Reference-format simulation (edus_sim_*): host set-up + engine."""
+
+    def __init__(self, params: dict, precision: int = PREC_FP64, layout: int = LAYOUT_FULL):
+        self._p = _sim_params(params, precision, layout)
+        self.h = lib().edus_sim_create(C.byref(self._p))
+        if not self.h:
+            raise EdusError(-1, lib().edus_last_error().decode())
+        nb, n0, n1, ns = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        dt = C.c_double()
+        _check(lib().edus_sim_info(self.h, C.byref(nb), C.byref(n0), C.byref(n1), C.byref(ns), C.byref(dt)))
+        self.nb, self.N0, self.N1, self.nsteps, self.dt = nb.value, n0.value, n1.value, ns.value, dt.value
+        self.Nk = self.N0 * self.N1
+        self.engine = Engine(self.nb, self.N0, self.N1, None, _handle=lib().edus_sim_engine(self.h), _owner=self)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().edus_sim_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    @property
+    def time(self) -> float:
+        return lib().edus_sim_time(self.h)
+
+    def field(self, t: float) -> np.ndarray:
+        out = np.empty(3)
+        _check(lib().edus_sim_field(self.h, float(t), _dp(out)))
+        return out
+
+    def get(self, name: str) -> np.ndarray:
+        n = lib().edus_sim_get(self.h, name.encode(), None)
+        if n < 0:
+            raise KeyError(name)
+        if name == "eps":
+            out = np.empty(n)
+            lib().edus_sim_get(self.h, name.encode(), _dp(out))
+            return out.reshape(self.Nk, self.nb)
+        out = np.empty(n, dtype=np.complex128)
+        if lib().edus_sim_get(self.h, name.encode(), _dp(out.view(np.float64))) < 0:
+            raise EdusError(-1, lib().edus_last_error().decode())
+        if name == "hartree":
+            return out.reshape(self.nb, self.nb)
+        return out.reshape(-1, self.nb, self.nb)
+
+    def steps(self, n: int):
+        _check(lib().edus_sim_steps(self.h, int(n)))
+
+    def propagate(self, nsteps: int | None = None) -> dict:
+        n = self.nsteps if nsteps is None else int(nsteps)
+        times, laser = np.empty(n), np.empty((n, 3))
+        pb, pw, vel = np.empty((n, self.nb)), np.empty((n, self.nb)), np.empty((n, 6))
+        k = C.c_int()
+        _check(lib().edus_sim_propagate(self.h, n, _dp(times), _dp(laser), _dp(pb), _dp(pw), _dp(vel),
+                                        C.byref(k)))
+        k = k.value
+        return {"time": times[:k], "laser": laser[:k], "pop_bloch": pb[:k], "pop_wannier": pw[:k],
+                "velocity": vel[:k]}

@@ -1,0 +1,185 @@
+/*
+ * edus_gpu.h -- C ABI of the MI355X EDUS propagation engine.
+ *
+ * The engine advances the one-body density matrix rho(k) of the reference
+ * (gcistaro/EDUS_2.0) under
+ *     d rho(k)/dt = -i [H0(k) + E(t).xi(k) + H_ee(k), rho(k)] + E(t).grad_k rho(k)
+ * on one GPU, for the whole k grid.  It replaces the reference's hot path:
+ *   - DESolver<Operator<complex<double>>>::Propagate_RK / Propagate_AB
+ *     (src/DESolver/DESolver.hpp:155-220) and its SumWithProduct sweeps
+ *     (DESolver.hpp:10-20),
+ *   - the SourceTerm right-hand side (src/Simulation/Functional_SourceTerm.hpp:11-57)
+ *     with everything it calls: Operator::go_to_R/go_to_k
+ *     (src/Operator/Operator.hpp:566-606), FourierTransform::fft
+ *     (src/fftPair/fftPair.cpp:89-107), kGradient::Calculate
+ *     (src/kGradient/kGradient_definitions.hpp:7-41), Calculate_TDHamiltonian
+ *     (src/Simulation/Simulation.cpp:296-349), Coulomb::EffectiveHamiltonian
+ *     (src/Coulomb/Coulomb.cpp:134-180) and commutator/multiply
+ *     (src/Operator/BlockMatrix_definitions.hpp:149-227),
+ *   - the observable reductions Print_Population / Print_Velocity / TraceK
+ *     (src/Simulation/Simulation.cpp:439-470,583-622; Simulation.hpp:125-142).
+ *
+ * Conventions at this boundary
+ *   - Operator fields are passed in the reference's BlockMatrix layout
+ *     [Nk][nb][nb], row-major, complex<double> as interleaved (re, im)
+ *     doubles (src/Operator/BlockMatrix.hpp:48), k index i1 + N1*i0
+ *     (src/MeshGrid/MeshGrid.cpp:117-127).  Pointers are host pointers.
+ *   - All quantities in Hartree atomic units.
+ *   - Every function returns 0 on success or a non-zero EDUS_ERR_* code;
+ *     edus_last_error() gives a message (thread-local).  No C++ exception
+ *     crosses this boundary.
+ *   - One host thread drives a context; calls are stream-ordered on the
+ *     context's own HIP stream; contexts are independent.
+ */
+#ifndef EDUS_GPU_H
+#define EDUS_GPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EDUS_OK 0
+#define EDUS_ERR_ARG 1
+#define EDUS_ERR_HIP 2
+#define EDUS_ERR_STATE 3
+#define EDUS_ERR_UNSUPPORTED 4
+#define EDUS_ERR_MONITOR 5
+#define EDUS_ERR_IO 6
+
+#define EDUS_PREC_FP64 0  /* everything FP64 (parity path) */
+#define EDUS_PREC_MIXED 1 /* FP32 MFMA commutator, FP64 state and updates */
+
+/* Storage of operator fields on the device.
+ * FULL: all nb*nb elements per k, the reference's arithmetic element for
+ *   element (it propagates every element, BlockMatrix.hpp:48).  On even grids
+ *   the reference's Gamma-centred Nyquist row (R = -N/2 has no +N/2 partner,
+ *   MeshGrid.cpp:458-486) gives i(R.E)rho(R) a small anti-Hermitian part;
+ *   FULL reproduces it.
+ * HERMITIAN: upper triangle only (nb(nb+1)/2 planes), the lower triangle being
+ *   the conjugate -- 1.9x less memory traffic at nb = 14.  Identical to FULL
+ *   on odd grids; on even grids it drops the reference's anti-Hermitian
+ *   Nyquist component (edus_check reports its size in FULL runs). */
+#define EDUS_LAYOUT_FULL 0
+#define EDUS_LAYOUT_HERMITIAN 1
+
+typedef struct edus_ctx edus_ctx;
+typedef struct edus_sim edus_sim;
+
+/* Everything the propagation needs, set up on the host in FP64.
+ * Replaces the state the reference's Simulation constructor builds
+ * (src/Simulation/Simulation.cpp:10-233). */
+typedef struct {
+    int nb;                /* bands (Wannier functions), 1..16 */
+    int N0, N1;            /* k grid; the third dimension must be 1 */
+    int precision;         /* EDUS_PREC_* */
+    int layout;            /* EDUS_LAYOUT_* */
+    const double* H0k;     /* H0(k)   [Nk][nb][nb] complex (Hermitian) */
+    const double* xik[3];  /* r_d(k)  [Nk][nb][nb] complex, NULL = zero */
+    double A[3][3];        /* A[i][j] = Cartesian component i of lattice vector j (bohr) */
+    int coulomb;           /* 1: Hartree + screened exchange (HSEX) on */
+    const double* W;       /* screened potential W(R) on the FFT R grid [Nk][nb][nb] complex */
+    const double* hartree; /* nb x nb complex: sum_R V(R), symmetrised */
+    const double* rho0k;   /* rho(t=0) in k [Nk][nb][nb] complex: HSEX reference state (Coulomb::set_DM0) */
+    const double* U;       /* eigenvectors per k, columns [Nk][nb][nb] complex; NULL = no Bloch populations */
+    const double* vel[3];  /* velocity operator V_d(k) [Nk][nb][nb]; NULL = no current */
+    int filledbands;
+} edus_desc;
+
+/* ---- context ----------------------------------------------------------- */
+edus_ctx* edus_create(const edus_desc* desc);
+int edus_destroy(edus_ctx* ctx);
+const char* edus_last_error(void);
+
+/* State transfer (Operator::get_Operator_k() data of the reference). */
+int edus_set_state(edus_ctx* ctx, const double* rho_k);
+int edus_get_state(edus_ctx* ctx, double* rho_k);
+
+/* One right-hand-side evaluation f = SourceTerm(t, rho) with the field E
+ * given (the reference evaluates SetOfLaser at t).  in/out host arrays. */
+int edus_rhs(edus_ctx* ctx, const double E[3], const double* in, double* out);
+
+/* nsteps RK4 steps of size dt on the device-resident state
+ * (DESolver.hpp:155-187).  E[(s*3 + j)*3 + d]: field component d at
+ * t_s, t_s + dt/2, t_s + dt for j = 0, 1, 2. */
+int edus_rk4_steps(edus_ctx* ctx, double dt, int nsteps, const double* E);
+
+/* nsteps Adams-Bashforth steps of order 4 or 5 (DESolver.hpp:202-220), history
+ * starting at zero as in the reference.  E[s*3 + d] at t_s. */
+int edus_ab_steps(edus_ctx* ctx, double dt, int order, int nsteps, const double* E);
+/* Clear the Adams-Bashforth history (zero, as DESolver.cpp:26-29). */
+int edus_ab_reset(edus_ctx* ctx);
+
+/* Observables of the current state (Simulation.cpp:439-470, 583-622):
+ * pop_bloch[nb] (filled bands as 1 - P_n), pop_wannier[nb], vel[6] =
+ * (Re vx, Im vx, Re vy, Im vy, Re vz, Im vz).  Any pointer may be NULL. */
+int edus_observables(edus_ctx* ctx, double* pop_bloch, double* pop_wannier, double* vel);
+
+/* Invariant monitors accumulated over the steps since the last call (and
+ * reset by it): max over steps and k of |Tr rho(k) - filledbands| and of the
+ * Hermiticity defect max_{n<=m} |rho_nm - conj(rho_mn)| (HERMITIAN layout:
+ * twice the largest |Im rho_nn|, the only part its storage leaves free). */
+int edus_check(edus_ctx* ctx, double* trace_dev, double* herm_dev);
+
+int edus_synchronize(edus_ctx* ctx);
+
+/* Kernel timing with HIP events on the context stream.  When enabled, every
+ * launch of the named kernel class is bracketed by events; edus_timing_get
+ * returns the summed milliseconds and launch count since the last reset.
+ * Classes: "grad_col", "grad_row", "comm", "hf_row", "hf_col", "monitor". */
+int edus_timing_enable(edus_ctx* ctx, int on);
+int edus_timing_get(edus_ctx* ctx, const char* kernel_class, double* total_ms, long* count);
+
+/* ---- reference-format simulation (host set-up mirrored in C++) --------- */
+typedef struct {
+    double intensity; char intensity_units[32];
+    double frequency; char frequency_units[32];
+    double wavelength; char wavelength_units[32];
+    double polarization[3];
+    double cycles;
+    double t0; char t0_units[32];
+    double phase;
+} edus_laser_params;
+
+/* Keys of the reference's JSON input (src/InputVariables/input_schema.json);
+ * unit strings are resolved with the reference's unit() (src/ConvertUnits.cpp:22-37). */
+typedef struct {
+    char tb_file[1024];     /* seedname; "_tb.dat" is appended */
+    int grid[3];
+    int filledbands;
+    double dt; char dt_units[32];
+    double initialtime; char initialtime_units[32];
+    double finaltime; char finaltime_units[32];
+    int printresolution, printresolution_pulse;
+    char solver[8]; int order;
+    int coulomb; double epsilon;
+    int nr0; double r0[3]; char r0_units[32];
+    int coulomb_center_quirk; /* 1: keep ModelCoulomb.cpp:76-79 accumulation */
+    int nlasers;
+    edus_laser_params lasers[10];
+    int precision;            /* EDUS_PREC_* */
+    int layout;               /* EDUS_LAYOUT_* */
+} edus_sim_params;
+
+edus_sim* edus_sim_create(const edus_sim_params* p);
+int edus_sim_destroy(edus_sim* sim);
+int edus_sim_info(edus_sim* sim, int* nb, int* N0, int* N1, int* nsteps, double* dt);
+edus_ctx* edus_sim_engine(edus_sim* sim);
+/* Field of the set of lasers at t (SetOfLaser::operator(), Laser.hpp:128-170). */
+int edus_sim_field(edus_sim* sim, double t, double E[3]);
+double edus_sim_time(edus_sim* sim);
+/* Simulation::Propagate (Simulation.cpp:352-432): nsteps steps (-1: the
+ * reference's int((tf-t0)/dt)+2), observables recorded before each step on
+ * the print stride.  Output arrays hold one row per printed step:
+ * times[n], laser[n][3], pop_bloch[n][nb], pop_wannier[n][nb], vel[n][6]. */
+int edus_sim_propagate(edus_sim* sim, int nsteps, double* times, double* laser, double* pop_bloch,
+                       double* pop_wannier, double* vel, int* nprinted);
+/* Advance nsteps without observables (benchmark path). */
+int edus_sim_steps(edus_sim* sim, int nsteps);
+/* Set-up arrays for inspection: "H0k", "rk0", "rk1", "rk2", "U", "Vel0",
+ * "Vel1", "Vel2", "W", "rho0R", "hartree" ([nb][nb]), "eps" (real [Nk][nb]). */
+long edus_sim_get(edus_sim* sim, const char* name, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* EDUS_GPU_H */

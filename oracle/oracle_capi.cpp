@@ -1,0 +1,215 @@
+// C entry points of the FP64 CPU oracle -- TEST INFRASTRUCTURE ONLY.
+// Loaded by tests/ (ctypes), by the smoke check and by bench.py's CPU-baseline
+// leg; never by the product library.
+#include <cstring>
+#include <exception>
+#include <string>
+
+#include "oracle.hpp"
+
+using namespace oracle;
+
+extern "C" {
+
+// Same field meaning as the reference JSON keys (input_schema.json); unit
+// strings are passed through and resolved with the reference's unit() rules.
+struct oracle_laser {
+    double intensity; char intensity_units[32];
+    double frequency; char frequency_units[32];
+    double wavelength; char wavelength_units[32];
+    double polarization[3];
+    double cycles;
+    double t0; char t0_units[32];
+    double phase;
+};
+struct oracle_params {
+    char tb_file[1024];
+    int grid[3];
+    int filledbands;
+    double dt; char dt_units[32];
+    double initialtime; char initialtime_units[32];
+    double finaltime; char finaltime_units[32];
+    int printresolution, printresolution_pulse;
+    char solver[8]; int order;
+    int coulomb; double epsilon;
+    int nr0; double r0[3]; char r0_units[32];
+    int coulomb_center_quirk;
+    int nlasers;
+    oracle_laser lasers[10];
+};
+
+static thread_local std::string g_err;
+const char* oracle_last_error() { return g_err.c_str(); }
+
+void* oracle_create(const oracle_params* p)
+{
+    try {
+        SimParams s;
+        s.tb_file = p->tb_file;
+        for (int d = 0; d < 3; ++d) s.grid[d] = p->grid[d];
+        s.filledbands = p->filledbands;
+        s.dt = p->dt; s.dt_units = p->dt_units;
+        s.initialtime = p->initialtime; s.initialtime_units = p->initialtime_units;
+        s.finaltime = p->finaltime; s.finaltime_units = p->finaltime_units;
+        s.printresolution = p->printresolution; s.printresolution_pulse = p->printresolution_pulse;
+        s.solver = p->solver; s.order = p->order;
+        s.coulomb = p->coulomb != 0; s.epsilon = p->epsilon;
+        s.r0.assign(p->r0, p->r0 + (p->nr0 > 0 ? p->nr0 : 1));
+        s.r0_units = p->r0_units;
+        s.coulomb_center_quirk = p->coulomb_center_quirk != 0;
+        for (int i = 0; i < p->nlasers; ++i) {
+            const oracle_laser& l = p->lasers[i];
+            LaserParams lp;
+            lp.intensity = l.intensity; lp.intensity_units = l.intensity_units;
+            lp.frequency = l.frequency; lp.frequency_units = l.frequency_units;
+            lp.wavelength = l.wavelength; lp.wavelength_units = l.wavelength_units;
+            for (int d = 0; d < 3; ++d) lp.polarization[d] = l.polarization[d];
+            lp.cycles = l.cycles; lp.t0 = l.t0; lp.t0_units = l.t0_units; lp.phase = l.phase;
+            s.lasers.push_back(lp);
+        }
+        return new Simulation(s);
+    } catch (std::exception& e) {
+        g_err = e.what();
+        return nullptr;
+    }
+}
+
+void oracle_destroy(void* h) { delete static_cast<Simulation*>(h); }
+
+void oracle_dims(void* h, int* nb, int* n0, int* n1, int* nsteps, double* dt)
+{
+    auto* s = static_cast<Simulation*>(h);
+    *nb = s->nb; *n0 = s->N[0]; *n1 = s->N[1]; *nsteps = s->nsteps; *dt = s->dt;
+}
+
+double oracle_time(void* h) { return static_cast<Simulation*>(h)->t_current; }
+
+static std::vector<cplx>* named(Simulation* s, const char* name)
+{
+    std::string n = name;
+    if (n == "rho") return &s->rho;
+    if (n == "H0k") return &s->H0k;
+    if (n == "H0R") return &s->H0R;
+    if (n == "rk0") return &s->rk[0];
+    if (n == "rk1") return &s->rk[1];
+    if (n == "rk2") return &s->rk[2];
+    if (n == "U") return &s->U;
+    if (n == "Vel0") return &s->Vel[0];
+    if (n == "Vel1") return &s->Vel[1];
+    if (n == "Vel2") return &s->Vel[2];
+    if (n == "rho0R") return &s->rho0R;
+    if (n == "Wscr") return &s->Wscr;
+    if (n == "Vbare") return &s->Vbare;
+    if (n == "hartree") return &s->hartree;
+    return nullptr;
+}
+
+// copy a named complex array out ([k][n][m] interleaved re/im); returns element count
+long oracle_get(void* h, const char* name, double* out)
+{
+    auto* v = named(static_cast<Simulation*>(h), name);
+    if (!v) return -1;
+    if (out) std::memcpy(out, v->data(), v->size() * sizeof(cplx));
+    return long(v->size());
+}
+
+int oracle_set(void* h, const char* name, const double* in)
+{
+    auto* v = named(static_cast<Simulation*>(h), name);
+    if (!v) return -1;
+    std::memcpy(v->data(), in, v->size() * sizeof(cplx));
+    return 0;
+}
+
+void oracle_eigenvalues(void* h, double* out)
+{
+    auto* s = static_cast<Simulation*>(h);
+    std::memcpy(out, s->eps.data(), s->eps.size() * sizeof(double));
+}
+
+void oracle_rhs(void* h, double t, const double* in, double* out)
+{
+    auto* s = static_cast<Simulation*>(h);
+    std::vector<cplx> vin(size_t(s->Nk) * s->M), vout;
+    std::memcpy(vin.data(), in, vin.size() * sizeof(cplx));
+    s->rhs(vout, t, vin);
+    std::memcpy(out, vout.data(), vout.size() * sizeof(cplx));
+}
+
+void oracle_rhs_E(void* h, const double* E, const double* in, double* out)
+{
+    auto* s = static_cast<Simulation*>(h);
+    std::vector<cplx> vin(size_t(s->Nk) * s->M), vout;
+    std::memcpy(vin.data(), in, vin.size() * sizeof(cplx));
+    s->rhs_E(vout, E, vin);
+    std::memcpy(out, vout.data(), vout.size() * sizeof(cplx));
+}
+
+void oracle_lattice(void* h, double* A)
+{
+    auto* s = static_cast<Simulation*>(h);
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) A[3 * i + j] = s->A[i][j];
+}
+
+void oracle_step(void* h, int n)
+{
+    auto* s = static_cast<Simulation*>(h);
+    for (int i = 0; i < n; ++i) s->step();
+}
+
+void oracle_E(void* h, double t, double* E) { static_cast<Simulation*>(h)->E_of_t(t, E); }
+
+void oracle_observables(void* h, double* popb, double* popw, double* vel6)
+{
+    auto* s = static_cast<Simulation*>(h);
+    std::vector<double> pb, pw;
+    s->populations(pb, pw);
+    for (int n = 0; n < s->nb; ++n) { popb[n] = pb[n]; popw[n] = pw[n]; }
+    s->velocity(vel6);
+}
+
+void oracle_rhoR(void* h, double* out)
+{
+    auto* s = static_cast<Simulation*>(h);
+    std::vector<cplx> r;
+    s->rho_R(r);
+    std::memcpy(out, r.data(), r.size() * sizeof(cplx));
+}
+
+// Full propagation with observables printed before each step as
+// Simulation::do_onestep does.  Arrays are sized by the caller for nsteps rows.
+int oracle_propagate(void* h, int nsteps, double* times, double* laser, double* popb, double* popw, double* vel)
+{
+    auto* s = static_cast<Simulation*>(h);
+    Observables o;
+    s->propagate(&o, nsteps);
+    int n = int(o.time.size());
+    for (int i = 0; i < n; ++i) {
+        times[i] = o.time[i];
+        for (int d = 0; d < 3; ++d) laser[3 * i + d] = o.laser[i][d];
+        for (int b = 0; b < s->nb; ++b) {
+            popb[i * s->nb + b] = o.pop_bloch[i][b];
+            popw[i * s->nb + b] = o.pop_wannier[i][b];
+        }
+        for (int c = 0; c < 6; ++c) vel[6 * i + c] = o.velocity[i][c];
+    }
+    return n;
+}
+
+// stand-alone building blocks for the closed-form tests
+void oracle_eigh(int n, const double* A, double* w, double* V)
+{
+    eigh(n, reinterpret_cast<const cplx*>(A), w, reinterpret_cast<cplx*>(V));
+}
+
+void oracle_fft2d(int n0, int n1, double* data, int sign)
+{
+    auto p0 = make_plan(n0), p1 = make_plan(n1);
+    std::vector<cplx> scratch(2 * (n0 > n1 ? n0 : n1) + 2);
+    fft2d(p0, p1, reinterpret_cast<cplx*>(data), sign, scratch.data());
+}
+
+double oracle_struve0(double x) { return struve0(x); }
+
+}  // extern "C"
