@@ -145,6 +145,50 @@ __global__ void __launch_bounds__(256) comm_kernel(CommArgs a)
             u.y[og] = yv;
         }
     }
+
+    if (u.mon) {
+        // The block's new state is in global memory (visible to the block after
+        // the barrier, mostly still in cache): check trace and Hermiticity of
+        // its TK k-points without another pass over HBM.
+        __shared__ double sred[2 * TK];
+        __syncthreads();
+        const double2* ys = u.mon_src;
+        for (int kk = threadIdx.x; kk < TK; kk += blockDim.x) {
+            const int k = k0 + kk;
+            double mt = 0., mh = 0.;
+            if (k < Nk) {
+                double2 tr = make_double2(-u.nf, 0.);
+                for (int n = 0; n < NB; ++n) {
+                    bool c;
+                    const double2 v = ys[long(slot<NB, LAYOUT>(n, n, c)) * Nk + k];
+                    tr = cadd(tr, v);
+                    mh = fmax(mh, 2.0 * fabs(v.y));
+                }
+                if (LAYOUT == kFull)
+                    for (int n = 0; n < NB; ++n)
+                        for (int m = n + 1; m < NB; ++m) {
+                            const double2 x = ys[long(n * NB + m) * Nk + k];
+                            const double2 z = ys[long(m * NB + n) * Nk + k];
+                            mh = fmax(mh, sqrt((x.x - z.x) * (x.x - z.x) + (x.y + z.y) * (x.y + z.y)));
+                        }
+                mt = sqrt(tr.x * tr.x + tr.y * tr.y);
+                if (mt != mt) mt = __builtin_huge_val();  // NaN trips the monitor
+                if (mh != mh) mh = __builtin_huge_val();
+            }
+            sred[kk] = mt;
+            sred[TK + kk] = mh;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double mt = 0., mh = 0.;
+            for (int kk = 0; kk < TK; ++kk) {
+                mt = fmax(mt, sred[kk]);
+                mh = fmax(mh, sred[TK + kk]);
+            }
+            atomicMax(&u.mon[0], static_cast<unsigned long long>(__double_as_longlong(mt)));
+            atomicMax(&u.mon[1], static_cast<unsigned long long>(__double_as_longlong(mh)));
+        }
+    }
 }
 
 template <int NB>

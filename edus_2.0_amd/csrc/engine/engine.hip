@@ -384,10 +384,11 @@ void Engine::rk4_step(double dt, const double* E3x3)
     u.x2 = y; u.out2 = aux; u.b2 = dt;
     evaluate(aux, E3x3 + 3, u);
     // k4 = f(t+dt, aux); y = acc + dt/6 k4                       (:184-185)
+    // with the per-step trace / Hermiticity monitor fused into the same kernel
     u = StageUpdate{};
     u.x1 = acc; u.out1 = y; u.b1 = dt / 6.;
+    u.mon = mon; u.mon_src = y; u.nf = double(nf);
     evaluate(aux, E3x3 + 6, u);
-    monitor(y);
 }
 
 void Engine::ab_step(double dt, int order, const double* E3)
@@ -404,11 +405,11 @@ void Engine::ab_step(double dt, int order, const double* E3)
         u.ab_c[i] = dt * beta[i];
         u.hist[i] = hist[abidx[i]];
     }
+    u.mon = mon; u.mon_src = y; u.nf = double(nf);
     evaluate(y, E3, u);
     const int tmp = abidx[order - 1];
     for (int i = order - 1; i > 0; --i) abidx[i] = abidx[i - 1];
     abidx[0] = tmp;
-    monitor(y);
 }
 
 }  // namespace edus

@@ -41,6 +41,11 @@ struct StageUpdate {
     const double2* hist[5];  // hist[0] unused (= f)
     double ab_c[5];
     double2* y;               // AB target
+    // invariant monitor on the array this stage wrote (mon_src): max over the
+    // tile of |Tr - nf| into mon[0], Hermiticity defect into mon[1]
+    unsigned long long* mon;
+    const double2* mon_src;
+    double nf;
 };
 
 struct CommArgs {

@@ -44,7 +44,8 @@ def random_hermitian(nk, nb, seed, hermitian=True):
 
 
 def _model(name, model_dir, grid, **kw):
-    return models.workload(name, os.path.join(model_dir, f"{name}_{grid[0]}x{grid[1]}"), grid=grid, **kw)
+    g = [int(grid[0]), int(grid[1]), 1]
+    return models.workload(name, os.path.join(model_dir, f"{name}_{g[0]}x{g[1]}"), grid=g, **kw)
 
 
 RHS_CASES = [
@@ -69,6 +70,24 @@ def test_rhs_matches_oracle(model_dir, name, grid, layout):
         scale = max(1.0, np.abs(f_ref).max())
         _say("  E", E, "max|df|", err, "scale", scale)
         assert err <= 1e-11 * scale
+
+
+@pytest.mark.parametrize("layout", [FULL, HERM])
+def test_rhs_hbn_coulomb_from_oracle_arrays(layout):
+    """hBN with Hartree + screened exchange: engine kernels fed with the
+    oracle's own set-up arrays (isolates the kernels from the host set-up)."""
+    p = hbn_params(coulomb=True, grid=[20, 20, 1])
+    o = oracle_py.Oracle(p)
+    eng = engine_from_oracle(o, layout, 1, coulomb=True, rho0=o.rho)
+    rng = np.random.default_rng(7)
+    for E in ([0.0, 0.0, 0.0], [1e-3, 2e-3, 0.0]):
+        d = 0.01 * (rng.standard_normal((o.Nk, 2, 2)) + 1j * rng.standard_normal((o.Nk, 2, 2)))
+        rho = o.rho + 0.5 * (d + d.conj().transpose(0, 2, 1))
+        f_ref = o.rhs_E(E, rho)
+        f = eng.rhs(E, rho)
+        err = np.abs(f - f_ref).max()
+        _say("E", E, "max|df|", err, "scale", np.abs(f_ref).max())
+        assert err <= 1e-12
 
 
 def _compare_runs(o, sim, nsteps):
