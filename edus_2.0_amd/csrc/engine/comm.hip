@@ -65,26 +65,38 @@ template <int NB, int TK, int LAYOUT>
 __global__ void __launch_bounds__(256) comm_kernel(CommArgs a)
 {
     constexpr int NPL = LAYOUT == kFull ? NB * NB : NB * (NB + 1) / 2;
+    constexpr int NT = 256;                        // launch block size
+    constexpr int PER = (NPL * TK + NT - 1) / NT;  // elements per thread
     __shared__ double2 sR[NPL * TK];
     __shared__ double2 sH[NPL * TK];
     const int k0 = blockIdx.x * TK;
     const int Nk = a.Nk;
+    const StageUpdate& u = a.up;
 
-    for (int idx = threadIdx.x; idx < NPL * TK; idx += blockDim.x) {
+    // Load phase.  Element (p, kk) is loaded and later produced by the same
+    // thread, so the operands of its update (D and the stage arrays) are
+    // fetched here too: every global load of the tile is in flight before the
+    // first barrier.
+    const bool x1_in = (u.x1 == a.rho), x2_in = (u.x2 == a.rho), y_in = (u.y == a.rho);
+    double2 dv[PER], x1v[PER], x2v[PER];
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+        const int idx = threadIdx.x + j * NT;
+        if (idx >= NPL * TK) break;
         const int p = idx / TK;
         const int kk = idx - p * TK;
         const int k = k0 + kk;
-        double2 r = make_double2(0., 0.), h = make_double2(0., 0.);
+        double2 r = make_double2(0., 0.), h = r, d = r, x1 = r, x2 = r;
         if (k < Nk) {
             const long o = long(p) * Nk + k;
             r = a.rho[o];
             h = a.H0[o];
 #pragma unroll
-            for (int d = 0; d < 3; ++d)
-                if (a.xi[d]) {
-                    const double2 x = a.xi[d][o];
-                    h.x += a.E[d] * x.x;
-                    h.y += a.E[d] * x.y;
+            for (int dd = 0; dd < 3; ++dd)
+                if (a.xi[dd]) {
+                    const double2 x = a.xi[dd][o];
+                    h.x += a.E[dd] * x.x;
+                    h.y += a.E[dd] * x.y;
                 }
             if (a.sigma) h = cadd(h, a.sigma[o]);
             if (a.hdiag) {
@@ -92,14 +104,23 @@ __global__ void __launch_bounds__(256) comm_kernel(CommArgs a)
                 decode_plane<NB, LAYOUT>(p, n, m);
                 if (n == m) h = cadd(h, a.hdiag[n]);
             }
+            if (a.D) d = a.D[o];
+            if (u.out1) x1 = x1_in ? r : u.x1[o];
+            if (u.out2) x2 = x2_in ? r : u.x2[o];
+            if (u.ab_order) x1 = y_in ? r : u.y[o];
         }
         sR[idx] = r;
         sH[idx] = h;
+        dv[j] = d;
+        x1v[j] = x1;
+        x2v[j] = x2;
     }
     __syncthreads();
 
-    const StageUpdate& u = a.up;
-    for (int o = threadIdx.x; o < NPL * TK; o += blockDim.x) {
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+        const int o = threadIdx.x + j * NT;
+        if (o >= NPL * TK) break;
         const int p = o / TK;
         const int kk = o - p * TK;
         const int k = k0 + kk;
@@ -121,20 +142,14 @@ __global__ void __launch_bounds__(256) comm_kernel(CommArgs a)
         }
         const long og = long(p) * Nk + k;
         // f = D - i * acc
-        double2 f = make_double2(acc.y, -acc.x);
-        if (a.D) f = cadd(a.D[og], f);
+        const double2 f = make_double2(dv[j].x + acc.y, dv[j].y - acc.x);
 
         if (u.out_f) u.out_f[og] = f;
-        if (u.out1) {
-            const double2 x = u.x1[og];
-            u.out1[og] = make_double2(x.x + u.b1 * f.x, x.y + u.b1 * f.y);
-        }
-        if (u.out2) {
-            const double2 x = u.x2[og];
-            u.out2[og] = make_double2(x.x + u.b2 * f.x, x.y + u.b2 * f.y);
-        }
+        if (u.out1) u.out1[og] = make_double2(x1v[j].x + u.b1 * f.x, x1v[j].y + u.b1 * f.y);
+        if (u.out2) u.out2[og] = make_double2(x2v[j].x + u.b2 * f.x, x2v[j].y + u.b2 * f.y);
         if (u.ab_order) {
-            double2 yv = u.y[og];
+            // y <- y + dt beta_0 f, then + dt beta_i f_{n-i} in the reference's order
+            double2 yv = x1v[j];
             yv.x += u.ab_c[0] * f.x;
             yv.y += u.ab_c[0] * f.y;
             for (int i = 1; i < u.ab_order; ++i) {
@@ -150,43 +165,58 @@ __global__ void __launch_bounds__(256) comm_kernel(CommArgs a)
         // The block's new state is in global memory (visible to the block after
         // the barrier, mostly still in cache): check trace and Hermiticity of
         // its TK k-points without another pass over HBM.
-        __shared__ double sred[2 * TK];
+        constexpr int NT = 256;  // launch block size
+        __shared__ double sred[2 * NT];
         __syncthreads();
         const double2* ys = u.mon_src;
-        for (int kk = threadIdx.x; kk < TK; kk += blockDim.x) {
+        double mt = 0., mh = 0.;
+        // trace and diagonal imaginary part: one thread per k-point of the tile
+        for (int kk = threadIdx.x; kk < TK; kk += NT) {
             const int k = k0 + kk;
-            double mt = 0., mh = 0.;
-            if (k < Nk) {
-                double2 tr = make_double2(-u.nf, 0.);
-                for (int n = 0; n < NB; ++n) {
-                    bool c;
-                    const double2 v = ys[long(slot<NB, LAYOUT>(n, n, c)) * Nk + k];
-                    tr = cadd(tr, v);
-                    mh = fmax(mh, 2.0 * fabs(v.y));
-                }
-                if (LAYOUT == kFull)
-                    for (int n = 0; n < NB; ++n)
-                        for (int m = n + 1; m < NB; ++m) {
-                            const double2 x = ys[long(n * NB + m) * Nk + k];
-                            const double2 z = ys[long(m * NB + n) * Nk + k];
-                            mh = fmax(mh, sqrt((x.x - z.x) * (x.x - z.x) + (x.y + z.y) * (x.y + z.y)));
-                        }
-                mt = sqrt(tr.x * tr.x + tr.y * tr.y);
-                if (mt != mt) mt = __builtin_huge_val();  // NaN trips the monitor
-                if (mh != mh) mh = __builtin_huge_val();
+            if (k >= Nk) continue;
+            double2 tr = make_double2(-u.nf, 0.);
+            for (int n = 0; n < NB; ++n) {
+                bool c;
+                const double2 v = ys[long(slot<NB, LAYOUT>(n, n, c)) * Nk + k];
+                tr = cadd(tr, v);
+                mh = fmax(mh, 2.0 * fabs(v.y));
             }
-            sred[kk] = mt;
-            sred[TK + kk] = mh;
+            const double t = sqrt(tr.x * tr.x + tr.y * tr.y);
+            mt = (t != t) ? __builtin_huge_val() : fmax(mt, t);  // NaN trips the monitor
         }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            double mt = 0., mh = 0.;
-            for (int kk = 0; kk < TK; ++kk) {
-                mt = fmax(mt, sred[kk]);
-                mh = fmax(mh, sred[TK + kk]);
+        // |rho_nm - conj(rho_mn)| over all pairs n < m, spread over the block
+        if (LAYOUT == kFull) {
+            constexpr int NPAIR = NB * (NB - 1) / 2;
+            for (int item = threadIdx.x; item < TK * NPAIR; item += NT) {
+                const int q = item / TK;
+                const int kk = item - q * TK;
+                const int k = k0 + kk;
+                if (k >= Nk) continue;
+                int n = 0, r = q;
+                while (r >= NB - 1 - n) {
+                    r -= NB - 1 - n;
+                    ++n;
+                }
+                const int m = n + 1 + r;
+                const double2 x = ys[long(n * NB + m) * Nk + k];
+                const double2 z = ys[long(m * NB + n) * Nk + k];
+                const double d = sqrt((x.x - z.x) * (x.x - z.x) + (x.y + z.y) * (x.y + z.y));
+                mh = (d != d) ? __builtin_huge_val() : fmax(mh, d);
             }
-            atomicMax(&u.mon[0], static_cast<unsigned long long>(__double_as_longlong(mt)));
-            atomicMax(&u.mon[1], static_cast<unsigned long long>(__double_as_longlong(mh)));
+        }
+        sred[threadIdx.x] = mt;
+        sred[NT + threadIdx.x] = mh;
+        __syncthreads();
+        for (int st = NT / 2; st > 0; st >>= 1) {
+            if (threadIdx.x < st) {
+                sred[threadIdx.x] = fmax(sred[threadIdx.x], sred[threadIdx.x + st]);
+                sred[NT + threadIdx.x] = fmax(sred[NT + threadIdx.x], sred[NT + threadIdx.x + st]);
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            atomicMax(&u.mon[0], static_cast<unsigned long long>(__double_as_longlong(sred[0])));
+            atomicMax(&u.mon[1], static_cast<unsigned long long>(__double_as_longlong(sred[NT])));
         }
     }
 }

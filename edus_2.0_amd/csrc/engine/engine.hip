@@ -41,18 +41,18 @@ namespace {
 
 FFTFactors factorize(int n)
 {
+    // radix 8 first (fewest passes), then 4, 2, 3, 5 and the primes 7, 11, 13
     FFTFactors f{};
     int x = n;
-    // radix 4 first (fewest passes), then 2, 3, 5, then other primes
-    while (x % 4 == 0) { f.radix[f.count++] = 4; x /= 4; }
-    for (int r : {2, 3, 5})
-        while (x % r == 0) { f.radix[f.count++] = r; x /= r; }
-    for (int r = 7; x > 1; r += 2)
+    for (int r : {8, 4, 2, 3, 5, 7, 11, 13})
         while (x % r == 0) {
             if (f.count >= kMaxFactors) throw EngineError(EDUS_ERR_UNSUPPORTED, "grid size has too many factors");
             f.radix[f.count++] = r;
             x /= r;
         }
+    if (x != 1)
+        throw EngineError(EDUS_ERR_UNSUPPORTED,
+                          "grid size " + std::to_string(n) + " has a prime factor above 13 (FFT radices 2..13)");
     return f;
 }
 
@@ -68,12 +68,11 @@ std::vector<double2> twiddles(int n)
     return t;
 }
 
-// lines per work-group: ~2048 values in flight, LDS (twiddles + two buffers) <= 64 KB
+// lines per work-group: L*n <= 2048 values (one LDS buffer of <= 32 KB)
 int lines_per_group(int n)
 {
-    n = std::max(n, 1);
-    const int by_lds = (4096 / n - 1) / 2;
-    return std::max(1, std::min({32, 2048 / n, by_lds}));
+    if (n > 2048) throw EngineError(EDUS_ERR_UNSUPPORTED, "grid axes longer than 2048 are not supported");
+    return std::max(1, std::min(32, 2048 / std::max(n, 1)));
 }
 
 }  // namespace

@@ -492,10 +492,10 @@ void Simulation::setup_coulomb(const TBModel& tb)
     for (int iR = 0; iR < tb.nR; ++iR)
         if (std::abs(tb.R[iR][0]) + std::abs(tb.R[iR][1]) + std::abs(tb.R[iR][2]) < 1e-6) iorig = iR;
     if (iorig < 0) throw std::runtime_error("tb model has no R=0 block");
+    // Wannier centres = diagonal of r(R=0); tb.r is already in bohr here
     std::vector<std::array<double, 3>> c(nb);
     for (int n = 0; n < nb; ++n)
-        for (int d = 0; d < 3; ++d)
-            c[n][d] = convert(tb.r[d][size_t(iorig) * M + n * nb + n].real(), Angstrom, AuLength);
+        for (int d = 0; d < 3; ++d) c[n][d] = tb.r[d][size_t(iorig) * M + n * nb + n].real();
     std::array<double, 3> mind = {100, 0, 0};
     double mindn = 100;
     for (int n = 0; n < nb; ++n)
