@@ -22,6 +22,7 @@
 #include <map>
 #include <stdexcept>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "../../../include/edus_gpu.h"
@@ -68,11 +69,11 @@ std::vector<double2> twiddles(int n)
     return t;
 }
 
-// lines per work-group: L*n <= 2048 values (one LDS buffer of <= 32 KB)
-int lines_per_group(int n)
+// shapes already tuned in this process: (axis length, other axis, planes) -> config
+std::map<std::tuple<int, int, int, int>, int>& tuned_shapes()
 {
-    if (n > 2048) throw EngineError(EDUS_ERR_UNSUPPORTED, "grid axes longer than 2048 are not supported");
-    return std::max(1, std::min(32, 2048 / std::max(n, 1)));
+    static std::map<std::tuple<int, int, int, int>, int> m;
+    return m;
 }
 
 }  // namespace
@@ -97,6 +98,7 @@ Engine::Engine(const edus_desc& d, const EngineOptions& opt)
     std::memcpy(A, d.A, sizeof(A));
     if (nb < 1 || nb > kMaxNb) throw EngineError(EDUS_ERR_UNSUPPORTED, "nb must be in 1..16");
     if (N0 < 1 || N1 < 1) throw EngineError(EDUS_ERR_ARG, "grid sizes must be positive");
+    if (N0 > 2048 || N1 > 2048) throw EngineError(EDUS_ERR_UNSUPPORTED, "grid axes longer than 2048 are not supported");
     if (layout != EDUS_LAYOUT_FULL && layout != EDUS_LAYOUT_HERMITIAN) throw EngineError(EDUS_ERR_ARG, "unknown layout");
     if (!d.H0k) throw EngineError(EDUS_ERR_ARG, "H0k is required");
     if (prec != EDUS_PREC_FP64 && prec != EDUS_PREC_MIXED) throw EngineError(EDUS_ERR_ARG, "unknown precision");
@@ -144,6 +146,7 @@ Engine::Engine(const edus_desc& d, const EngineOptions& opt)
         HIP_CHECK(hipStreamSynchronize(s));
     }
 
+    autotune();
     if (opt.velocity_det > 0.0) build_velocity(opt.velocity_det);
 
     if (coulomb) {
@@ -278,8 +281,8 @@ void Engine::collect_timing()
     pending.clear();
 }
 
-void Engine::axis_fft(int axis, int mode, const double2* in, const double2* addend, double2* out, double coef,
-                      double scale, const double* mult, const char* cls)
+AxisFFTArgs Engine::axis_args(int axis, int cfg, int mode, const double2* in, const double2* addend, double2* out,
+                              double coef, double scale, const double* mult) const
 {
     AxisFFTArgs a{};
     a.in = in;
@@ -290,7 +293,7 @@ void Engine::axis_fft(int axis, int mode, const double2* in, const double2* adde
     a.N1 = N1;
     a.axis = axis;
     a.n = axis == 0 ? N0 : N1;
-    a.L = lines_per_group(a.n);
+    a.L = fft_lines(cfg, a.n);
     if (axis == 0) a.L = std::min(a.L, N1);
     a.mode = mode;
     a.coef = coef;
@@ -298,10 +301,62 @@ void Engine::axis_fft(int axis, int mode, const double2* in, const double2* adde
     a.mult = mult;
     a.fac = fac[axis];
     a.tw = tw[axis];
+    return a;
+}
+
+void Engine::axis_fft(int axis, int mode, const double2* in, const double2* addend, double2* out, double coef,
+                      double scale, const double* mult, const char* cls)
+{
+    const AxisFFTArgs a = axis_args(axis, fft_cfg[axis], mode, in, addend, out, coef, scale, mult);
     TimedLaunch t;
     tbegin(cls, t);
-    launch_axis_fft(a, s);
+    launch_axis_fft(a, fft_cfg[axis], s);
     tend(t);
+}
+
+// Tile autotuner: time every axis-FFT configuration on this engine's own
+// arrays (derivative mode, the hot use) and keep the fastest per axis.
+// Results are cached per shape for the life of the process.
+void Engine::autotune()
+{
+    for (int axis = 0; axis < 2; ++axis) {
+        const int n = axis == 0 ? N0 : N1;
+        if (n < 2) continue;
+        const auto key = std::make_tuple(axis, n, axis == 0 ? N1 : N0, npl);
+        auto it = tuned_shapes().find(key);
+        if (it != tuned_shapes().end()) {
+            fft_cfg[axis] = it->second;
+            continue;
+        }
+        hipEvent_t e0, e1;
+        HIP_CHECK(hipEventCreate(&e0));
+        HIP_CHECK(hipEventCreate(&e1));
+        float best = 1e30f;
+        int best_cfg = 0;
+        for (int cfg = 0; cfg < kNumFFTConfigs; ++cfg) {
+            if (fft_lines(cfg, n) * n > kFFTConfigs[cfg].values) continue;  // line longer than the tile
+            const AxisFFTArgs a = axis_args(axis, cfg, 0, y, nullptr, G, 1e-3, 1.0, nullptr);
+            launch_axis_fft(a, cfg, s);  // warm-up
+            HIP_CHECK(hipEventRecord(e0, s));
+            for (int r = 0; r < 3; ++r) launch_axis_fft(a, cfg, s);
+            HIP_CHECK(hipEventRecord(e1, s));
+            HIP_CHECK(hipEventSynchronize(e1));
+            float ms = 0.f;
+            HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+            if (ms < best) {
+                best = ms;
+                best_cfg = cfg;
+            }
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        HIP_CHECK(hipGetLastError());
+        if (best >= 1e30f) throw EngineError(EDUS_ERR_UNSUPPORTED, "no FFT tile fits a line of " + std::to_string(n));
+        fft_cfg[axis] = best_cfg;
+        tuned_shapes()[key] = best_cfg;
+    }
+    HIP_CHECK(hipMemsetAsync(G, 0, plane_bytes(), s));
+    HIP_CHECK(hipStreamSynchronize(s));
 }
 
 // D = E . grad_k rho  (null when the field has no component along the grid):

@@ -64,6 +64,7 @@ class Engine {
     int abidx[5] = {0, 1, 2, 3, 4};
     double2* tw[2] = {nullptr, nullptr};
     FFTFactors fac[2];
+    int fft_cfg[2] = {0, 0};  // autotuned tile configuration per grid axis
 
     bool timing = false;
     std::vector<TimedLaunch> pending;
@@ -87,6 +88,9 @@ class Engine {
     const double2* gradient(const double2* in, const double E[3]);
     void fock(const double2* in);
     void build_velocity(double det);
+    void autotune();
+    AxisFFTArgs axis_args(int axis, int cfg, int mode, const double2* in, const double2* addend, double2* out,
+                          double coef, double scale, const double* mult) const;
     void tbegin(const char* cls, TimedLaunch& t);
     void tend(TimedLaunch& t);
 };

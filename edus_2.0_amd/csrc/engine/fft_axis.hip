@@ -15,20 +15,18 @@
 // The same kernel does plain forward / inverse transforms and the R-space
 // point-wise product of the Hartree-Fock convolution along an axis.
 //
-// Work-group: 256 threads, L lines of length n with L*n <= 2048.  Global data
+// Work-group: NT threads, L lines of length n with L*n <= MAXV.  Global data
 // moves through registers (every load of the tile is issued before the first
-// LDS write); the passes run in place in one LDS buffer: each thread reads all
-// its butterflies' inputs, a barrier, then writes all outputs.
+// LDS write); the passes ping-pong between two LDS buffers, one barrier each.
+// (NT, MAXV) is picked per grid axis by the engine's autotuner.
+#include <algorithm>
+
 #include "common.hpp"
 #include "kernels.hpp"
 
 namespace edus {
 
 namespace {
-
-constexpr int kThreads = 256;
-constexpr int kMaxValues = 2048;              // L * n per work-group
-constexpr int kPer = kMaxValues / kThreads;   // values per thread
 
 constexpr double kC5_1 = 0.30901699437494742;   // cos(2 pi/5)
 constexpr double kC5_2 = -0.80901699437494742;  // cos(4 pi/5)
@@ -118,76 +116,65 @@ __device__ inline void dft(double2 (&v)[R], double sign, const double2* __restri
     }
 }
 
-// One in-place Stockham pass of radix R over `lines` lines of length n.
-template <int R>
-__device__ void pass(double2* __restrict__ buf, int lines, int n, int Ns, const double2* __restrict__ tw,
-                     double sign)
+// One Stockham pass of radix R over `lines` lines of length n, src -> dst.
+template <int R, int NT>
+__device__ void pass(const double2* __restrict__ src, double2* __restrict__ dst, int lines, int n, int Ns,
+                     const double2* __restrict__ tw, double sign)
 {
-    constexpr int MB = (kMaxValues / R + kThreads - 1) / kThreads;  // butterflies per thread
     const int nr = n / R;
     const int tasks = lines * nr;
     const int tstride = n / (Ns * R);
-    double2 v[MB][R];
+    for (int t = threadIdx.x; t < tasks; t += NT) {
+        const int line = t / nr;
+        const int j = t - line * nr;
+        const int k = j % Ns;
+        const double2* s = src + line * n;
+        double2 v[R];
 #pragma unroll
-    for (int b = 0; b < MB; ++b) {
-        const int t = threadIdx.x + b * kThreads;
-        if (t < tasks) {
-            const int line = t / nr;
-            const int j = t - line * nr;
-            const int k = j % Ns;
-            const double2* s = buf + line * n;
-#pragma unroll
-            for (int r = 0; r < R; ++r) {
-                double2 x = s[j + r * nr];
-                if (r > 0) {
-                    double2 w = tw[r * k * tstride];  // forward uses w, inverse conj(w)
-                    if (sign > 0) w.y = -w.y;
-                    x = cmul(x, w);
-                }
-                v[b][r] = x;
+        for (int r = 0; r < R; ++r) {
+            double2 x = s[j + r * nr];
+            if (r > 0) {
+                double2 w = tw[r * k * tstride];  // forward uses w, inverse conj(w)
+                if (sign > 0) w.y = -w.y;
+                x = cmul(x, w);
             }
+            v[r] = x;
         }
-    }
-    __syncthreads();
+        dft<R>(v, sign, tw, nr);
+        double2* d = dst + line * n + (j / Ns) * Ns * R + k;
 #pragma unroll
-    for (int b = 0; b < MB; ++b) {
-        const int t = threadIdx.x + b * kThreads;
-        if (t < tasks) {
-            const int line = t / nr;
-            const int j = t - line * nr;
-            const int k = j % Ns;
-            dft<R>(v[b], sign, tw, nr);
-            double2* d = buf + line * n + (j / Ns) * Ns * R + k;
-#pragma unroll
-            for (int r = 0; r < R; ++r) d[r * Ns] = v[b][r];
-        }
+        for (int r = 0; r < R; ++r) d[r * Ns] = v[r];
     }
     __syncthreads();
 }
 
-__device__ void run_fft(double2* buf, int lines, int n, const FFTFactors& f, const double2* tw, double sign)
+// All passes, ping-ponging between a and b; returns the buffer with the result.
+template <int NT>
+__device__ double2* run_fft(double2* a, double2* b, int lines, int n, const FFTFactors& f, const double2* tw,
+                            double sign)
 {
     int Ns = 1;
     for (int i = 0; i < f.count; ++i) {
-        const int R = f.radix[i];
-        switch (R) {
-            case 2: pass<2>(buf, lines, n, Ns, tw, sign); break;
-            case 3: pass<3>(buf, lines, n, Ns, tw, sign); break;
-            case 4: pass<4>(buf, lines, n, Ns, tw, sign); break;
-            case 5: pass<5>(buf, lines, n, Ns, tw, sign); break;
-            case 7: pass<7>(buf, lines, n, Ns, tw, sign); break;
-            case 8: pass<8>(buf, lines, n, Ns, tw, sign); break;
-            case 11: pass<11>(buf, lines, n, Ns, tw, sign); break;
-            case 13: pass<13>(buf, lines, n, Ns, tw, sign); break;
+        switch (f.radix[i]) {
+            case 2: pass<2, NT>(a, b, lines, n, Ns, tw, sign); break;
+            case 3: pass<3, NT>(a, b, lines, n, Ns, tw, sign); break;
+            case 4: pass<4, NT>(a, b, lines, n, Ns, tw, sign); break;
+            case 5: pass<5, NT>(a, b, lines, n, Ns, tw, sign); break;
+            case 7: pass<7, NT>(a, b, lines, n, Ns, tw, sign); break;
+            case 8: pass<8, NT>(a, b, lines, n, Ns, tw, sign); break;
+            case 11: pass<11, NT>(a, b, lines, n, Ns, tw, sign); break;
+            case 13: pass<13, NT>(a, b, lines, n, Ns, tw, sign); break;
             default: break;  // rejected on the host
         }
-        Ns *= R;
+        Ns *= f.radix[i];
+        double2* t = a;
+        a = b;
+        b = t;
     }
+    return a;
 }
 
-}  // namespace
-
-// One work-group handles `L` lines.
+// One work-group handles `L` lines (L*n <= MAXV values).
 //   axis 1: lines are consecutive rows (global row index p*N0 + i0): one
 //           contiguous chunk of L*N1 values;
 //   axis 0: lines are L consecutive columns of one plane.
@@ -195,14 +182,17 @@ __device__ void run_fft(double2* buf, int lines, int n, const FFTFactors& f, con
 // mode 1: forward     out = scale * F in          (sign -1)
 // mode 2: inverse     out = scale * F^-1 in       (sign +1)
 // mode 3: convolution out = F^-1 diag(scale * mult(R)) F in
-__global__ void __launch_bounds__(kThreads) axis_fft_kernel(AxisFFTArgs a)
+template <int NT, int MAXV>
+__global__ void __launch_bounds__(NT) axis_fft_kernel(AxisFFTArgs a)
 {
+    constexpr int PER = MAXV / NT;
     extern __shared__ double2 lds[];
     const int n = a.n;
-    double2* twl = lds;      // n twiddles
-    double2* buf = lds + n;  // L*n values, line-major
-    const int Nk = a.N0 * a.N1;
     const int L = a.L;
+    double2* twl = lds;                // n twiddles
+    double2* bufA = lds + n;           // L*n values, line-major
+    double2* bufB = bufA + L * n;
+    const int Nk = a.N0 * a.N1;
 
     // global offset of value j of line c: base + c * cstride + j * rowstride
     long base, cstride, rowstride;
@@ -222,64 +212,79 @@ __global__ void __launch_bounds__(kThreads) axis_fft_kernel(AxisFFTArgs a)
         cstride = 1;
         rowstride = a.N1;
     }
-    // global enumeration of the tile: axis 1 walks the contiguous chunk (line
-    // after line), axis 0 walks row after row across the L columns
+    // global enumeration of the tile: axis 1 walks the contiguous chunk line
+    // after line, axis 0 walks row after row across the L columns
     const int span = a.axis == 1 ? nlines * n : n * L;
 
-    double2 v[kPer], add[kPer];
-    double mul[kPer];
+    {
+        double2 v[PER];
 #pragma unroll
-    for (int e = 0; e < kPer; ++e) {
-        const int idx = threadIdx.x + e * kThreads;
-        const int c = a.axis == 1 ? idx / n : idx % L;
-        const int j = a.axis == 1 ? idx - c * n : idx / L;
-        if (idx < span && c < nlines) {
-            const long off = base + c * cstride + long(j) * rowstride;
-            v[e] = a.in[off];
-            if (a.addend) add[e] = a.addend[off];
-            if (a.mode == 3) mul[e] = a.mult[off];
+        for (int e = 0; e < PER; ++e) {
+            const int idx = threadIdx.x + e * NT;
+            const int c = a.axis == 1 ? idx / n : idx % L;
+            const int j = a.axis == 1 ? idx - c * n : idx / L;
+            if (idx < span && c < nlines) v[e] = a.in[base + c * cstride + long(j) * rowstride];
         }
-    }
-    for (int i = threadIdx.x; i < n; i += kThreads) twl[i] = a.tw[i];
+        for (int i = threadIdx.x; i < n; i += NT) twl[i] = a.tw[i];
 #pragma unroll
-    for (int e = 0; e < kPer; ++e) {
-        const int idx = threadIdx.x + e * kThreads;
-        const int c = a.axis == 1 ? idx / n : idx % L;
-        const int j = a.axis == 1 ? idx - c * n : idx / L;
-        if (idx < span && c < nlines) buf[c * n + j] = v[e];
+        for (int e = 0; e < PER; ++e) {
+            const int idx = threadIdx.x + e * NT;
+            const int c = a.axis == 1 ? idx / n : idx % L;
+            const int j = a.axis == 1 ? idx - c * n : idx / L;
+            if (idx < span && c < nlines) bufA[c * n + j] = v[e];
+        }
     }
     __syncthreads();
 
+    double2* res;
     if (a.mode == 0 || a.mode == 3) {
-        run_fft(buf, nlines, n, a.fac, twl, -1.0);
+        double2* x = run_fft<NT>(bufA, bufB, nlines, n, a.fac, twl, -1.0);
         if (a.mode == 0) {
-            for (int idx = threadIdx.x; idx < nlines * n; idx += kThreads) {
+            for (int idx = threadIdx.x; idx < nlines * n; idx += NT) {
                 const int j = idx % n;
-                buf[idx] = cmul_i(buf[idx], a.coef * double(gamma_centred(j, n)));
+                x[idx] = cmul_i(x[idx], a.coef * double(gamma_centred(j, n)));
             }
         } else {
+            double mul[PER];
 #pragma unroll
-            for (int e = 0; e < kPer; ++e) {
-                const int idx = threadIdx.x + e * kThreads;
+            for (int e = 0; e < PER; ++e) {
+                const int idx = threadIdx.x + e * NT;
                 const int c = a.axis == 1 ? idx / n : idx % L;
                 const int j = a.axis == 1 ? idx - c * n : idx / L;
-                if (idx < span && c < nlines) buf[c * n + j] = cscale(buf[c * n + j], a.scale * mul[e]);
+                if (idx < span && c < nlines) mul[e] = a.mult[base + c * cstride + long(j) * rowstride];
+            }
+#pragma unroll
+            for (int e = 0; e < PER; ++e) {
+                const int idx = threadIdx.x + e * NT;
+                const int c = a.axis == 1 ? idx / n : idx % L;
+                const int j = a.axis == 1 ? idx - c * n : idx / L;
+                if (idx < span && c < nlines) x[c * n + j] = cscale(x[c * n + j], a.scale * mul[e]);
             }
         }
         __syncthreads();
-        run_fft(buf, nlines, n, a.fac, twl, +1.0);
+        res = run_fft<NT>(x, x == bufA ? bufB : bufA, nlines, n, a.fac, twl, +1.0);
     } else {
-        run_fft(buf, nlines, n, a.fac, twl, a.mode == 1 ? -1.0 : +1.0);
+        res = run_fft<NT>(bufA, bufB, nlines, n, a.fac, twl, a.mode == 1 ? -1.0 : +1.0);
     }
     const double sc = (a.mode == 1 || a.mode == 2) ? a.scale : 1.0;
 
+    double2 add[PER];
+    if (a.addend) {
 #pragma unroll
-    for (int e = 0; e < kPer; ++e) {
-        const int idx = threadIdx.x + e * kThreads;
+        for (int e = 0; e < PER; ++e) {
+            const int idx = threadIdx.x + e * NT;
+            const int c = a.axis == 1 ? idx / n : idx % L;
+            const int j = a.axis == 1 ? idx - c * n : idx / L;
+            if (idx < span && c < nlines) add[e] = a.addend[base + c * cstride + long(j) * rowstride];
+        }
+    }
+#pragma unroll
+    for (int e = 0; e < PER; ++e) {
+        const int idx = threadIdx.x + e * NT;
         const int c = a.axis == 1 ? idx / n : idx % L;
         const int j = a.axis == 1 ? idx - c * n : idx / L;
         if (idx < span && c < nlines) {
-            double2 x = buf[c * n + j];
+            double2 x = res[c * n + j];
             if (sc != 1.0) x = cscale(x, sc);
             if (a.addend) x = cadd(x, add[e]);
             a.out[base + c * cstride + long(j) * rowstride] = x;
@@ -287,15 +292,45 @@ __global__ void __launch_bounds__(kThreads) axis_fft_kernel(AxisFFTArgs a)
     }
 }
 
-void launch_axis_fft(const AxisFFTArgs& a, hipStream_t s)
+}  // namespace
+
+// Tile configurations: (threads per work-group, values per work-group).
+// The engine's autotuner times them on the actual grid and picks one per axis.
+const FFTConfig kFFTConfigs[kNumFFTConfigs] = {{256, 2048}, {256, 1024}, {128, 1024}, {128, 512}, {64, 512}};
+
+int fft_lines(int cfg, int n)
+{
+    return std::max(1, std::min(32, kFFTConfigs[cfg].values / std::max(n, 1)));
+}
+
+template <int NT, int MAXV>
+void launch_cfg(const AxisFFTArgs& a, int blocks, size_t lds, hipStream_t s)
+{
+    // dynamic LDS above 64 KB must be allowed explicitly, once per instantiation
+    static bool raised = false;
+    if (!raised) {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&axis_fft_kernel<NT, MAXV>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, int((2048 + 2 * MAXV) * 16));
+        raised = true;
+    }
+    hipLaunchKernelGGL((axis_fft_kernel<NT, MAXV>), dim3(blocks), dim3(NT), lds, s, a);
+}
+
+void launch_axis_fft(const AxisFFTArgs& a, int cfg, hipStream_t s)
 {
     int blocks;
     if (a.axis == 1)
         blocks = int((long(a.P) * a.N0 + a.L - 1) / a.L);
     else
         blocks = a.P * ((a.N1 + a.L - 1) / a.L);
-    const size_t lds = size_t(a.n + a.L * a.n) * sizeof(double2);
-    hipLaunchKernelGGL(axis_fft_kernel, dim3(blocks), dim3(kThreads), lds, s, a);
+    const size_t lds = size_t(a.n + 2 * a.L * a.n) * sizeof(double2);
+    switch (cfg) {
+        case 0: launch_cfg<256, 2048>(a, blocks, lds, s); break;
+        case 1: launch_cfg<256, 1024>(a, blocks, lds, s); break;
+        case 2: launch_cfg<128, 1024>(a, blocks, lds, s); break;
+        case 3: launch_cfg<128, 512>(a, blocks, lds, s); break;
+        default: launch_cfg<64, 512>(a, blocks, lds, s); break;
+    }
 }
 
 }  // namespace edus

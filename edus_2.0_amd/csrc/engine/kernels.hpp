@@ -60,7 +60,15 @@ struct CommArgs {
     StageUpdate up;
 };
 
-void launch_axis_fft(const AxisFFTArgs& a, hipStream_t s);
+// Axis-FFT tile configurations (threads per work-group, values per work-group)
+struct FFTConfig {
+    int threads, values;
+};
+constexpr int kNumFFTConfigs = 5;
+extern const FFTConfig kFFTConfigs[kNumFFTConfigs];
+int fft_lines(int cfg, int n);  // lines per work-group for a line length n
+
+void launch_axis_fft(const AxisFFTArgs& a, int cfg, hipStream_t s);
 void launch_comm(int nb, int layout, const CommArgs& a, hipStream_t s);
 
 }  // namespace edus
