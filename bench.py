@@ -60,6 +60,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="ipa14_hex120")
     ap.add_argument("--layout", choices=["full", "hermitian"], default="full")
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64",
+                    help="mixed: FP32-MFMA commutator with FP64 state, gradient and updates")
     ap.add_argument("--cpu-steps", type=int, default=4)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -77,7 +79,8 @@ def main():
     tmp = tempfile.mkdtemp(prefix="edus_bench_")
     params = models.workload(args.workload, tmp)
     layout = edus.LAYOUT_FULL if args.layout == "full" else edus.LAYOUT_HERMITIAN
-    sim = edus.Simulation(params, layout=layout)
+    prec = edus.PREC_FP64 if args.precision == "fp64" else edus.PREC_MIXED
+    sim = edus.Simulation(params, layout=layout, precision=prec)
     eng = sim.engine
     nb, nk = sim.nb, sim.Nk
     npl = nb * nb if layout == edus.LAYOUT_FULL else nb * (nb + 1) // 2
@@ -107,15 +110,17 @@ def main():
     roofline = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": None, "bytes_per_launch": byts,
                 "avg_launch_us": avg_s * 1e6,
-                "kernel_share": {c: round(times[c][0] / 1e3 / elapsed, 4) for c in classes if times[c][1]}}
+                "kernel_share": {c: round(times[c][0] / 1e3 / elapsed, 4) for c in classes if times[c][1]},
+                "avg_us": {c: round(times[c][0] * 1e3 / times[c][1], 2) for c in classes if times[c][1]}}
 
     out = {
         "metric": METRIC, "value": value, "unit": "k-point·RK4-steps/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": elapsed / args.steps * 1e3, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64" if args.precision == "fp64" else "f64 (f32 commutator)",
+        "data": "synthetic",
         "config": {"workload": args.workload, "nb": nb, "grid": [sim.N0, sim.N1], "dt_au": sim.dt,
-                   "integrator": "RK4", "layout": args.layout, "precision": "fp64",
-                   "model": "seeded random tight-binding (edus.models)"},
+                   "integrator": "RK4", "layout": args.layout, "precision": args.precision,
+                   "model": "seeded random tight-binding (edus.models)", "tuning": eng.get_tuning().as_dict()},
         "roofline": roofline,
         "monitors": {"trace_dev": trace_dev, "herm_dev": herm_dev},
     }

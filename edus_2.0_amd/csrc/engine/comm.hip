@@ -1,4 +1,4 @@
-// Fused commutator + integrator-stage kernel (FP64).
+// Fused commutator + integrator-stage kernel.
 //
 // For a tile of TK consecutive k-points it forms, per k,
 //     H(k) = H0(k) + sum_d E_d xi_d(k) [+ Sigma_HF(k)] [+ Hartree diagonal]
@@ -10,16 +10,32 @@
 //     commutator(Output, -i, H, Input, false) (BlockMatrix_definitions.hpp:203-227),
 //     SumWithProduct stage updates (DESolver.hpp:10-20,155-220)
 // in separate sweeps over the whole grid.  Here H and rho of the tile sit in
-// LDS and each thread produces output elements
-//     f_nm = D_nm - i sum_l (H_nl rho_lm - rho_nl H_lm).
+// LDS, every global load of the tile is issued before the first barrier, and
+// the product [H, rho] is formed by one of three engines (KIND):
+//   kScalar   FP64 FMAs, one output element per thread:
+//             C_nm = sum_l (H_nl rho_lm - rho_nl H_lm);
+//   kMfma64   FP64 MFMA (v_mfma_f64_16x16x4): per k-point one wave forms the
+//             real-split complex products H.rho and rho.H on 16x16 tiles
+//             (nb <= 16, zero padded), 32 MFMAs per k;
+//   kMfma32   the same with FP32 operands and FP32 accumulation
+//             (v_mfma_f32_16x16x4): the mixed-precision mode.  D, the stage
+//             updates and the state stay FP64.
 // LAYOUT = kFull keeps every element; kHermitian keeps the upper triangle and
 // expands the lower one by conjugation while reading LDS.
+#include <algorithm>
+#include <type_traits>
+
 #include "common.hpp"
 #include "kernels.hpp"
 
 namespace edus {
 
 namespace {
+
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+typedef float flt4 __attribute__((ext_vector_type(4)));
+
+constexpr int kCommThreads = 256;
 
 template <int NB, int LAYOUT>
 __device__ inline void decode_plane(int p, int& n, int& m)
@@ -52,23 +68,115 @@ __device__ inline int slot(int a, int b, bool& conj)
 }
 
 template <int NB, int LAYOUT>
-constexpr int tile_for()
+constexpr int npl_of()
 {
-    // keep the two LDS tiles (rho and H) of NPL*TK complex doubles under ~64 KB
-    const int npl = LAYOUT == kFull ? NB * NB : NB * (NB + 1) / 2;
+    return LAYOUT == kFull ? NB * NB : NB * (NB + 1) / 2;
+}
+
+// scalar engine: keep the two LDS tiles (rho and H) under ~64 KB
+template <int NB, int LAYOUT>
+constexpr int scalar_tile()
+{
+    const int npl = npl_of<NB, LAYOUT>();
     int tk = 64;
     while (tk > 4 && 2 * npl * tk * 16 > 65536) tk /= 2;
     return tk;
 }
 
-template <int NB, int TK, int LAYOUT>
-__global__ void __launch_bounds__(256) comm_kernel(CommArgs a)
+// MFMA engines: 8 k-points per tile (two per wave; 8 x 16 B = one 128-byte
+// line per plane), plane stride padded to 9 against LDS bank conflicts
+constexpr int kMfmaTile = 8;
+
+template <int KIND, int TK>
+constexpr int plane_stride()
 {
-    constexpr int NPL = LAYOUT == kFull ? NB * NB : NB * (NB + 1) / 2;
-    constexpr int NT = 256;                        // launch block size
+    return KIND == kCommScalar ? TK : TK + 1;
+}
+
+template <int NB, int LAYOUT, int KIND>
+constexpr size_t comm_lds_bytes()
+{
+    constexpr int TK = KIND == kCommScalar ? scalar_tile<NB, LAYOUT>() : kMfmaTile;
+    // the monitor's block reduction reuses the tile: 2 doubles per thread
+    return std::max(size_t(2) * npl_of<NB, LAYOUT>() * plane_stride<KIND, TK>() * sizeof(double2),
+                    size_t(2) * kCommThreads * sizeof(double));
+}
+
+// element (r, c) of a tile in LDS (zero outside the nb x nb block)
+template <int NB, int LAYOUT, int PS>
+__device__ inline double2 lds_elem(const double2* s, int r, int c, int kk)
+{
+    if (r >= NB || c >= NB) return make_double2(0.0, 0.0);
+    bool cj;
+    const int sl = slot<NB, LAYOUT>(r, c, cj);
+    double2 v = s[sl * PS + kk];
+    if (cj) v.y = -v.y;
+    return v;
+}
+
+// [H, rho] of k-point kk of the tile on the matrix cores.  Lane l holds, per
+// K step s, A-fragments (row i = l%16, column c = 4s + l/16) and B-fragments
+// (row c, column i); the result element (4*(l/16) + v, l%16) is in slot v.
+//   Re C = Hr.Rr - Hi.Ri - Rr.Hr + Ri.Hi,   Im C = Hr.Ri + Hi.Rr - Rr.Hi - Ri.Hr
+template <int NB, int LAYOUT, int PS>
+__device__ inline void mfma_commutator64(const double2* sH, const double2* sR, int kk, int lane, dbl4& cre, dbl4& cim)
+{
+    const int i = lane & 15, q = lane >> 4;
+    cre = dbl4{0.0, 0.0, 0.0, 0.0};
+    cim = dbl4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        const int c = 4 * s + q;
+        if (4 * s >= NB) break;
+        const double2 hA = lds_elem<NB, LAYOUT, PS>(sH, i, c, kk), rA = lds_elem<NB, LAYOUT, PS>(sR, i, c, kk);
+        const double2 hB = lds_elem<NB, LAYOUT, PS>(sH, c, i, kk), rB = lds_elem<NB, LAYOUT, PS>(sR, c, i, kk);
+        cre = __builtin_amdgcn_mfma_f64_16x16x4f64(hA.x, rB.x, cre, 0, 0, 0);
+        cre = __builtin_amdgcn_mfma_f64_16x16x4f64(-hA.y, rB.y, cre, 0, 0, 0);
+        cre = __builtin_amdgcn_mfma_f64_16x16x4f64(-rA.x, hB.x, cre, 0, 0, 0);
+        cre = __builtin_amdgcn_mfma_f64_16x16x4f64(rA.y, hB.y, cre, 0, 0, 0);
+        cim = __builtin_amdgcn_mfma_f64_16x16x4f64(hA.x, rB.y, cim, 0, 0, 0);
+        cim = __builtin_amdgcn_mfma_f64_16x16x4f64(hA.y, rB.x, cim, 0, 0, 0);
+        cim = __builtin_amdgcn_mfma_f64_16x16x4f64(-rA.x, hB.y, cim, 0, 0, 0);
+        cim = __builtin_amdgcn_mfma_f64_16x16x4f64(-rA.y, hB.x, cim, 0, 0, 0);
+    }
+}
+
+template <int NB, int LAYOUT, int PS>
+__device__ inline void mfma_commutator32(const double2* sH, const double2* sR, int kk, int lane, flt4& cre, flt4& cim)
+{
+    const int i = lane & 15, q = lane >> 4;
+    cre = flt4{0.f, 0.f, 0.f, 0.f};
+    cim = flt4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        const int c = 4 * s + q;
+        if (4 * s >= NB) break;
+        const double2 hA = lds_elem<NB, LAYOUT, PS>(sH, i, c, kk), rA = lds_elem<NB, LAYOUT, PS>(sR, i, c, kk);
+        const double2 hB = lds_elem<NB, LAYOUT, PS>(sH, c, i, kk), rB = lds_elem<NB, LAYOUT, PS>(sR, c, i, kk);
+        const float hAx = float(hA.x), hAy = float(hA.y), rAx = float(rA.x), rAy = float(rA.y);
+        const float hBx = float(hB.x), hBy = float(hB.y), rBx = float(rB.x), rBy = float(rB.y);
+        cre = __builtin_amdgcn_mfma_f32_16x16x4f32(hAx, rBx, cre, 0, 0, 0);
+        cre = __builtin_amdgcn_mfma_f32_16x16x4f32(-hAy, rBy, cre, 0, 0, 0);
+        cre = __builtin_amdgcn_mfma_f32_16x16x4f32(-rAx, hBx, cre, 0, 0, 0);
+        cre = __builtin_amdgcn_mfma_f32_16x16x4f32(rAy, hBy, cre, 0, 0, 0);
+        cim = __builtin_amdgcn_mfma_f32_16x16x4f32(hAx, rBy, cim, 0, 0, 0);
+        cim = __builtin_amdgcn_mfma_f32_16x16x4f32(hAy, rBx, cim, 0, 0, 0);
+        cim = __builtin_amdgcn_mfma_f32_16x16x4f32(-rAx, hBy, cim, 0, 0, 0);
+        cim = __builtin_amdgcn_mfma_f32_16x16x4f32(-rAy, hBx, cim, 0, 0, 0);
+    }
+}
+
+template <int NB, int LAYOUT, int KIND>
+__global__ void __launch_bounds__(kCommThreads) comm_kernel(CommArgs a)
+{
+    constexpr int NT = kCommThreads;
+    constexpr int NPL = npl_of<NB, LAYOUT>();
+    constexpr int TK = KIND == kCommScalar ? scalar_tile<NB, LAYOUT>() : kMfmaTile;
+    constexpr int PS = plane_stride<KIND, TK>();
     constexpr int PER = (NPL * TK + NT - 1) / NT;  // elements per thread
-    __shared__ double2 sR[NPL * TK];
-    __shared__ double2 sH[NPL * TK];
+    extern __shared__ double2 lds[];
+    double2* sR = lds;
+    double2* sH = lds + NPL * PS;
     const int k0 = blockIdx.x * TK;
     const int Nk = a.Nk;
     const StageUpdate& u = a.up;
@@ -105,17 +213,67 @@ __global__ void __launch_bounds__(256) comm_kernel(CommArgs a)
                 if (n == m) h = cadd(h, a.hdiag[n]);
             }
             if (a.D) d = a.D[o];
-            if (u.out1) x1 = x1_in ? r : u.x1[o];
-            if (u.out2) x2 = x2_in ? r : u.x2[o];
-            if (u.ab_order) x1 = y_in ? r : u.y[o];
+            // (if/else rather than ?: on double2 keeps the values out of scratch)
+            if (u.out1) {
+                if (x1_in)
+                    x1 = r;
+                else
+                    x1 = u.x1[o];
+            }
+            if (u.out2) {
+                if (x2_in)
+                    x2 = r;
+                else
+                    x2 = u.x2[o];
+            }
+            if (u.ab_order) {
+                if (y_in)
+                    x1 = r;
+                else
+                    x1 = u.y[o];
+            }
         }
-        sR[idx] = r;
-        sH[idx] = h;
+        sR[p * PS + kk] = r;
+        sH[p * PS + kk] = h;
         dv[j] = d;
         x1v[j] = x1;
         x2v[j] = x2;
     }
     __syncthreads();
+
+    if constexpr (KIND != kCommScalar) {
+        // every wave takes TK/4 k-points of the tile; the products wait in
+        // registers until all waves are done reading rho and H, then replace
+        // rho in LDS
+        constexpr int KPW = TK / (NT / 64);
+        const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+        using acc_t = typename std::conditional<KIND == kCommMfma64, dbl4, flt4>::type;
+        acc_t cre[KPW], cim[KPW];
+#pragma unroll
+        for (int w = 0; w < KPW; ++w) {
+            const int kk = wave + (NT / 64) * w;
+            if constexpr (KIND == kCommMfma64)
+                mfma_commutator64<NB, LAYOUT, PS>(sH, sR, kk, lane, cre[w], cim[w]);
+            else
+                mfma_commutator32<NB, LAYOUT, PS>(sH, sR, kk, lane, cre[w], cim[w]);
+        }
+        __syncthreads();
+        const int i = lane & 15, q = lane >> 4;
+#pragma unroll
+        for (int w = 0; w < KPW; ++w) {
+            const int kk = wave + (NT / 64) * w;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int n = 4 * q + v, m = i;
+                if (n < NB && m < NB && (LAYOUT == kFull || n <= m)) {
+                    bool cj;
+                    const int sl = slot<NB, LAYOUT>(n, m, cj);
+                    sR[sl * PS + kk] = make_double2(double(cre[w][v]), double(cim[w][v]));
+                }
+            }
+        }
+        __syncthreads();
+    }
 
 #pragma unroll
     for (int j = 0; j < PER; ++j) {
@@ -125,23 +283,28 @@ __global__ void __launch_bounds__(256) comm_kernel(CommArgs a)
         const int kk = o - p * TK;
         const int k = k0 + kk;
         if (k >= Nk) continue;
-        int n, m;
-        decode_plane<NB, LAYOUT>(p, n, m);
-        double2 acc = make_double2(0., 0.);
+        double2 acc;
+        if constexpr (KIND == kCommScalar) {
+            int n, m;
+            decode_plane<NB, LAYOUT>(p, n, m);
+            acc = make_double2(0., 0.);
 #pragma unroll
-        for (int l = 0; l < NB; ++l) {
-            bool cnl, clm;
-            const int inl = slot<NB, LAYOUT>(n, l, cnl);
-            const int ilm = slot<NB, LAYOUT>(l, m, clm);
-            double2 Hnl = sH[inl * TK + kk], Rnl = sR[inl * TK + kk];
-            double2 Rlm = sR[ilm * TK + kk], Hlm = sH[ilm * TK + kk];
-            if (cnl) { Hnl.y = -Hnl.y; Rnl.y = -Rnl.y; }
-            if (clm) { Rlm.y = -Rlm.y; Hlm.y = -Hlm.y; }
-            acc.x += Hnl.x * Rlm.x - Hnl.y * Rlm.y - (Rnl.x * Hlm.x - Rnl.y * Hlm.y);
-            acc.y += Hnl.x * Rlm.y + Hnl.y * Rlm.x - (Rnl.x * Hlm.y + Rnl.y * Hlm.x);
+            for (int l = 0; l < NB; ++l) {
+                bool cnl, clm;
+                const int inl = slot<NB, LAYOUT>(n, l, cnl);
+                const int ilm = slot<NB, LAYOUT>(l, m, clm);
+                double2 Hnl = sH[inl * PS + kk], Rnl = sR[inl * PS + kk];
+                double2 Rlm = sR[ilm * PS + kk], Hlm = sH[ilm * PS + kk];
+                if (cnl) { Hnl.y = -Hnl.y; Rnl.y = -Rnl.y; }
+                if (clm) { Rlm.y = -Rlm.y; Hlm.y = -Hlm.y; }
+                acc.x += Hnl.x * Rlm.x - Hnl.y * Rlm.y - (Rnl.x * Hlm.x - Rnl.y * Hlm.y);
+                acc.y += Hnl.x * Rlm.y + Hnl.y * Rlm.x - (Rnl.x * Hlm.y + Rnl.y * Hlm.x);
+            }
+        } else {
+            acc = sR[p * PS + kk];
         }
         const long og = long(p) * Nk + k;
-        // f = D - i * acc
+        // f = D - i * [H, rho]
         const double2 f = make_double2(dv[j].x + acc.y, dv[j].y - acc.x);
 
         if (u.out_f) u.out_f[og] = f;
@@ -152,10 +315,13 @@ __global__ void __launch_bounds__(256) comm_kernel(CommArgs a)
             double2 yv = x1v[j];
             yv.x += u.ab_c[0] * f.x;
             yv.y += u.ab_c[0] * f.y;
-            for (int i = 1; i < u.ab_order; ++i) {
-                const double2 h = u.hist[i][og];
-                yv.x += u.ab_c[i] * h.x;
-                yv.y += u.ab_c[i] * h.y;
+#pragma unroll
+            for (int i = 1; i < 5; ++i) {  // constant indices: the pointer table stays in SGPRs
+                if (i < u.ab_order) {
+                    const double2 h = u.hist[i][og];
+                    yv.x += u.ab_c[i] * h.x;
+                    yv.y += u.ab_c[i] * h.y;
+                }
             }
             u.y[og] = yv;
         }
@@ -165,9 +331,8 @@ __global__ void __launch_bounds__(256) comm_kernel(CommArgs a)
         // The block's new state is in global memory (visible to the block after
         // the barrier, mostly still in cache): check trace and Hermiticity of
         // its TK k-points without another pass over HBM.
-        constexpr int NT = 256;  // launch block size
-        __shared__ double sred[2 * NT];
         __syncthreads();
+        double* sred = reinterpret_cast<double*>(lds);  // the tile is no longer needed
         const double2* ys = u.mon_src;
         double mt = 0., mh = 0.;
         // trace and diagonal imaginary part: one thread per k-point of the tile
@@ -221,39 +386,62 @@ __global__ void __launch_bounds__(256) comm_kernel(CommArgs a)
     }
 }
 
+template <int NB, int LAYOUT, int KIND>
+void launch_kind(const CommArgs& a, hipStream_t s)
+{
+    constexpr int TK = KIND == kCommScalar ? scalar_tile<NB, LAYOUT>() : kMfmaTile;
+    constexpr size_t lds = comm_lds_bytes<NB, LAYOUT, KIND>();
+    auto kern = &comm_kernel<NB, LAYOUT, KIND>;
+    static bool raised = false;
+    if (!raised) {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  int(lds));
+        raised = true;
+    }
+    hipLaunchKernelGGL(kern, dim3((a.Nk + TK - 1) / TK), dim3(kCommThreads), lds, s, a);
+}
+
 template <int NB>
-void launch_nb(const CommArgs& a, int layout, hipStream_t s)
+void launch_nb(const CommArgs& a, int layout, int kind, hipStream_t s)
 {
     if (layout == kFull) {
-        constexpr int TK = tile_for<NB, kFull>();
-        hipLaunchKernelGGL((comm_kernel<NB, TK, kFull>), dim3((a.Nk + TK - 1) / TK), dim3(256), 0, s, a);
+        if (kind == kCommMfma64)
+            launch_kind<NB, kFull, kCommMfma64>(a, s);
+        else if (kind == kCommMfma32)
+            launch_kind<NB, kFull, kCommMfma32>(a, s);
+        else
+            launch_kind<NB, kFull, kCommScalar>(a, s);
     } else {
-        constexpr int TK = tile_for<NB, kHermitian>();
-        hipLaunchKernelGGL((comm_kernel<NB, TK, kHermitian>), dim3((a.Nk + TK - 1) / TK), dim3(256), 0, s, a);
+        if (kind == kCommMfma64)
+            launch_kind<NB, kHermitian, kCommMfma64>(a, s);
+        else if (kind == kCommMfma32)
+            launch_kind<NB, kHermitian, kCommMfma32>(a, s);
+        else
+            launch_kind<NB, kHermitian, kCommScalar>(a, s);
     }
 }
 
 }  // namespace
 
-void launch_comm(int nb, int layout, const CommArgs& a, hipStream_t s)
+void launch_comm(int nb, int layout, int kind, const CommArgs& a, hipStream_t s)
 {
     switch (nb) {
-        case 1: launch_nb<1>(a, layout, s); break;
-        case 2: launch_nb<2>(a, layout, s); break;
-        case 3: launch_nb<3>(a, layout, s); break;
-        case 4: launch_nb<4>(a, layout, s); break;
-        case 5: launch_nb<5>(a, layout, s); break;
-        case 6: launch_nb<6>(a, layout, s); break;
-        case 7: launch_nb<7>(a, layout, s); break;
-        case 8: launch_nb<8>(a, layout, s); break;
-        case 9: launch_nb<9>(a, layout, s); break;
-        case 10: launch_nb<10>(a, layout, s); break;
-        case 11: launch_nb<11>(a, layout, s); break;
-        case 12: launch_nb<12>(a, layout, s); break;
-        case 13: launch_nb<13>(a, layout, s); break;
-        case 14: launch_nb<14>(a, layout, s); break;
-        case 15: launch_nb<15>(a, layout, s); break;
-        case 16: launch_nb<16>(a, layout, s); break;
+        case 1: launch_nb<1>(a, layout, kind, s); break;
+        case 2: launch_nb<2>(a, layout, kind, s); break;
+        case 3: launch_nb<3>(a, layout, kind, s); break;
+        case 4: launch_nb<4>(a, layout, kind, s); break;
+        case 5: launch_nb<5>(a, layout, kind, s); break;
+        case 6: launch_nb<6>(a, layout, kind, s); break;
+        case 7: launch_nb<7>(a, layout, kind, s); break;
+        case 8: launch_nb<8>(a, layout, kind, s); break;
+        case 9: launch_nb<9>(a, layout, kind, s); break;
+        case 10: launch_nb<10>(a, layout, kind, s); break;
+        case 11: launch_nb<11>(a, layout, kind, s); break;
+        case 12: launch_nb<12>(a, layout, kind, s); break;
+        case 13: launch_nb<13>(a, layout, kind, s); break;
+        case 14: launch_nb<14>(a, layout, kind, s); break;
+        case 15: launch_nb<15>(a, layout, kind, s); break;
+        case 16: launch_nb<16>(a, layout, kind, s); break;
         default: break;  // rejected at context creation
     }
 }

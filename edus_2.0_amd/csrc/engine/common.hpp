@@ -31,16 +31,16 @@ __host__ __device__ inline int num_planes(int nb, int layout) { return layout ==
 // plane of the diagonal element n
 __host__ __device__ inline int diag_plane(int n, int nb, int layout) { return layout == kFull ? n * (nb + 1) : tri_index(n, n, nb); }
 
-__device__ inline double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
-__device__ inline double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
-__device__ inline double2 cmul(double2 a, double2 b)
+__host__ __device__ inline double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__host__ __device__ inline double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__host__ __device__ inline double2 cmul(double2 a, double2 b)
 {
     return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
-__device__ inline double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
-__device__ inline double2 cscale(double2 a, double s) { return make_double2(a.x * s, a.y * s); }
+__host__ __device__ inline double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
+__host__ __device__ inline double2 cscale(double2 a, double s) { return make_double2(a.x * s, a.y * s); }
 // a * (i s)
-__device__ inline double2 cmul_i(double2 a, double s) { return make_double2(-a.y * s, a.x * s); }
+__host__ __device__ inline double2 cmul_i(double2 a, double s) { return make_double2(-a.y * s, a.x * s); }
 
 // element (r, c) of a field stored in `layout` at grid point k
 __device__ inline double2 element(const double2* __restrict__ f, int r, int c, int nb, int layout, long Nk, int k)

@@ -70,9 +70,16 @@ std::vector<double2> twiddles(int n)
 }
 
 // shapes already tuned in this process: (axis length, other axis, planes) -> config
-std::map<std::tuple<int, int, int, int>, int>& tuned_shapes()
+std::map<std::tuple<int, int, int, int>, FFTChoice>& tuned_shapes()
 {
-    static std::map<std::tuple<int, int, int, int>, int> m;
+    static std::map<std::tuple<int, int, int, int>, FFTChoice> m;
+    return m;
+}
+
+// commutator engine per (nb, layout, Nk)
+std::map<std::tuple<int, int, int>, int>& tuned_comm()
+{
+    static std::map<std::tuple<int, int, int>, int> m;
     return m;
 }
 
@@ -102,7 +109,6 @@ Engine::Engine(const edus_desc& d, const EngineOptions& opt)
     if (layout != EDUS_LAYOUT_FULL && layout != EDUS_LAYOUT_HERMITIAN) throw EngineError(EDUS_ERR_ARG, "unknown layout");
     if (!d.H0k) throw EngineError(EDUS_ERR_ARG, "H0k is required");
     if (prec != EDUS_PREC_FP64 && prec != EDUS_PREC_MIXED) throw EngineError(EDUS_ERR_ARG, "unknown precision");
-    if (prec == EDUS_PREC_MIXED) throw EngineError(EDUS_ERR_UNSUPPORTED, "mixed precision not built in this library");
     if (coulomb && (!d.W || !d.hartree || !d.rho0k))
         throw EngineError(EDUS_ERR_ARG, "coulomb requires W, hartree and rho0k");
     npl = num_planes(nb, layout);
@@ -195,7 +201,7 @@ void Engine::build_velocity(double det)
         c.up.x1 = f;
         c.up.out1 = vel[d];
         c.up.b1 = 1.0 / det;
-        launch_comm(nb, layout, c, s);
+        launch_comm(nb, layout, comm_kind64, c, s);
     }
     HIP_CHECK(hipMemsetAsync(aux, 0, pb, s));
     HIP_CHECK(hipStreamSynchronize(s));
@@ -281,8 +287,8 @@ void Engine::collect_timing()
     pending.clear();
 }
 
-AxisFFTArgs Engine::axis_args(int axis, int cfg, int mode, const double2* in, const double2* addend, double2* out,
-                              double coef, double scale, const double* mult) const
+AxisFFTArgs Engine::axis_args(int axis, const FFTChoice& c, int mode, const double2* in, const double2* addend,
+                              double2* out, double coef, double scale, const double* mult) const
 {
     AxisFFTArgs a{};
     a.in = in;
@@ -293,8 +299,7 @@ AxisFFTArgs Engine::axis_args(int axis, int cfg, int mode, const double2* in, co
     a.N1 = N1;
     a.axis = axis;
     a.n = axis == 0 ? N0 : N1;
-    a.L = fft_lines(cfg, a.n);
-    if (axis == 0) a.L = std::min(a.L, N1);
+    a.L = c.L;
     a.mode = mode;
     a.coef = coef;
     a.scale = scale;
@@ -307,55 +312,158 @@ AxisFFTArgs Engine::axis_args(int axis, int cfg, int mode, const double2* in, co
 void Engine::axis_fft(int axis, int mode, const double2* in, const double2* addend, double2* out, double coef,
                       double scale, const double* mult, const char* cls)
 {
-    const AxisFFTArgs a = axis_args(axis, fft_cfg[axis], mode, in, addend, out, coef, scale, mult);
+    const AxisFFTArgs a = axis_args(axis, fft_choice[axis], mode, in, addend, out, coef, scale, mult);
     TimedLaunch t;
     tbegin(cls, t);
-    launch_axis_fft(a, fft_cfg[axis], s);
+    launch_fft(a, fft_choice[axis], s);
     tend(t);
 }
 
-// Tile autotuner: time every axis-FFT configuration on this engine's own
-// arrays (derivative mode, the hot use) and keep the fastest per axis.
+namespace {
+
+// Tile candidates for one axis.  Columns (axis 0) are read in row segments
+// of L adjacent columns, so whole 128-byte lines (multiples of 8 columns) and
+// divisors of the row length (no ragged last group) are tried.
+std::vector<FFTChoice> fft_candidates(int axis, int n, int other)
+{
+    std::vector<FFTChoice> out;
+    auto add = [&](FFTChoice c) {
+        if (axis == 0) c.L = std::min(c.L, other);
+        if (c.L < 1) return;
+        for (const FFTChoice& o : out)
+            if (o.two_step == c.two_step && o.cfg == c.cfg && o.L == c.L) return;
+        out.push_back(c);
+    };
+    const int maxl = two_step_max_lines(n);
+    if (maxl > 0) {
+        std::vector<int> Ls{maxl};
+        if (axis == 1) {
+            Ls.push_back(std::max(1, maxl / 2));
+        } else {
+            for (int L : {8 * (maxl / 8), 16, 8})
+                if (L >= 1 && L <= maxl) Ls.push_back(L);
+            for (int L = std::min(maxl, other); L >= 1; --L) {
+                if (other % L == 0) {
+                    Ls.push_back(L);
+                    break;
+                }
+            }
+        }
+        for (int variant : {0, 1}) {
+            for (int L : Ls) {
+                FFTChoice c;
+                c.two_step = 1;
+                c.cfg = variant;
+                c.L = L;
+                add(c);
+            }
+        }
+    } else {
+        for (int cfg = 0; cfg < kNumFFTConfigs; ++cfg) {
+            if (fft_lines(cfg, n) * n > kFFTConfigs[cfg].values) continue;  // line longer than the tile
+            FFTChoice c;
+            c.cfg = cfg;
+            c.L = fft_lines(cfg, n);
+            add(c);
+        }
+    }
+    return out;
+}
+
+}  // namespace
+
+void launch_fft(const AxisFFTArgs& a, const FFTChoice& c, hipStream_t s)
+{
+    if (c.two_step)
+        launch_two_step(a, c.cfg, s);
+    else
+        launch_axis_fft(a, c.cfg, s);
+}
+
+// Tile autotuner: time every candidate tile of the axis' kernel family on this
+// engine's own arrays (derivative mode, the hot use) and keep the fastest.
 // Results are cached per shape for the life of the process.
 void Engine::autotune()
 {
     for (int axis = 0; axis < 2; ++axis) {
         const int n = axis == 0 ? N0 : N1;
+        const int other = axis == 0 ? N1 : N0;
         if (n < 2) continue;
-        const auto key = std::make_tuple(axis, n, axis == 0 ? N1 : N0, npl);
+        const auto key = std::make_tuple(axis, n, other, npl);
         auto it = tuned_shapes().find(key);
         if (it != tuned_shapes().end()) {
-            fft_cfg[axis] = it->second;
+            fft_choice[axis] = it->second;
             continue;
         }
+        const std::vector<FFTChoice> cands = fft_candidates(axis, n, other);
+        if (cands.empty()) throw EngineError(EDUS_ERR_UNSUPPORTED, "no FFT tile fits a line of " + std::to_string(n));
+        FFTChoice best_c = cands[0];
+        if (cands.size() > 1) {
+            hipEvent_t e0, e1;
+            HIP_CHECK(hipEventCreate(&e0));
+            HIP_CHECK(hipEventCreate(&e1));
+            float best = 1e30f;
+            for (const FFTChoice& c : cands) {
+                const AxisFFTArgs a = axis_args(axis, c, 0, y, nullptr, G, 1e-3, 1.0, nullptr);
+                launch_fft(a, c, s);  // warm-up
+                HIP_CHECK(hipEventRecord(e0, s));
+                for (int r = 0; r < 3; ++r) launch_fft(a, c, s);
+                HIP_CHECK(hipEventRecord(e1, s));
+                HIP_CHECK(hipEventSynchronize(e1));
+                float ms = 0.f;
+                HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+                if (ms < best) {
+                    best = ms;
+                    best_c = c;
+                }
+            }
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            HIP_CHECK(hipGetLastError());
+        }
+        fft_choice[axis] = best_c;
+        tuned_shapes()[key] = best_c;
+    }
+
+    // FP64 commutator engine: FMAs or matrix cores, whichever is faster here
+    // (they differ in summation order only)
+    const auto ckey = std::make_tuple(nb, layout, Nk);
+    auto cit = tuned_comm().find(ckey);
+    if (cit != tuned_comm().end()) {
+        comm_kind64 = cit->second;
+    } else {
         hipEvent_t e0, e1;
         HIP_CHECK(hipEventCreate(&e0));
         HIP_CHECK(hipEventCreate(&e1));
+        CommArgs c{};
+        c.Nk = Nk;
+        c.P = npl;
+        c.rho = y;
+        c.H0 = H0;
+        c.up.out_f = f;
         float best = 1e30f;
-        int best_cfg = 0;
-        for (int cfg = 0; cfg < kNumFFTConfigs; ++cfg) {
-            if (fft_lines(cfg, n) * n > kFFTConfigs[cfg].values) continue;  // line longer than the tile
-            const AxisFFTArgs a = axis_args(axis, cfg, 0, y, nullptr, G, 1e-3, 1.0, nullptr);
-            launch_axis_fft(a, cfg, s);  // warm-up
+        for (int kind : {int(kCommScalar), int(kCommMfma64)}) {
+            launch_comm(nb, layout, kind, c, s);  // warm-up
             HIP_CHECK(hipEventRecord(e0, s));
-            for (int r = 0; r < 3; ++r) launch_axis_fft(a, cfg, s);
+            for (int r = 0; r < 3; ++r) launch_comm(nb, layout, kind, c, s);
             HIP_CHECK(hipEventRecord(e1, s));
             HIP_CHECK(hipEventSynchronize(e1));
             float ms = 0.f;
             HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
             if (ms < best) {
                 best = ms;
-                best_cfg = cfg;
+                comm_kind64 = kind;
             }
         }
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         HIP_CHECK(hipGetLastError());
-        if (best >= 1e30f) throw EngineError(EDUS_ERR_UNSUPPORTED, "no FFT tile fits a line of " + std::to_string(n));
-        fft_cfg[axis] = best_cfg;
-        tuned_shapes()[key] = best_cfg;
+        tuned_comm()[ckey] = comm_kind64;
     }
+    comm_kind = prec == EDUS_PREC_MIXED ? int(kCommMfma32) : comm_kind64;
+
     HIP_CHECK(hipMemsetAsync(G, 0, plane_bytes(), s));
+    HIP_CHECK(hipMemsetAsync(f, 0, plane_bytes(), s));
     HIP_CHECK(hipStreamSynchronize(s));
 }
 
@@ -408,7 +516,7 @@ void Engine::evaluate(const double2* in, const double E[3], const StageUpdate& u
     c.up = up;
     TimedLaunch t;
     tbegin("comm", t);
-    launch_comm(nb, layout, c, s);
+    launch_comm(nb, layout, comm_kind, c, s);
     tend(t);
 }
 
@@ -605,6 +713,56 @@ int edus_synchronize(edus_ctx* ctx)
 {
     EDUS_REQUIRE(ctx);
     return guarded([&] { HIP_CHECK(hipStreamSynchronize(ctx->e->s)); });
+}
+
+int edus_get_tuning(edus_ctx* ctx, edus_tuning* out)
+{
+    EDUS_REQUIRE(ctx && out);
+    const Engine& e = *ctx->e;
+    for (int a = 0; a < 2; ++a) {
+        out->fft_family[a] = e.fft_choice[a].two_step;
+        out->fft_cfg[a] = e.fft_choice[a].cfg;
+        out->fft_lines[a] = e.fft_choice[a].L;
+    }
+    out->comm_kind = e.comm_kind;
+    return EDUS_OK;
+}
+
+int edus_set_tuning(edus_ctx* ctx, const edus_tuning* in)
+{
+    EDUS_REQUIRE(ctx && in);
+    return guarded([&] {
+        Engine& e = *ctx->e;
+        edus::FFTChoice next[2];
+        for (int a = 0; a < 2; ++a) {
+            const int n = a == 0 ? e.N0 : e.N1;
+            const int maxl = edus::two_step_max_lines(n);
+            edus::FFTChoice c;
+            c.two_step = in->fft_family[a];
+            c.cfg = in->fft_cfg[a];
+            c.L = in->fft_lines[a];
+            if (c.two_step != (maxl > 0 ? 1 : 0)) throw EngineError(EDUS_ERR_ARG, "fft_family is fixed by the axis length");
+            int lmax;
+            if (c.two_step) {
+                if (c.cfg < 0 || c.cfg > 1) throw EngineError(EDUS_ERR_ARG, "two-step variant must be 0 or 1");
+                lmax = maxl;
+            } else {
+                if (c.cfg < 0 || c.cfg >= edus::kNumFFTConfigs || edus::fft_lines(c.cfg, n) * n > edus::kFFTConfigs[c.cfg].values)
+                    throw EngineError(EDUS_ERR_ARG, "Stockham configuration does not fit this axis");
+                lmax = edus::fft_lines(c.cfg, n);
+            }
+            if (a == 0) lmax = std::min(lmax, e.N1);
+            if (c.L < 1 || c.L > lmax) throw EngineError(EDUS_ERR_ARG, "fft_lines out of range for this axis");
+            next[a] = c;
+        }
+        if (e.prec == EDUS_PREC_MIXED ? in->comm_kind != edus::kCommMfma32
+                                      : (in->comm_kind != edus::kCommScalar && in->comm_kind != edus::kCommMfma64))
+            throw EngineError(EDUS_ERR_ARG, "comm_kind does not match the context precision");
+        e.fft_choice[0] = next[0];
+        e.fft_choice[1] = next[1];
+        e.comm_kind = in->comm_kind;
+        if (e.prec != EDUS_PREC_MIXED) e.comm_kind64 = in->comm_kind;
+    });
 }
 
 int edus_timing_enable(edus_ctx* ctx, int on)

@@ -64,7 +64,9 @@ class Engine {
     int abidx[5] = {0, 1, 2, 3, 4};
     double2* tw[2] = {nullptr, nullptr};
     FFTFactors fac[2];
-    int fft_cfg[2] = {0, 0};  // autotuned tile configuration per grid axis
+    FFTChoice fft_choice[2];  // autotuned kernel and tile per grid axis
+    int comm_kind = kCommScalar;     // engine of the RHS commutator (precision-dependent)
+    int comm_kind64 = kCommScalar;   // FP64 engine, also used for set-up products
 
     bool timing = false;
     std::vector<TimedLaunch> pending;
@@ -89,8 +91,8 @@ class Engine {
     void fock(const double2* in);
     void build_velocity(double det);
     void autotune();
-    AxisFFTArgs axis_args(int axis, int cfg, int mode, const double2* in, const double2* addend, double2* out,
-                          double coef, double scale, const double* mult) const;
+    AxisFFTArgs axis_args(int axis, const FFTChoice& c, int mode, const double2* in, const double2* addend,
+                          double2* out, double coef, double scale, const double* mult) const;
     void tbegin(const char* cls, TimedLaunch& t);
     void tend(TimedLaunch& t);
 };

@@ -1,0 +1,341 @@
+// Batched 1-D FFTs along one grid axis as two in-register steps around one
+// LDS exchange ("two-step" kernel; same modes and results as the Stockham
+// kernel in fft_axis.hip, which remains for line lengths not listed here).
+//
+// A line of length n = RA * RB, index j = ja*RB + jb, frequency
+// k = ka + RA*kb:
+//   step A  thread (line, jb) holds x[ja*RB + jb], ja < RA, does an RA-point
+//           DFT in registers and multiplies by w_n^(jb*ka);
+//   LDS     one exchange;
+//   step B  thread (line, ka) holds the RB values of row ka, does an RB-point
+//           DFT in registers: X[ka + RA*kb].
+// The derivative sandwich (mode 0) multiplies by i coef R(k) and runs the
+// inverse steps B^-1 (same thread, no exchange), LDS, A^-1, so one line costs
+// two barriers and two LDS round trips.  Mode 3 multiplies by a real field
+// instead (the Hartree-Fock convolution along this axis), mode 1 stops after
+// step B (forward transform), mode 2 starts at step B^-1 (inverse).
+//
+// Every thread owns at most one A task and one B task (L*max(RA,RB) <= NT,
+// checked on the host), so each thread's global loads are all issued before
+// the first arithmetic and the field loads of later phases (addend, multiplier)
+// are issued one phase ahead.
+#include <algorithm>
+
+#include "common.hpp"
+#include "fft_reg.hpp"
+#include "kernels.hpp"
+
+namespace edus {
+
+namespace {
+
+using reg::sfor;
+
+constexpr int kTwoStepThreads = 256;
+
+template <int RA, int RB>
+struct TwoStepShape {
+    static constexpr int n = RA * RB;
+    static constexpr int RBP = RB | 1;             // odd row stride in LDS
+    static constexpr int LS = (RA * RBP) | 1;      // odd line stride in LDS
+};
+
+// EARLY: the addend of the last phase is loaded before the first exchange
+// (latency hidden, more registers) or at the start of the last phase.
+template <int RA, int RB, int NT, int MODE, bool EARLY>
+__global__ void __launch_bounds__(NT) two_step_kernel(AxisFFTArgs a)
+{
+    using Sh = TwoStepShape<RA, RB>;
+    constexpr int n = Sh::n, RBP = Sh::RBP, LS = Sh::LS;
+    extern __shared__ double2 lds[];
+    double2* twl = lds;      // w_n^e, e < n
+    double2* buf = lds + n;  // L lines of LS values
+    const int L = a.L;
+    const int Nk = a.N0 * a.N1;
+    const int tid = threadIdx.x;
+
+    long base, cstride, rowstride;
+    int nlines;
+    if (a.axis == 1) {
+        const long g0 = long(blockIdx.x) * L;
+        nlines = int(min(long(L), long(a.P) * a.N0 - g0));
+        base = g0 * a.N1;
+        cstride = a.N1;
+        rowstride = 1;
+    } else {
+        const int nblk = (a.N1 + L - 1) / L;
+        const int plane = blockIdx.x / nblk;
+        const int first = (blockIdx.x - plane * nblk) * L;
+        nlines = min(L, a.N1 - first);
+        base = long(plane) * Nk + first;
+        cstride = 1;
+        rowstride = a.N1;
+    }
+    // A task: (line, jb); B task: (line, ka).  Along rows the fastest thread
+    // index runs inside a line, along columns across lines (adjacent columns
+    // are adjacent in memory).
+    int lineA, jb, lineB, ka;
+    if (a.axis == 1) {
+        lineA = tid / RB;
+        jb = tid - lineA * RB;
+        lineB = tid / RA;
+        ka = tid - lineB * RA;
+    } else {
+        lineA = tid % L;
+        jb = tid / L;
+        lineB = lineA;
+        ka = jb;
+    }
+    const bool doA = jb < RB && lineA < nlines;
+    const bool doB = ka < RA && lineB < nlines;
+    const long gA = base + lineA * cstride;
+    const long gB = base + lineB * cstride;
+    constexpr int mode = MODE;
+    constexpr bool sandwich = mode == 0 || mode == 3;
+
+    if constexpr (RB == 1) {
+        // the whole line in one thread: no exchange
+        if (!doA) return;
+        double2 v[RA];
+        sfor<RA>([&](auto j_) {
+            constexpr int j = decltype(j_)::value;
+            v[j] = a.in[gA + long(j) * rowstride];
+        });
+        double mul[RA];
+        if constexpr (mode == 3) {
+            sfor<RA>([&](auto j_) {
+                constexpr int j = decltype(j_)::value;
+                mul[j] = a.mult[gA + long(j) * rowstride];
+            });
+        }
+        if constexpr (mode == 2) {
+            reg::rdft<RA, 1>(v);
+        } else {
+            reg::rdft<RA, -1>(v);
+            if constexpr (mode == 0) {
+                sfor<RA>([&](auto k_) {
+                    constexpr int k = decltype(k_)::value;
+                    v[k] = cmul_i(v[k], a.coef * double(gamma_centred(k, RA)));
+                });
+            } else if constexpr (mode == 3) {
+                sfor<RA>([&](auto k_) {
+                    constexpr int k = decltype(k_)::value;
+                    v[k] = cscale(v[k], a.scale * mul[k]);
+                });
+            }
+            if constexpr (sandwich) reg::rdft<RA, 1>(v);
+        }
+        const double sc = sandwich ? 1.0 : a.scale;
+        sfor<RA>([&](auto j_) {
+            constexpr int j = decltype(j_)::value;
+            double2 x = v[j];
+            if constexpr (!sandwich) x = cscale(x, sc);
+            if (a.addend) x = cadd(x, a.addend[gA + long(j) * rowstride]);
+            a.out[gA + long(j) * rowstride] = x;
+        });
+        return;
+    } else {
+        double2* lb = buf + lineB * LS + ka * RBP;  // this B task's row in LDS
+        double2* la = buf + lineA * LS + jb;        // this A task's column in LDS
+
+        if constexpr (mode != 2) {
+            // ---- load (A layout), step A forward
+            double2 v[RA];
+            if (doA) {
+                sfor<RA>([&](auto j_) {
+                    constexpr int j = decltype(j_)::value;
+                    v[j] = a.in[gA + long(j * RB) * rowstride + long(jb) * rowstride];
+                });
+            }
+            for (int i = tid; i < n; i += NT) twl[i] = a.tw[i];
+            __syncthreads();
+            if (doA) {
+                reg::rdft<RA, -1>(v);
+                sfor<RA>([&](auto k_) {
+                    constexpr int k = decltype(k_)::value;
+                    double2 x = v[k];
+                    if (k > 0) x = cmul(x, twl[jb * k]);
+                    la[k * RBP] = x;
+                });
+            }
+        } else {
+            // ---- load (B layout), step B inverse
+            double2 u[RB];
+            if (doB) {
+                sfor<RB>([&](auto k_) {
+                    constexpr int k = decltype(k_)::value;
+                    u[k] = a.in[gB + long(ka) * rowstride + long(k * RA) * rowstride];
+                });
+            }
+            for (int i = tid; i < n; i += NT) twl[i] = a.tw[i];
+            __syncthreads();
+            if (doB) {
+                reg::rdft<RB, 1>(u);
+                sfor<RB>([&](auto j_) {
+                    constexpr int j = decltype(j_)::value;
+                    double2 x = u[j];
+                    if (j > 0) x = cmul(x, cconj(twl[j * ka]));
+                    lb[j] = x;
+                });
+            }
+        }
+
+        // field loads of the next phases, issued before the exchange
+        double mul[RB];
+        if constexpr (mode == 3) {
+            if (doB) {
+                sfor<RB>([&](auto k_) {
+                    constexpr int k = decltype(k_)::value;
+                    mul[k] = a.mult[gB + long(ka) * rowstride + long(k * RA) * rowstride];
+                });
+            }
+        }
+        double2 add[RA];
+        const bool has_add = mode != 1 && a.addend != nullptr;
+        auto load_add = [&]() {
+            if (has_add && doA) {
+                sfor<RA>([&](auto j_) {
+                    constexpr int j = decltype(j_)::value;
+                    add[j] = a.addend[gA + long(j * RB) * rowstride + long(jb) * rowstride];
+                });
+            }
+        };
+        if constexpr (EARLY) load_add();
+        __syncthreads();
+
+        if constexpr (mode != 2) {
+            // ---- step B forward (+ multiply + step B inverse)
+            if (doB) {
+                double2 u[RB];
+                sfor<RB>([&](auto j_) {
+                    constexpr int j = decltype(j_)::value;
+                    u[j] = lb[j];
+                });
+                reg::rdft<RB, -1>(u);
+                if constexpr (mode == 1) {
+                    sfor<RB>([&](auto k_) {
+                        constexpr int k = decltype(k_)::value;
+                        double2 x = cscale(u[k], a.scale);
+                        if (a.addend) x = cadd(x, a.addend[gB + long(ka) * rowstride + long(k * RA) * rowstride]);
+                        a.out[gB + long(ka) * rowstride + long(k * RA) * rowstride] = x;
+                    });
+                } else {
+                    if constexpr (mode == 0) {
+                        sfor<RB>([&](auto k_) {
+                            constexpr int k = decltype(k_)::value;
+                            u[k] = cmul_i(u[k], a.coef * double(gamma_centred(ka + RA * k, n)));
+                        });
+                    } else {
+                        sfor<RB>([&](auto k_) {
+                            constexpr int k = decltype(k_)::value;
+                            u[k] = cscale(u[k], a.scale * mul[k]);
+                        });
+                    }
+                    reg::rdft<RB, 1>(u);
+                    sfor<RB>([&](auto j_) {
+                        constexpr int j = decltype(j_)::value;
+                        double2 x = u[j];
+                        if (j > 0) x = cmul(x, cconj(twl[j * ka]));
+                        lb[j] = x;
+                    });
+                }
+            }
+            if constexpr (mode == 1) return;
+            __syncthreads();
+        }
+
+        // ---- step A inverse, store (A layout)
+        if constexpr (!EARLY) load_add();
+        if (doA) {
+            double2 v[RA];
+            sfor<RA>([&](auto k_) {
+                constexpr int k = decltype(k_)::value;
+                v[k] = la[k * RBP];
+            });
+            reg::rdft<RA, 1>(v);
+            const double sc = a.scale;
+            sfor<RA>([&](auto j_) {
+                constexpr int j = decltype(j_)::value;
+                double2 x = v[j];
+                if constexpr (mode == 2) x = cscale(x, sc);
+                if (has_add) x = cadd(x, add[j]);
+                a.out[gA + long(j * RB) * rowstride + long(jb) * rowstride] = x;
+            });
+        }
+    }
+}
+
+template <int RA, int RB, int MODE, bool EARLY>
+void launch_mode(const AxisFFTArgs& a, int blocks, size_t lds, hipStream_t s)
+{
+    using Sh = TwoStepShape<RA, RB>;
+    auto kern = &two_step_kernel<RA, RB, kTwoStepThreads, MODE, EARLY>;
+    static bool raised = false;
+    if (!raised && lds > 0) {
+        const int maxl = kTwoStepThreads / std::max(RA, RB);
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  int((Sh::n + maxl * Sh::LS) * sizeof(double2)));
+        raised = true;
+    }
+    hipLaunchKernelGGL(kern, dim3(blocks), dim3(kTwoStepThreads), lds, s, a);
+}
+
+template <int RA, int RB>
+void launch_two(const AxisFFTArgs& a, int variant, hipStream_t s)
+{
+    using Sh = TwoStepShape<RA, RB>;
+    int blocks;
+    if (a.axis == 1)
+        blocks = int((long(a.P) * a.N0 + a.L - 1) / a.L);
+    else
+        blocks = a.P * ((a.N1 + a.L - 1) / a.L);
+    const size_t lds = RB == 1 ? 0 : size_t(Sh::n + a.L * Sh::LS) * sizeof(double2);
+    switch (a.mode) {
+        case 0:
+            if (variant == 1)
+                launch_mode<RA, RB, 0, true>(a, blocks, lds, s);
+            else
+                launch_mode<RA, RB, 0, false>(a, blocks, lds, s);
+            break;
+        case 1: launch_mode<RA, RB, 1, false>(a, blocks, lds, s); break;
+        case 2: launch_mode<RA, RB, 2, false>(a, blocks, lds, s); break;
+        default: launch_mode<RA, RB, 3, false>(a, blocks, lds, s); break;
+    }
+}
+
+// Line lengths with a two-step kernel and their split n = RA * RB.
+#define EDUS_TWO_STEP_SIZES(X)                                                                          \
+    X(2, 2, 1) X(3, 3, 1) X(4, 4, 1) X(5, 5, 1) X(6, 6, 1) X(7, 7, 1) X(8, 8, 1) X(9, 9, 1) X(10, 10, 1)    \
+    X(11, 11, 1) X(12, 12, 1) X(13, 13, 1) X(14, 14, 1) X(15, 15, 1) X(16, 16, 1) X(20, 5, 4)             \
+    X(24, 8, 3) X(30, 6, 5) X(32, 8, 4) X(36, 6, 6) X(40, 8, 5) X(48, 8, 6) X(60, 10, 6) X(64, 8, 8)       \
+    X(72, 8, 9) X(80, 8, 10) X(90, 10, 9) X(96, 8, 12) X(100, 10, 10) X(120, 8, 15) X(128, 8, 16)        \
+    X(144, 12, 12) X(160, 16, 10) X(180, 12, 15) X(192, 16, 12) X(240, 16, 15) X(256, 16, 16)
+
+}  // namespace
+
+int two_step_max_lines(int n)
+{
+    switch (n) {
+#define EDUS_X(N, A, B) \
+    case N:             \
+        return (B) == 1 ? kTwoStepThreads : kTwoStepThreads / std::max((A), (B));
+        EDUS_TWO_STEP_SIZES(EDUS_X)
+#undef EDUS_X
+        default: return 0;
+    }
+}
+
+void launch_two_step(const AxisFFTArgs& a, int variant, hipStream_t s)
+{
+    switch (a.n) {
+#define EDUS_X(N, A, B)             \
+    case N:                         \
+        launch_two<(A), (B)>(a, variant, s); \
+        break;
+        EDUS_TWO_STEP_SIZES(EDUS_X)
+#undef EDUS_X
+        default: break;
+    }
+}
+
+}  // namespace edus

@@ -69,6 +69,26 @@ extern const FFTConfig kFFTConfigs[kNumFFTConfigs];
 int fft_lines(int cfg, int n);  // lines per work-group for a line length n
 
 void launch_axis_fft(const AxisFFTArgs& a, int cfg, hipStream_t s);
-void launch_comm(int nb, int layout, const CommArgs& a, hipStream_t s);
+
+// Two-step (register DFT + one LDS exchange) axis FFT: lines per work-group
+// it allows for a line length n (0: no kernel for this n), and its launch.
+// variant 1 loads the derivative's addend one phase ahead.
+int two_step_max_lines(int n);
+void launch_two_step(const AxisFFTArgs& a, int variant, hipStream_t s);
+
+// The kernel family and tile used for one grid axis.  The family follows
+// from the line length alone (two-step where it has a kernel), so results do
+// not depend on timing; the autotuner picks the tile inside the family, and
+// tiles of one family give bit-identical results.
+struct FFTChoice {
+    int two_step = 0;  // 1: two-step kernel, 0: Stockham
+    int cfg = 0;       // Stockham configuration, or two-step variant
+    int L = 1;         // lines per work-group
+};
+void launch_fft(const AxisFFTArgs& a, const FFTChoice& c, hipStream_t s);
+// Engines for [H, rho] in the commutator kernel: FP64 FMAs, FP64 MFMA, or
+// FP32 MFMA with FP32 accumulation (mixed precision; everything else FP64).
+enum CommKind { kCommScalar = 0, kCommMfma64 = 1, kCommMfma32 = 2 };
+void launch_comm(int nb, int layout, int kind, const CommArgs& a, hipStream_t s);
 
 }  // namespace edus

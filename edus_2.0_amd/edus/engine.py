@@ -39,6 +39,16 @@ class _Desc(C.Structure):
     ]
 
 
+class Tuning(C.Structure):
+    """edus_tuning: kernel and tile choices of a context."""
+    _fields_ = [("fft_family", C.c_int * 2), ("fft_cfg", C.c_int * 2), ("fft_lines", C.c_int * 2),
+                ("comm_kind", C.c_int)]
+
+    def as_dict(self):
+        return {"fft_family": list(self.fft_family), "fft_cfg": list(self.fft_cfg),
+                "fft_lines": list(self.fft_lines), "comm_kind": int(self.comm_kind)}
+
+
 class _Laser(C.Structure):
     _fields_ = [
         ("intensity", C.c_double), ("intensity_units", C.c_char * 32),
@@ -74,7 +84,7 @@ class _SimParams(C.Structure):
 EXPORTS = [
     "edus_create", "edus_destroy", "edus_last_error", "edus_set_state", "edus_get_state", "edus_rhs",
     "edus_rk4_steps", "edus_ab_steps", "edus_ab_reset", "edus_observables", "edus_check", "edus_synchronize",
-    "edus_timing_enable", "edus_timing_get", "edus_sim_create", "edus_sim_destroy", "edus_sim_info",
+    "edus_timing_enable", "edus_timing_get", "edus_get_tuning", "edus_set_tuning", "edus_sim_create", "edus_sim_destroy", "edus_sim_info",
     "edus_sim_engine", "edus_sim_field", "edus_sim_time", "edus_sim_propagate", "edus_sim_steps", "edus_sim_get",
 ]
 
@@ -101,6 +111,8 @@ def lib():
         L.edus_synchronize.argtypes = [vp]
         L.edus_timing_enable.argtypes = [vp, C.c_int]
         L.edus_timing_get.argtypes = [vp, C.c_char_p, dp, lp]
+        L.edus_get_tuning.argtypes = [vp, C.POINTER(Tuning)]
+        L.edus_set_tuning.argtypes = [vp, C.POINTER(Tuning)]
         L.edus_sim_create.restype = vp
         L.edus_sim_create.argtypes = [C.POINTER(_SimParams)]
         L.edus_sim_destroy.argtypes = [vp]
@@ -231,6 +243,14 @@ class Engine:
         ms, n = C.c_double(), C.c_long()
         _check(lib().edus_timing_get(self.h, cls.encode(), C.byref(ms), C.byref(n)))
         return ms.value, n.value
+
+    def get_tuning(self) -> Tuning:
+        t = Tuning()
+        _check(lib().edus_get_tuning(self.h, C.byref(t)))
+        return t
+
+    def set_tuning(self, t: Tuning) -> None:
+        _check(lib().edus_set_tuning(self.h, C.byref(t)))
 
 
 def _sim_params(p: dict, precision: int, layout: int) -> _SimParams:

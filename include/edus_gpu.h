@@ -129,6 +129,28 @@ int edus_synchronize(edus_ctx* ctx);
 int edus_timing_enable(edus_ctx* ctx, int on);
 int edus_timing_get(edus_ctx* ctx, const char* kernel_class, double* total_ms, long* count);
 
+/* Kernel and tile choices.  No counterpart in the reference (its FFTW plans
+ * and MKL calls have no tiles); edus_create fills them by timing candidates on
+ * the context's own arrays.  A caller may read them, and force others (for
+ * instance to check that every choice gives the same results):
+ *   fft_family[a]  1: two-step register FFT, 0: LDS Stockham; fixed by the
+ *                  length of grid axis a (not settable)
+ *   fft_cfg[a]     Stockham configuration 0..4, or two-step variant 0..1
+ *   fft_lines[a]   lines per work-group
+ *   comm_kind      0: FP64 FMAs, 1: FP64 MFMA (EDUS_PREC_FP64);
+ *                  2: FP32 MFMA (EDUS_PREC_MIXED, not settable)
+ * Within one family, FFT tiles give bit-identical results; the two FP64
+ * commutator engines differ in summation order only.
+ * edus_set_tuning returns EDUS_ERR_ARG for a choice that does not fit. */
+typedef struct {
+    int fft_family[2];
+    int fft_cfg[2];
+    int fft_lines[2];
+    int comm_kind;
+} edus_tuning;
+int edus_get_tuning(edus_ctx* ctx, edus_tuning* out);
+int edus_set_tuning(edus_ctx* ctx, const edus_tuning* in);
+
 /* ---- reference-format simulation (host set-up mirrored in C++) --------- */
 typedef struct {
     double intensity; char intensity_units[32];

@@ -61,6 +61,11 @@ def _model(name, model_dir, grid, **kw):
 RHS_CASES = [
     ("ipa2_square30", [6, 6]), ("ipa2_square30", [9, 7]), ("ipa2_square30", [30, 30]),
     ("ipa14_hex120", [6, 8]), ("ipa14_hex120", [10, 9]), ("hsex8_hex90", [12, 12]), ("hsex8_hex90", [9, 11]),
+    # the benchmark grids' axis lengths: two-step kernels 120 = 8*15, 96 = 8*12,
+    # 90 = 10*9, 48 = 8*6, 20 = 5*4 (every transform mode with exchange), and
+    # 45, 27 on the Stockham kernel
+    ("ipa2_square30", [120, 120]), ("ipa2_square30", [96, 90]), ("ipa2_square30", [48, 20]),
+    ("hsex8_hex90", [90, 96]), ("hsex8_hex90", [45, 27]),
 ]
 
 

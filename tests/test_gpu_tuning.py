@@ -1,0 +1,145 @@
+"""GPU: kernel and tile choices, and the FP32-MFMA (mixed precision) mode.
+
+- FFT tiles of one kernel family give bit-identical right-hand sides;
+- the two FP64 commutator engines (FMA, MFMA) agree to rounding and both
+  match the oracle at the parity bar;
+- mixed precision (FP32 MFMA commutator, FP64 state) stays within its stated
+  error of the FP64 oracle; the error is printed per case.
+"""
+import itertools
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import hbn_params
+
+import edus
+from edus import models
+
+oracle_py = pytest.importorskip("oracle_py")
+
+pytestmark = pytest.mark.gpu
+
+FULL, HERM = edus.LAYOUT_FULL, edus.LAYOUT_HERMITIAN
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _engine(o, layout, filled, precision=edus.PREC_FP64, coulomb=False):
+    args = dict(H0k=o.get("H0k"), xik=(o.get("rk0"), o.get("rk1"), o.get("rk2")), A=o.lattice(),
+                filledbands=filled, U=o.get("U"), vel=(o.get("Vel0"), o.get("Vel1"), o.get("Vel2")),
+                layout=layout, precision=precision)
+    if coulomb:
+        args.update(coulomb=True, W=o.get("Wscr"), hartree=o.get("hartree"), rho0k=o.rho)
+    return edus.Engine(o.nb, o.N0, o.N1, **args)
+
+
+def _rho(nk, nb, seed, hermitian):
+    rng = np.random.default_rng(seed)
+    a = rng.standard_normal((nk, nb, nb)) + 1j * rng.standard_normal((nk, nb, nb))
+    return 0.5 * (a + a.conj().transpose(0, 2, 1)) if hermitian else a
+
+
+def _upper(f):
+    iu = np.triu_indices(f.shape[1])
+    return f[:, iu[0], iu[1]]
+
+
+def _model(model_dir, name, grid):
+    return models.workload(name, os.path.join(model_dir, f"{name}_{grid[0]}x{grid[1]}"), grid=[grid[0], grid[1], 1])
+
+
+@pytest.mark.parametrize("name,grid,layout", [("ipa2_square30", [40, 48], FULL), ("hsex8_hex90", [45, 27], FULL),
+                                              ("hsex8_hex90", [15, 21], HERM)])
+def test_fft_tiles_give_identical_results(model_dir, name, grid, layout):
+    p = _model(model_dir, name, grid)
+    o = oracle_py.Oracle(p)
+    eng = _engine(o, layout, p["filledbands"], coulomb=bool(p["coulomb"]))
+    base = eng.get_tuning()
+    _say("case", name, grid, layout, "autotuned", base.as_dict())
+    rho = _rho(o.Nk, o.nb, 3, hermitian=(layout == HERM))
+    E = [2e-3, -1e-3, 0.0]
+    ref = eng.rhs(E, rho)
+    tried = 0
+    for cfg0, cfg1, l0, l1 in itertools.product(range(5), range(5), (1, 3, 8, 16, 24), (1, 5, 16, 32)):
+        t = edus.Tuning()
+        t.fft_family[:] = list(base.fft_family)
+        t.fft_cfg[:] = [cfg0, cfg1]
+        t.fft_lines[:] = [l0, l1]
+        t.comm_kind = base.comm_kind
+        try:
+            eng.set_tuning(t)
+        except edus.EdusError:
+            continue  # this tile does not fit the axis
+        tried += 1
+        got = eng.rhs(E, rho)
+        assert np.array_equal(got, ref), t.as_dict()
+    _say("  tiles tried", tried)
+    assert tried >= 4
+
+
+@pytest.mark.parametrize("name,grid", [("ipa2_square30", [12, 15]), ("hsex8_hex90", [9, 11]),
+                                       ("ipa14_hex120", [10, 9]), ("ipa14_hex120", [8, 8])])
+@pytest.mark.parametrize("layout", [FULL, HERM])
+def test_fp64_commutator_engines(model_dir, name, grid, layout):
+    p = _model(model_dir, name, grid)
+    coul = bool(p["coulomb"])
+    if layout == HERM and coul and (grid[0] % 2 == 0 or grid[1] % 2 == 0):
+        pytest.skip("HERMITIAN layout with exchange is exact on odd grids only")
+    o = oracle_py.Oracle(p)
+    eng = _engine(o, layout, p["filledbands"], coulomb=coul)
+    rho = _rho(o.Nk, o.nb, 5, hermitian=(layout == HERM))
+    E = [1e-3, 3e-3, 0.0]
+    f_ref = o.rhs_E(E, rho)
+    scale = max(1.0, np.abs(f_ref).max())
+    outs = {}
+    for kind in (0, 1):
+        t = eng.get_tuning()
+        t.comm_kind = kind
+        eng.set_tuning(t)
+        f = eng.rhs(E, rho)
+        err = np.abs(f - f_ref).max() if layout == FULL else np.abs(_upper(f) - _upper(f_ref)).max()
+        _say(name, grid, "layout", layout, "comm_kind", kind, "max|df| vs oracle", err, "scale", scale)
+        assert err <= 1e-11 * scale
+        outs[kind] = f
+    assert np.abs(outs[0] - outs[1]).max() <= 1e-13 * scale
+
+
+@pytest.mark.parametrize("name,grid", [("ipa2_square30", [12, 15]), ("ipa14_hex120", [10, 9]),
+                                       ("hsex8_hex90", [9, 11])])
+def test_mixed_precision_rhs_error(model_dir, name, grid):
+    """FP32-MFMA commutator: relative RHS error against the FP64 oracle."""
+    p = _model(model_dir, name, grid)
+    o = oracle_py.Oracle(p)
+    eng = _engine(o, FULL, p["filledbands"], precision=edus.PREC_MIXED, coulomb=bool(p["coulomb"]))
+    assert eng.get_tuning().comm_kind == 2
+    rho = o.rho + 0.1 * _rho(o.Nk, o.nb, 9, hermitian=True)  # off the ground state: [H, rho] = O(|H|)
+    E = [5e-3, -2e-3, 0.0]
+    f_ref = o.rhs_E(E, rho)
+    f = eng.rhs(E, rho)
+    rel = np.abs(f - f_ref).max() / np.abs(f_ref).max()
+    _say(name, grid, "mixed precision: max|df| / max|f| =", rel)
+    assert rel <= 2e-6
+
+
+def test_mixed_precision_propagation_error():
+    """hBN, 400 RK4 steps in mixed precision against the FP64 oracle:
+    populations and current, and the density matrix itself."""
+    p = hbn_params()
+    o = oracle_py.Oracle(p)
+    sim = edus.Simulation(p, precision=edus.PREC_MIXED)
+    ref = o.propagate(400)
+    got = sim.propagate(400)
+    drho = np.abs(sim.engine.get_state() - o.rho).max()
+    dpop = np.abs(got["pop_bloch"] - ref["pop_bloch"]).max()
+    vref = np.abs(ref["velocity"]).max()
+    dvel = np.abs(got["velocity"] - ref["velocity"]).max() / vref
+    _say("mixed precision hBN 400 steps: max|d rho|", drho, "max|d pop|", dpop, "rel d current", dvel)
+    assert drho <= 1e-5
+    assert dpop <= 1e-5
+    assert dvel <= 1e-3
