@@ -116,7 +116,8 @@ __device__ inline double2 lds_elem(const double2* s, int r, int c, int kk)
 
 // [H, rho] of k-point kk of the tile on the matrix cores.  Lane l holds, per
 // K step s, A-fragments (row i = l%16, column c = 4s + l/16) and B-fragments
-// (row c, column i); the result element (4*(l/16) + v, l%16) is in slot v.
+// (row c, column i); the result element (l/16 + 4v, l%16) is in slot v of the
+// f64 instruction, (4*(l/16) + v, l%16) in slot v of the f32 one.
 //   Re C = Hr.Rr - Hi.Ri - Rr.Hr + Ri.Hi,   Im C = Hr.Ri + Hi.Rr - Rr.Hi - Ri.Hr
 template <int NB, int LAYOUT, int PS>
 __device__ inline void mfma_commutator64(const double2* sH, const double2* sR, int kk, int lane, dbl4& cre, dbl4& cim)
@@ -177,8 +178,15 @@ __global__ void __launch_bounds__(kCommThreads) comm_kernel(CommArgs a)
     extern __shared__ double2 lds[];
     double2* sR = lds;
     double2* sH = lds + NPL * PS;
-    const int k0 = blockIdx.x * TK;
     const int Nk = a.Nk;
+    // blocks cover the tiles of instance 0, then instance 1, ...
+    const int ntile = (Nk + TK - 1) / TK;
+    const int inst = blockIdx.x / ntile;
+    const int k0 = (blockIdx.x - inst * ntile) * TK;
+    const long ioff = long(inst) * NPL * Nk;
+    double Ev[3];
+#pragma unroll
+    for (int dd = 0; dd < 3; ++dd) Ev[dd] = a.E_tab ? a.E_tab[inst * 3 + dd] : a.E[dd];
     const StageUpdate& u = a.up;
 
     // Load phase.  Element (p, kk) is loaded and later produced by the same
@@ -196,21 +204,21 @@ __global__ void __launch_bounds__(kCommThreads) comm_kernel(CommArgs a)
         const int k = k0 + kk;
         double2 r = make_double2(0., 0.), h = r, d = r, x1 = r, x2 = r;
         if (k < Nk) {
-            const long o = long(p) * Nk + k;
+            const long o = ioff + long(p) * Nk + k;
             r = a.rho[o];
             h = a.H0[o];
 #pragma unroll
             for (int dd = 0; dd < 3; ++dd)
                 if (a.xi[dd]) {
                     const double2 x = a.xi[dd][o];
-                    h.x += a.E[dd] * x.x;
-                    h.y += a.E[dd] * x.y;
+                    h.x += Ev[dd] * x.x;
+                    h.y += Ev[dd] * x.y;
                 }
             if (a.sigma) h = cadd(h, a.sigma[o]);
             if (a.hdiag) {
                 int n, m;
                 decode_plane<NB, LAYOUT>(p, n, m);
-                if (n == m) h = cadd(h, a.hdiag[n]);
+                if (n == m) h = cadd(h, a.hdiag[inst * NB + n]);
             }
             if (a.D) d = a.D[o];
             // (if/else rather than ?: on double2 keeps the values out of scratch)
@@ -264,7 +272,8 @@ __global__ void __launch_bounds__(kCommThreads) comm_kernel(CommArgs a)
             const int kk = wave + (NT / 64) * w;
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const int n = 4 * q + v, m = i;
+                // result rows: f64 16x16x4 puts row q + 4v in slot v, f32 16x16x4 row 4q + v
+                const int n = KIND == kCommMfma64 ? q + 4 * v : 4 * q + v, m = i;
                 if (n < NB && m < NB && (LAYOUT == kFull || n <= m)) {
                     bool cj;
                     const int sl = slot<NB, LAYOUT>(n, m, cj);
@@ -303,7 +312,7 @@ __global__ void __launch_bounds__(kCommThreads) comm_kernel(CommArgs a)
         } else {
             acc = sR[p * PS + kk];
         }
-        const long og = long(p) * Nk + k;
+        const long og = ioff + long(p) * Nk + k;
         // f = D - i * [H, rho]
         const double2 f = make_double2(dv[j].x + acc.y, dv[j].y - acc.x);
 
@@ -342,7 +351,7 @@ __global__ void __launch_bounds__(kCommThreads) comm_kernel(CommArgs a)
             double2 tr = make_double2(-u.nf, 0.);
             for (int n = 0; n < NB; ++n) {
                 bool c;
-                const double2 v = ys[long(slot<NB, LAYOUT>(n, n, c)) * Nk + k];
+                const double2 v = ys[ioff + long(slot<NB, LAYOUT>(n, n, c)) * Nk + k];
                 tr = cadd(tr, v);
                 mh = fmax(mh, 2.0 * fabs(v.y));
             }
@@ -363,8 +372,8 @@ __global__ void __launch_bounds__(kCommThreads) comm_kernel(CommArgs a)
                     ++n;
                 }
                 const int m = n + 1 + r;
-                const double2 x = ys[long(n * NB + m) * Nk + k];
-                const double2 z = ys[long(m * NB + n) * Nk + k];
+                const double2 x = ys[ioff + long(n * NB + m) * Nk + k];
+                const double2 z = ys[ioff + long(m * NB + n) * Nk + k];
                 const double d = sqrt((x.x - z.x) * (x.x - z.x) + (x.y + z.y) * (x.y + z.y));
                 mh = (d != d) ? __builtin_huge_val() : fmax(mh, d);
             }
@@ -398,7 +407,7 @@ void launch_kind(const CommArgs& a, hipStream_t s)
                                   int(lds));
         raised = true;
     }
-    hipLaunchKernelGGL(kern, dim3((a.Nk + TK - 1) / TK), dim3(kCommThreads), lds, s, a);
+    hipLaunchKernelGGL(kern, dim3(((a.Nk + TK - 1) / TK) * std::max(a.B, 1)), dim3(kCommThreads), lds, s, a);
 }
 
 template <int NB>

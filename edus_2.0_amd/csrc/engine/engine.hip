@@ -103,6 +103,11 @@ Engine::Engine(const edus_desc& d, const EngineOptions& opt)
     nf = d.filledbands;
     coulomb = d.coulomb != 0;
     std::memcpy(A, d.A, sizeof(A));
+    {
+        std::array<double, 9> a;
+        std::memcpy(a.data(), d.A, sizeof(d.A));
+        Ainst.push_back(a);
+    }
     if (nb < 1 || nb > kMaxNb) throw EngineError(EDUS_ERR_UNSUPPORTED, "nb must be in 1..16");
     if (N0 < 1 || N1 < 1) throw EngineError(EDUS_ERR_ARG, "grid sizes must be positive");
     if (N0 > 2048 || N1 > 2048) throw EngineError(EDUS_ERR_UNSUPPORTED, "grid axes longer than 2048 are not supported");
@@ -118,7 +123,6 @@ Engine::Engine(const edus_desc& d, const EngineOptions& opt)
     const size_t full = size_t(nb) * nb * Nk * sizeof(double2);
     for (double2** p : {&y, &acc, &aux, &G, &D, &f, &H0}) alloc(reinterpret_cast<void**>(p), pb);
     alloc(reinterpret_cast<void**>(&stage), full);
-    for (int i = 0; i < 5; ++i) alloc(reinterpret_cast<void**>(&hist[i]), pb);
     alloc(reinterpret_cast<void**>(&mon), 2 * sizeof(unsigned long long));
     const int nv = 2 * nb + 6;
     alloc(reinterpret_cast<void**>(&obs_partial), size_t(observables_blocks(Nk)) * nv * sizeof(double));
@@ -188,9 +192,8 @@ void Engine::build_velocity(double det)
         if (!vel[d]) alloc(reinterpret_cast<void**>(&vel[d]), pb);
         HIP_CHECK(hipMemsetAsync(aux, 0, pb, s));
         if (xi[d]) launch_axpy(aux, xi[d], -1.0, long(npl) * Nk, s);  // aux = -r_d
-        double e[3] = {0., 0., 0.};
-        e[d] = 1.0;
-        const double2* Dp = gradient(H0, e);
+        const double cu[2] = {A[d][0] / N0, A[d][1] / N1};  // unit field along d
+        const double2* Dp = gradient(H0, cu, nullptr, cu[0] != 0.0, cu[1] != 0.0);
         HIP_CHECK(hipMemsetAsync(f, 0, pb, s));  // zero base for the scaled store
         CommArgs c{};
         c.Nk = Nk;
@@ -207,15 +210,16 @@ void Engine::build_velocity(double det)
     HIP_CHECK(hipStreamSynchronize(s));
 }
 
-void Engine::observables(double* pop_bloch, double* pop_wannier, double* vel_out)
+void Engine::observables(double* pop_bloch, double* pop_wannier, double* vel_out, int inst)
 {
+    const size_t io = size_t(inst) * inst_elems();
     ObsArgs a{};
     a.nb = nb;
     a.Nk = Nk;
     a.layout = layout;
-    a.rho = y;
-    a.U = U;
-    for (int d = 0; d < 3; ++d) a.vel[d] = vel[d];
+    a.rho = y + io;
+    a.U = U ? U + size_t(inst) * nb * nb * Nk : nullptr;
+    for (int d = 0; d < 3; ++d) a.vel[d] = vel[d] ? vel[d] + io : nullptr;
     a.partial = obs_partial;
     launch_observables(a, obs_out, s);
     const int nv = 2 * nb + 6;
@@ -232,6 +236,7 @@ void Engine::observables(double* pop_bloch, double* pop_wannier, double* vel_out
 
 Engine::~Engine()
 {
+    if (self_batch) delete_batch(self_batch);  // its graph reads this engine's arrays
     if (s) (void)hipStreamSynchronize(s);
     for (auto& e : pending) {
         (void)hipEventDestroy(e.a);
@@ -294,7 +299,7 @@ AxisFFTArgs Engine::axis_args(int axis, const FFTChoice& c, int mode, const doub
     a.in = in;
     a.addend = addend;
     a.out = out;
-    a.P = npl;
+    a.P = B * npl;
     a.N0 = N0;
     a.N1 = N1;
     a.axis = axis;
@@ -310,9 +315,14 @@ AxisFFTArgs Engine::axis_args(int axis, const FFTChoice& c, int mode, const doub
 }
 
 void Engine::axis_fft(int axis, int mode, const double2* in, const double2* addend, double2* out, double coef,
-                      double scale, const double* mult, const char* cls)
+                      double scale, const double* mult, const char* cls, const double* coef_tab)
 {
-    const AxisFFTArgs a = axis_args(axis, fft_choice[axis], mode, in, addend, out, coef, scale, mult);
+    AxisFFTArgs a = axis_args(axis, fft_choice[axis], mode, in, addend, out, coef, scale, mult);
+    if (coef_tab) {
+        a.coef_tab = coef_tab;
+        a.coef_stride = 2;
+        a.ppi = npl;
+    }
     TimedLaunch t;
     tbegin(cls, t);
     launch_fft(a, fft_choice[axis], s);
@@ -389,7 +399,7 @@ void Engine::autotune()
         const int n = axis == 0 ? N0 : N1;
         const int other = axis == 0 ? N1 : N0;
         if (n < 2) continue;
-        const auto key = std::make_tuple(axis, n, other, npl);
+        const auto key = std::make_tuple(axis, n, other, B * npl);
         auto it = tuned_shapes().find(key);
         if (it != tuned_shapes().end()) {
             fft_choice[axis] = it->second;
@@ -470,20 +480,20 @@ void Engine::autotune()
 // D = E . grad_k rho  (null when the field has no component along the grid):
 // c_j = a_j . E for lattice vector j, spectral derivative along grid axis j
 // (Functional_SourceTerm.hpp:20-24, kGradient_definitions.hpp:35-41).
-const double2* Engine::gradient(const double2* in, const double E[3])
+const double2* Engine::gradient(const double2* in, const double c[2], const double* ctab, bool use0, bool use1)
 {
-    double c[2];
-    for (int j = 0; j < 2; ++j) c[j] = A[0][j] * E[0] + A[1][j] * E[1] + A[2][j] * E[2];
-    const bool use0 = (N0 > 1) && c[0] != 0.0;
-    const bool use1 = (N1 > 1) && c[1] != 0.0;
+    use0 = use0 && N0 > 1;
+    use1 = use1 && N1 > 1;
     if (!use0 && !use1) return nullptr;
+    const double* t0 = ctab ? ctab + 0 : nullptr;
+    const double* t1 = ctab ? ctab + 1 : nullptr;
     if (use0 && use1) {
-        axis_fft(0, 0, in, nullptr, G, c[0] / N0, 1.0, nullptr, "grad_col");
-        axis_fft(1, 0, in, G, D, c[1] / N1, 1.0, nullptr, "grad_row");
+        axis_fft(0, 0, in, nullptr, G, c[0], 1.0, nullptr, "grad_col", t0);
+        axis_fft(1, 0, in, G, D, c[1], 1.0, nullptr, "grad_row", t1);
     } else if (use0) {
-        axis_fft(0, 0, in, nullptr, D, c[0] / N0, 1.0, nullptr, "grad_col");
+        axis_fft(0, 0, in, nullptr, D, c[0], 1.0, nullptr, "grad_col", t0);
     } else {
-        axis_fft(1, 0, in, nullptr, D, c[1] / N1, 1.0, nullptr, "grad_row");
+        axis_fft(1, 0, in, nullptr, D, c[1], 1.0, nullptr, "grad_row", t1);
     }
     return D;
 }
@@ -494,12 +504,14 @@ void Engine::fock(const double2* in)
     axis_fft(1, 1, in, nullptr, T, 0.0, 1.0, nullptr, "hf_row");
     axis_fft(0, 3, T, nullptr, T, 0.0, -1.0 / double(Nk), W, "hf_col");
     axis_fft(1, 2, T, nullptr, T, 0.0, 1.0, nullptr, "hf_row");
-    launch_hartree(in, nb, Nk, layout, hart, diag_partial, hdiag, s);
+    launch_hartree(in, nb, Nk, layout, npl, B, hart, diag_partial, hdiag, s);
 }
 
 void Engine::evaluate(const double2* in, const double E[3], const StageUpdate& up)
 {
-    const double2* Dp = gradient(in, E);
+    double cg[2];
+    for (int j = 0; j < 2; ++j) cg[j] = (A[0][j] * E[0] + A[1][j] * E[1] + A[2][j] * E[2]) / (j == 0 ? N0 : N1);
+    const double2* Dp = gradient(in, cg, nullptr, cg[0] != 0.0, cg[1] != 0.0);
     if (coulomb) fock(in);
     CommArgs c{};
     c.Nk = Nk;
@@ -553,8 +565,166 @@ void Engine::rk4_step(double dt, const double* E3x3)
     evaluate(aux, E3x3 + 6, u);
 }
 
+Engine::Engine(const std::vector<const Engine*>& members)
+{
+    if (members.empty()) throw EngineError(EDUS_ERR_ARG, "empty batch group");
+    const Engine& m0 = *members[0];
+    nb = m0.nb;
+    N0 = m0.N0;
+    N1 = m0.N1;
+    Nk = m0.Nk;
+    layout = m0.layout;
+    prec = m0.prec;
+    nf = m0.nf;
+    coulomb = m0.coulomb;
+    npl = m0.npl;
+    B = int(members.size());
+    std::memcpy(A, m0.A, sizeof(A));
+    for (const Engine* m : members) {
+        if (m->nb != nb || m->N0 != N0 || m->N1 != N1 || m->layout != layout || m->prec != prec ||
+            m->coulomb != coulomb || m->nf != nf || m->B != 1)
+            throw EngineError(EDUS_ERR_ARG, "batch group members differ in shape");
+        std::array<double, 9> a;
+        std::memcpy(a.data(), m->A, sizeof(m->A));
+        Ainst.push_back(a);
+        HIP_CHECK(hipStreamSynchronize(m->s));
+    }
+
+    HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    const size_t pb = plane_bytes();
+    const size_t ib = inst_elems() * sizeof(double2);
+    const size_t full = size_t(nb) * nb * Nk * sizeof(double2);
+    for (double2** p : {&y, &acc, &aux, &G, &D, &f, &H0}) alloc(reinterpret_cast<void**>(p), pb);
+    alloc(reinterpret_cast<void**>(&stage), full);
+    alloc(reinterpret_cast<void**>(&mon), 2 * sizeof(unsigned long long));
+    const int nv = 2 * nb + 6;
+    alloc(reinterpret_cast<void**>(&obs_partial), size_t(observables_blocks(Nk)) * nv * sizeof(double));
+    alloc(reinterpret_cast<void**>(&obs_out), size_t(nv) * sizeof(double));
+
+    // stacked copies of every member's arrays (zero where a member has none)
+    auto stack = [&](double2* dst, auto get, size_t bytes) {
+        for (int i = 0; i < B; ++i) {
+            const void* src = get(members[i]);
+            if (src) HIP_CHECK(hipMemcpyAsync(reinterpret_cast<char*>(dst) + i * bytes, src, bytes,
+                                              hipMemcpyDeviceToDevice, s));
+        }
+    };
+    stack(y, [](const Engine* m) { return m->y; }, ib);
+    stack(H0, [](const Engine* m) { return m->H0; }, ib);
+    for (int d = 0; d < 3; ++d) {
+        bool any_xi = false, any_vel = false;
+        for (const Engine* m : members) {
+            any_xi |= m->xi[d] != nullptr;
+            any_vel |= m->vel[d] != nullptr;
+        }
+        if (any_xi) {
+            alloc(reinterpret_cast<void**>(&xi[d]), pb);
+            stack(xi[d], [d](const Engine* m) { return m->xi[d]; }, ib);
+        }
+        if (any_vel) {
+            alloc(reinterpret_cast<void**>(&vel[d]), pb);
+            stack(vel[d], [d](const Engine* m) { return m->vel[d]; }, ib);
+        }
+    }
+    bool all_u = true;
+    for (const Engine* m : members) all_u &= m->U != nullptr;
+    if (all_u) {
+        alloc(reinterpret_cast<void**>(&U), size_t(B) * full);
+        stack(U, [](const Engine* m) { return m->U; }, full);
+    }
+    fac[0] = m0.fac[0];
+    fac[1] = m0.fac[1];
+    for (int ax = 0; ax < 2; ++ax) {
+        const int n = ax == 0 ? N0 : N1;
+        alloc(reinterpret_cast<void**>(&tw[ax]), n * sizeof(double2));
+        HIP_CHECK(hipMemcpyAsync(tw[ax], m0.tw[ax], n * sizeof(double2), hipMemcpyDeviceToDevice, s));
+    }
+    if (coulomb) {
+        alloc(reinterpret_cast<void**>(&T), pb);
+        alloc(reinterpret_cast<void**>(&W), size_t(B) * npl * Nk * sizeof(double));
+        alloc(reinterpret_cast<void**>(&hart), size_t(B) * nb * nb * sizeof(double));
+        alloc(reinterpret_cast<void**>(&hdiag), size_t(B) * nb * sizeof(double2));
+        alloc(reinterpret_cast<void**>(&diag_partial), size_t(B) * nb * diag_blocks(Nk) * sizeof(double2));
+        for (int i = 0; i < B; ++i) {
+            HIP_CHECK(hipMemcpyAsync(W + size_t(i) * npl * Nk, members[i]->W, size_t(npl) * Nk * sizeof(double),
+                                     hipMemcpyDeviceToDevice, s));
+            HIP_CHECK(hipMemcpyAsync(hart + size_t(i) * nb * nb, members[i]->hart, size_t(nb) * nb * sizeof(double),
+                                     hipMemcpyDeviceToDevice, s));
+        }
+    }
+    HIP_CHECK(hipStreamSynchronize(s));
+    autotune();
+    comm_kind = m0.comm_kind;
+    comm_kind64 = m0.comm_kind64;
+    HIP_CHECK(hipStreamSynchronize(s));
+    HIP_CHECK(hipGetLastError());
+}
+
+void Engine::ensure_tables(int steps)
+{
+    if (steps <= tab_steps) return;
+    // grown tables replace the old ones; the old allocations stay owned
+    // (freed with the engine) because queued work may still read them
+    alloc(reinterpret_cast<void**>(&E_dev), size_t(steps) * 3 * B * 3 * sizeof(double));
+    alloc(reinterpret_cast<void**>(&coef_dev), size_t(steps) * 3 * B * 2 * sizeof(double));
+    tab_steps = steps;
+}
+
+void Engine::evaluate_tab(const double2* in, int ts, int j, const bool use_axis[2], const bool use_xi[3],
+                          const StageUpdate& up)
+{
+    const size_t row = size_t(ts) * 3 + j;
+    const double c0[2] = {0.0, 0.0};
+    const double2* Dp = gradient(in, c0, coef_dev + row * B * 2, use_axis[0], use_axis[1]);
+    if (coulomb) fock(in);
+    CommArgs c{};
+    c.Nk = Nk;
+    c.P = npl;
+    c.B = B;
+    c.rho = in;
+    c.D = Dp;
+    c.H0 = H0;
+    for (int d = 0; d < 3; ++d) c.xi[d] = (xi[d] && use_xi[d]) ? xi[d] : nullptr;
+    c.E_tab = E_dev + row * B * 3;
+    c.sigma = coulomb ? T : nullptr;
+    c.hdiag = coulomb ? hdiag : nullptr;
+    c.up = up;
+    TimedLaunch t;
+    tbegin("comm", t);
+    launch_comm(nb, layout, comm_kind, c, s);
+    tend(t);
+}
+
+void Engine::rk4_step_tab(double dt, int ts, const bool use_axis[2], const bool use_xi[3])
+{
+    // the stage sequence of rk4_step (DESolver.hpp:155-187)
+    StageUpdate u{};
+    u.x1 = y; u.out1 = acc; u.b1 = dt / 6.;
+    u.x2 = y; u.out2 = aux; u.b2 = dt / 2.;
+    evaluate_tab(y, ts, 0, use_axis, use_xi, u);
+    u = StageUpdate{};
+    u.x1 = acc; u.out1 = acc; u.b1 = dt / 3.;
+    u.x2 = y; u.out2 = aux; u.b2 = dt / 2.;
+    evaluate_tab(aux, ts, 1, use_axis, use_xi, u);
+    u = StageUpdate{};
+    u.x1 = acc; u.out1 = acc; u.b1 = dt / 3.;
+    u.x2 = y; u.out2 = aux; u.b2 = dt;
+    evaluate_tab(aux, ts, 1, use_axis, use_xi, u);
+    u = StageUpdate{};
+    u.x1 = acc; u.out1 = y; u.b1 = dt / 6.;
+    u.mon = mon; u.mon_src = y; u.nf = double(nf);
+    evaluate_tab(aux, ts, 2, use_axis, use_xi, u);
+}
+
+void Engine::ensure_hist()
+{
+    for (int i = 0; i < 5; ++i)
+        if (!hist[i]) alloc(reinterpret_cast<void**>(&hist[i]), plane_bytes());
+}
+
 void Engine::ab_step(double dt, int order, const double* E3)
 {
+    ensure_hist();
     // DESolver.hpp:110-126 coefficients, 202-220 update
     static const double b4[5] = {55. / 24., -59. / 24., 37. / 24., -3. / 8., 0.};
     static const double b5[5] = {1901. / 720., -2774. / 720., 2616. / 720., -1274. / 720., 251. / 720.};
@@ -579,28 +749,7 @@ void Engine::ab_step(double dt, int order, const double* E3)
 using edus::Engine;
 using edus::EngineError;
 
-template <class F>
-static int guarded(F&& fn)
-{
-    try {
-        fn();
-        return EDUS_OK;
-    } catch (EngineError& e) {
-        edus::set_error(e.what());
-        return e.code;
-    } catch (std::exception& e) {
-        edus::set_error(e.what());
-        return EDUS_ERR_STATE;
-    }
-}
-
-#define EDUS_REQUIRE(cond)                         \
-    do {                                           \
-        if (!(cond)) {                             \
-            edus::set_error("null argument");      \
-            return EDUS_ERR_ARG;                   \
-        }                                          \
-    } while (0)
+using edus::guarded;
 
 extern "C" {
 
@@ -614,6 +763,18 @@ edus_ctx* edus_create(const edus_desc* desc)
     }
     edus_ctx* c = nullptr;
     int rc = guarded([&] { c = new edus_ctx{new Engine(*desc)}; });
+    return rc == EDUS_OK ? c : nullptr;
+}
+
+edus_ctx* edus_clone(edus_ctx* src)
+{
+    if (!src || !src->e) {
+        edus::set_error("null context");
+        return nullptr;
+    }
+    edus_ctx* c = nullptr;
+    const std::vector<const Engine*> one{src->e};
+    int rc = guarded([&] { c = new edus_ctx{new Engine(one)}; });
     return rc == EDUS_OK ? c : nullptr;
 }
 
@@ -655,7 +816,7 @@ int edus_rk4_steps(edus_ctx* ctx, double dt, int nsteps, const double* E)
     EDUS_REQUIRE(ctx && (nsteps <= 0 || E));
     if (nsteps < 0) { edus::set_error("nsteps < 0"); return EDUS_ERR_ARG; }
     return guarded([&] {
-        for (int s = 0; s < nsteps; ++s) ctx->e->rk4_step(dt, E + 9 * size_t(s));
+        ctx->e->rk4_steps(dt, nsteps, E);
         HIP_CHECK(hipGetLastError());
     });
 }
@@ -680,7 +841,7 @@ int edus_ab_reset(edus_ctx* ctx)
     return guarded([&] {
         Engine& e = *ctx->e;
         for (int i = 0; i < 5; ++i) {
-            HIP_CHECK(hipMemsetAsync(e.hist[i], 0, e.plane_bytes(), e.s));
+            if (e.hist[i]) HIP_CHECK(hipMemsetAsync(e.hist[i], 0, e.plane_bytes(), e.s));
             e.abidx[i] = i;
         }
         HIP_CHECK(hipStreamSynchronize(e.s));
@@ -707,6 +868,21 @@ int edus_check(edus_ctx* ctx, double* trace_dev, double* herm_dev)
         if (trace_dev) *trace_dev = v[0];
         if (herm_dev) *herm_dev = v[1];
     });
+}
+
+int edus_graph(edus_ctx* ctx, int steps_per_graph)
+{
+    EDUS_REQUIRE(ctx);
+    if (steps_per_graph < 0) {
+        edus::set_error("steps_per_graph < 0");
+        return EDUS_ERR_ARG;
+    }
+    if (ctx->e->B != 1) {
+        edus::set_error("graph replay applies to single contexts");
+        return EDUS_ERR_ARG;
+    }
+    ctx->e->graph_steps = steps_per_graph;
+    return EDUS_OK;
 }
 
 int edus_synchronize(edus_ctx* ctx)

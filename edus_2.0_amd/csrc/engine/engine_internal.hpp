@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <array>
 #include <map>
 #include <stdexcept>
 #include <string>
@@ -41,16 +42,31 @@ struct EngineOptions {
     double velocity_det = 0.0;
 };
 
+class Batch;
+
 class Engine {
   public:
     explicit Engine(const edus_desc& d, const EngineOptions& opt = EngineOptions());
+    // A batch group: the members' set-up arrays and states stacked along the
+    // plane axis ([B][npl][Nk] in every per-k array).  All members share nb,
+    // grid, layout, precision, exchange on/off and filledbands.
+    explicit Engine(const std::vector<const Engine*>& members);
     ~Engine();
     Engine(const Engine&) = delete;
     Engine& operator=(const Engine&) = delete;
 
     int nb, N0, N1, Nk, npl, layout, prec, nf;
+    int B = 1;  // stacked instances
     bool coulomb;
     double A[3][3];
+    std::vector<std::array<double, 9>> Ainst;  // lattice of each stacked instance
+
+    // Field tables of stacked instances, device-resident for `tab_steps`
+    // steps: E [step][j][B][3] and gradient coefficients c_axis / N_axis
+    // [step][j][B][2] at the stage times t, t + dt/2, t + dt (j = 0, 1, 2).
+    double* E_dev = nullptr;
+    double* coef_dev = nullptr;
+    int tab_steps = 0;
     hipStream_t s = nullptr;
 
     double2 *y = nullptr, *acc = nullptr, *aux = nullptr, *G = nullptr, *D = nullptr, *f = nullptr;
@@ -72,22 +88,37 @@ class Engine {
     std::vector<TimedLaunch> pending;
     std::map<std::string, std::pair<double, long>> tsum;
 
-    size_t plane_bytes() const { return size_t(npl) * Nk * sizeof(double2); }
+    size_t inst_elems() const { return size_t(npl) * Nk; }                 // one instance's field
+    size_t plane_bytes() const { return size_t(B) * npl * Nk * sizeof(double2); }  // all instances
     void upload(const double* host_full, double2* dst);       // [Nk][nb][nb] -> planes
     void download(const double2* src, double* host_full);     // planes -> [Nk][nb][nb]
     void evaluate(const double2* in, const double E[3], const StageUpdate& up);
     void rk4_step(double dt, const double* E3x3);
+    // stacked instances: fields from the device tables at table step `ts`;
+    // use_axis / use_xi switch off gradient axes and E.xi terms that are zero
+    // for every instance of the chunk
+    void ensure_tables(int steps);
+    void evaluate_tab(const double2* in, int ts, int j, const bool use_axis[2], const bool use_xi[3],
+                      const StageUpdate& up);
+    void rk4_step_tab(double dt, int ts, const bool use_axis[2], const bool use_xi[3]);
+    // nsteps RK4 steps, E[(s * 3 + j) * 3 + d]; with graph_steps > 0 the steps
+    // are replayed from a hipGraph of graph_steps steps (self_batch)
+    void rk4_steps(double dt, int nsteps, const double* E);
+    int graph_steps = 0;
+    Batch* self_batch = nullptr;
     void ab_step(double dt, int order, const double* E3);
     void monitor(const double2* rho);
-    void observables(double* pop_bloch, double* pop_wannier, double* vel);
+    void observables(double* pop_bloch, double* pop_wannier, double* vel, int inst = 0);
     void collect_timing();
 
   private:
     std::vector<void*> owned;
     void alloc(void** p, size_t bytes);
     void axis_fft(int axis, int mode, const double2* in, const double2* addend, double2* out, double coef,
-                  double scale, const double* mult, const char* cls);
-    const double2* gradient(const double2* in, const double E[3]);
+                  double scale, const double* mult, const char* cls, const double* coef_tab = nullptr);
+    // D = E.grad rho with c[j] = c_j / N_j, or per instance from ctab [B][2]
+    const double2* gradient(const double2* in, const double c[2], const double* ctab, bool use0, bool use1);
+    void ensure_hist();
     void fock(const double2* in);
     void build_velocity(double det);
     void autotune();
@@ -97,7 +128,33 @@ class Engine {
     void tend(TimedLaunch& t);
 };
 
+void delete_batch(Batch* b);
+
+// Runs fn, turning exceptions into the C ABI's status codes and message.
+template <class F>
+int guarded(F&& fn)
+{
+    try {
+        fn();
+        return EDUS_OK;
+    } catch (EngineError& e) {
+        set_error(e.what());
+        return e.code;
+    } catch (std::exception& e) {
+        set_error(e.what());
+        return EDUS_ERR_STATE;
+    }
+}
+
 }  // namespace edus
+
+#define EDUS_REQUIRE(cond)                         \
+    do {                                           \
+        if (!(cond)) {                             \
+            edus::set_error("null argument");      \
+            return EDUS_ERR_ARG;                   \
+        }                                          \
+    } while (0)
 
 // The opaque handle of the C ABI.
 struct edus_ctx {

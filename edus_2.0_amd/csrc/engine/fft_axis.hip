@@ -196,19 +196,20 @@ __global__ void __launch_bounds__(NT) axis_fft_kernel(AxisFFTArgs a)
 
     // global offset of value j of line c: base + c * cstride + j * rowstride
     long base, cstride, rowstride;
+    long row0 = 0, plane0 = 0;  // first global row (axis 1), plane (axis 0)
     int nlines;
     if (a.axis == 1) {
-        const long g0 = long(blockIdx.x) * L;
-        nlines = int(min(long(L), long(a.P) * a.N0 - g0));
-        base = g0 * a.N1;
+        row0 = long(blockIdx.x) * L;
+        nlines = int(min(long(L), long(a.P) * a.N0 - row0));
+        base = row0 * a.N1;
         cstride = a.N1;
         rowstride = 1;
     } else {
         const int nblk = (a.N1 + L - 1) / L;
-        const int plane = blockIdx.x / nblk;
-        const int first = (blockIdx.x - plane * nblk) * L;
+        plane0 = blockIdx.x / nblk;
+        const int first = int(blockIdx.x - plane0 * nblk) * L;
         nlines = min(L, a.N1 - first);
-        base = long(plane) * Nk + first;
+        base = plane0 * Nk + first;
         cstride = 1;
         rowstride = a.N1;
     }
@@ -241,8 +242,10 @@ __global__ void __launch_bounds__(NT) axis_fft_kernel(AxisFFTArgs a)
         double2* x = run_fft<NT>(bufA, bufB, nlines, n, a.fac, twl, -1.0);
         if (a.mode == 0) {
             for (int idx = threadIdx.x; idx < nlines * n; idx += NT) {
-                const int j = idx % n;
-                x[idx] = cmul_i(x[idx], a.coef * double(gamma_centred(j, n)));
+                const int c = idx / n;
+                const int j = idx - c * n;
+                const double coef = line_coef(a, a.axis == 1 ? (row0 + c) / a.N0 : plane0);
+                x[idx] = cmul_i(x[idx], coef * double(gamma_centred(j, n)));
             }
         } else {
             double mul[PER];

@@ -55,19 +55,20 @@ __global__ void __launch_bounds__(NT) two_step_kernel(AxisFFTArgs a)
     const int tid = threadIdx.x;
 
     long base, cstride, rowstride;
+    long row0 = 0, plane0 = 0;  // first global row (axis 1), plane (axis 0)
     int nlines;
     if (a.axis == 1) {
-        const long g0 = long(blockIdx.x) * L;
-        nlines = int(min(long(L), long(a.P) * a.N0 - g0));
-        base = g0 * a.N1;
+        row0 = long(blockIdx.x) * L;
+        nlines = int(min(long(L), long(a.P) * a.N0 - row0));
+        base = row0 * a.N1;
         cstride = a.N1;
         rowstride = 1;
     } else {
         const int nblk = (a.N1 + L - 1) / L;
-        const int plane = blockIdx.x / nblk;
-        const int first = (blockIdx.x - plane * nblk) * L;
+        plane0 = blockIdx.x / nblk;
+        const int first = int(blockIdx.x - plane0 * nblk) * L;
         nlines = min(L, a.N1 - first);
-        base = long(plane) * Nk + first;
+        base = plane0 * Nk + first;
         cstride = 1;
         rowstride = a.N1;
     }
@@ -113,9 +114,10 @@ __global__ void __launch_bounds__(NT) two_step_kernel(AxisFFTArgs a)
         } else {
             reg::rdft<RA, -1>(v);
             if constexpr (mode == 0) {
+                const double coef = line_coef(a, a.axis == 1 ? (row0 + lineA) / a.N0 : plane0);
                 sfor<RA>([&](auto k_) {
                     constexpr int k = decltype(k_)::value;
-                    v[k] = cmul_i(v[k], a.coef * double(gamma_centred(k, RA)));
+                    v[k] = cmul_i(v[k], coef * double(gamma_centred(k, RA)));
                 });
             } else if constexpr (mode == 3) {
                 sfor<RA>([&](auto k_) {
@@ -221,9 +223,10 @@ __global__ void __launch_bounds__(NT) two_step_kernel(AxisFFTArgs a)
                     });
                 } else {
                     if constexpr (mode == 0) {
+                        const double coef = line_coef(a, a.axis == 1 ? (row0 + lineB) / a.N0 : plane0);
                         sfor<RB>([&](auto k_) {
                             constexpr int k = decltype(k_)::value;
-                            u[k] = cmul_i(u[k], a.coef * double(gamma_centred(ka + RA * k, n)));
+                            u[k] = cmul_i(u[k], coef * double(gamma_centred(ka + RA * k, n)));
                         });
                     } else {
                         sfor<RB>([&](auto k_) {

@@ -21,6 +21,10 @@ struct AxisFFTArgs {
     int L;                  // lines per work-group
     int mode;               // 0 derivative, 1 forward, 2 inverse, 3 convolution
     double coef;            // derivative: c_axis / n
+    // derivative of stacked instances: plane p uses coef_tab[(p / ppi) * coef_stride]
+    // (instance p / ppi, ppi = planes per instance) instead of coef
+    const double* coef_tab;
+    int coef_stride, ppi;
     double scale;           // transforms: output scale; convolution: factor on mult
     const double* mult;     // convolution: real multiplier field, triangle planes [P][N0][N1]
     FFTFactors fac;
@@ -50,15 +54,23 @@ struct StageUpdate {
 
 struct CommArgs {
     int Nk, P;
+    int B;                    // instances stacked as [B][P][Nk] in every per-k array (0 or 1: one)
     const double2* rho;       // stage input, planes
     const double2* D;         // E.grad rho part (may be null)
     const double2* H0;
     const double2* xi[3];     // null where E_d == 0 or operator absent
     double E[3];
+    const double* E_tab;      // per-instance field [B][3] on device, replaces E when set
     const double2* sigma;     // Hartree-Fock self energy in k (may be null)
-    const double2* hdiag;     // Hartree diagonal shift, nb complex on device (may be null)
+    const double2* hdiag;     // Hartree diagonal shift, [B][nb] complex on device (may be null)
     StageUpdate up;
 };
+
+// derivative coefficient of the line in plane `plane`
+__device__ inline double line_coef(const AxisFFTArgs& a, long plane)
+{
+    return a.coef_tab ? a.coef_tab[(plane / a.ppi) * a.coef_stride] : a.coef;
+}
 
 // Axis-FFT tile configurations (threads per work-group, values per work-group)
 struct FFTConfig {

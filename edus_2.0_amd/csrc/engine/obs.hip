@@ -143,11 +143,12 @@ __global__ void reduce_partials_kernel(const double* __restrict__ partial, int n
 
 // Per-block partial sums of the diagonal planes (for the Hartree k-mean).
 __global__ void __launch_bounds__(kObsBlock) diag_sum_kernel(const double2* __restrict__ rho, int nb, int Nk,
-                                                             int layout, double2* __restrict__ partial)
+                                                             int layout, int P, double2* __restrict__ partial)
 {
     __shared__ double2 sh[kObsBlock];
     const int n = blockIdx.y;
-    const double2* plane = rho + long(diag_plane(n, nb, layout)) * Nk;
+    const int inst = blockIdx.z;  // stacked instances [B][P][Nk]
+    const double2* plane = rho + (long(inst) * P + diag_plane(n, nb, layout)) * Nk;
     double2 s = make_double2(0., 0.);
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < Nk; k += gridDim.x * blockDim.x) s = cadd(s, plane[k]);
     sh[threadIdx.x] = s;
@@ -156,17 +157,21 @@ __global__ void __launch_bounds__(kObsBlock) diag_sum_kernel(const double2* __re
         if (threadIdx.x < st) sh[threadIdx.x] = cadd(sh[threadIdx.x], sh[threadIdx.x + st]);
         __syncthreads();
     }
-    if (threadIdx.x == 0) partial[long(n) * gridDim.x + blockIdx.x] = sh[0];
+    if (threadIdx.x == 0) partial[(long(inst) * nb + n) * gridDim.x + blockIdx.x] = sh[0];
 }
 
 // hdiag_n = sum_m Hartree_nm * rho_mm(R=0), rho_mm(R=0) = mean_k rho_mm(k)
 // (Coulomb.cpp:150-159; the rho0 part is folded into H0 at set-up).
-// One block; partials [nb][nblk].
+// One block per instance; partials [B][nb][nblk], hart [B][nb][nb], hdiag [B][nb].
 __global__ void hartree_kernel(const double2* __restrict__ partial, int nblk, const double* __restrict__ hart,
                                int nb, double invNk, double2* __restrict__ hdiag)
 {
     __shared__ double2 mean[kMaxNb];
     const int t = threadIdx.x;
+    const int inst = blockIdx.x;
+    partial += long(inst) * nb * nblk;
+    hart += long(inst) * nb * nb;
+    hdiag += long(inst) * nb;
     if (t < nb) {
         double2 s = make_double2(0., 0.);
         for (int b = 0; b < nblk; ++b) s = cadd(s, partial[long(t) * nblk + b]);
@@ -278,12 +283,12 @@ void launch_observables(const ObsArgs& a, double* out, hipStream_t s)
                        out);
 }
 int diag_blocks(int Nk) { return int(std::min<long>(grid1d(Nk, kObsBlock), 64)); }
-void launch_hartree(const double2* rho, int nb, int Nk, int layout, const double* hart, double2* partial,
-                    double2* hdiag, hipStream_t s)
+void launch_hartree(const double2* rho, int nb, int Nk, int layout, int P, int B, const double* hart,
+                    double2* partial, double2* hdiag, hipStream_t s)
 {
     const int nblk = diag_blocks(Nk);
-    hipLaunchKernelGGL(diag_sum_kernel, dim3(nblk, nb), dim3(kObsBlock), 0, s, rho, nb, Nk, layout, partial);
-    hipLaunchKernelGGL(hartree_kernel, dim3(1), dim3(64), 0, s, partial, nblk, hart, nb, 1.0 / double(Nk), hdiag);
+    hipLaunchKernelGGL(diag_sum_kernel, dim3(nblk, nb, B), dim3(kObsBlock), 0, s, rho, nb, Nk, layout, P, partial);
+    hipLaunchKernelGGL(hartree_kernel, dim3(B), dim3(64), 0, s, partial, nblk, hart, nb, 1.0 / double(Nk), hdiag);
 }
 void launch_monitor(const double2* rho, int nb, int Nk, int layout, double nf, unsigned long long* mon, hipStream_t s)
 {

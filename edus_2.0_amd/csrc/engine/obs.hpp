@@ -21,8 +21,9 @@ void launch_full_to_planes(const double2* full, double2* planes, int nb, int Nk,
 int observables_blocks(int Nk);
 void launch_observables(const ObsArgs& a, double* out, hipStream_t s);
 int diag_blocks(int Nk);
-void launch_hartree(const double2* rho, int nb, int Nk, int layout, const double* hart, double2* partial,
-                    double2* hdiag, hipStream_t s);
+// Hartree shift of B stacked instances ([B][P][Nk]); partial [B][nb][diag_blocks]
+void launch_hartree(const double2* rho, int nb, int Nk, int layout, int P, int B, const double* hart,
+                    double2* partial, double2* hdiag, hipStream_t s);
 void launch_monitor(const double2* rho, int nb, int Nk, int layout, double nf, unsigned long long* mon,
                     hipStream_t s);
 void launch_axpy(double2* y, const double2* x, double a, long n, hipStream_t s);

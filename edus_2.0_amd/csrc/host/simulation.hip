@@ -92,10 +92,8 @@ class DESolver {
                 lasers_->field(t + dt, &E[9 * size_t(s) + 6]);
                 t += dt;
             }
-            for (int s = 0; s < nstep; ++s) {
-                engine_->rk4_step(dt, &E[9 * size_t(s)]);
-                CurrentTime += dt;
-            }
+            engine_->rk4_steps(dt, nstep, E.data());
+            for (int s = 0; s < nstep; ++s) CurrentTime += dt;
         } else {
             std::vector<double> E(size_t(nstep) * 3);
             double t = CurrentTime;

@@ -82,9 +82,11 @@ class _SimParams(C.Structure):
 
 
 EXPORTS = [
-    "edus_create", "edus_destroy", "edus_last_error", "edus_set_state", "edus_get_state", "edus_rhs",
-    "edus_rk4_steps", "edus_ab_steps", "edus_ab_reset", "edus_observables", "edus_check", "edus_synchronize",
-    "edus_timing_enable", "edus_timing_get", "edus_get_tuning", "edus_set_tuning", "edus_sim_create", "edus_sim_destroy", "edus_sim_info",
+    "edus_create", "edus_clone", "edus_destroy", "edus_last_error", "edus_set_state", "edus_get_state", "edus_rhs",
+    "edus_rk4_steps", "edus_ab_steps", "edus_ab_reset", "edus_observables", "edus_check", "edus_synchronize", "edus_graph",
+    "edus_timing_enable", "edus_timing_get", "edus_get_tuning", "edus_set_tuning", "edus_batch_create",
+    "edus_batch_destroy", "edus_batch_groups", "edus_batch_graph", "edus_batch_rk4_steps", "edus_batch_store",
+    "edus_batch_load", "edus_batch_observables", "edus_batch_check", "edus_batch_synchronize", "edus_sim_create", "edus_sim_destroy", "edus_sim_info",
     "edus_sim_engine", "edus_sim_field", "edus_sim_time", "edus_sim_propagate", "edus_sim_steps", "edus_sim_get",
 ]
 
@@ -99,6 +101,8 @@ def lib():
         L.edus_create.restype = vp
         L.edus_create.argtypes = [C.POINTER(_Desc)]
         L.edus_destroy.argtypes = [vp]
+        L.edus_clone.restype = vp
+        L.edus_clone.argtypes = [vp]
         L.edus_last_error.restype = C.c_char_p
         for name in ("edus_set_state", "edus_get_state"):
             getattr(L, name).argtypes = [vp, dp]
@@ -109,9 +113,21 @@ def lib():
         L.edus_observables.argtypes = [vp, dp, dp, dp]
         L.edus_check.argtypes = [vp, dp, dp]
         L.edus_synchronize.argtypes = [vp]
+        L.edus_graph.argtypes = [vp, C.c_int]
         L.edus_timing_enable.argtypes = [vp, C.c_int]
         L.edus_timing_get.argtypes = [vp, C.c_char_p, dp, lp]
         L.edus_get_tuning.argtypes = [vp, C.POINTER(Tuning)]
+        L.edus_batch_create.restype = vp
+        L.edus_batch_create.argtypes = [C.c_int, C.POINTER(vp)]
+        L.edus_batch_destroy.argtypes = [vp]
+        L.edus_batch_groups.argtypes = [vp, ip, ip]
+        L.edus_batch_graph.argtypes = [vp, C.c_int]
+        L.edus_batch_rk4_steps.argtypes = [vp, C.c_double, C.c_int, dp]
+        L.edus_batch_store.argtypes = [vp, C.c_int]
+        L.edus_batch_load.argtypes = [vp, C.c_int]
+        L.edus_batch_observables.argtypes = [vp, C.c_int, dp, dp, dp]
+        L.edus_batch_check.argtypes = [vp, dp, dp]
+        L.edus_batch_synchronize.argtypes = [vp]
         L.edus_set_tuning.argtypes = [vp, C.POINTER(Tuning)]
         L.edus_sim_create.restype = vp
         L.edus_sim_create.argtypes = [C.POINTER(_SimParams)]
@@ -190,6 +206,15 @@ class Engine:
             lib().edus_destroy(self.h)
         self.h = None
 
+    def clone(self) -> "Engine":
+        """edus_clone: an independent context with copies of this one's arrays and state."""
+        h = lib().edus_clone(self.h)
+        if not h:
+            raise EdusError(-1, lib().edus_last_error().decode())
+        e = Engine(self.nb, self.N0, self.N1, None, _handle=h)
+        e._owned = True
+        return e
+
     def __del__(self):
         self.close()
 
@@ -236,6 +261,10 @@ class Engine:
     def synchronize(self):
         _check(lib().edus_synchronize(self.h))
 
+    def graph(self, steps_per_graph: int):
+        """edus_graph: replay RK4 steps from hipGraphs of this many steps (0: off)."""
+        _check(lib().edus_graph(self.h, int(steps_per_graph)))
+
     def timing_enable(self, on: bool = True):
         _check(lib().edus_timing_enable(self.h, int(bool(on))))
 
@@ -251,6 +280,64 @@ class Engine:
 
     def set_tuning(self, t: Tuning) -> None:
         _check(lib().edus_set_tuning(self.h, C.byref(t)))
+
+
+class Batch:
+    """edus_batch_*: many contexts advanced together.  The members (Engine
+    objects) must stay alive as long as the batch."""
+
+    def __init__(self, members):
+        self.members = list(members)
+        arr = (C.c_void_p * len(self.members))(*[m.h for m in self.members])
+        self.h = lib().edus_batch_create(len(self.members), arr)
+        if not self.h:
+            raise EdusError(-1, lib().edus_last_error().decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().edus_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    @property
+    def n(self):
+        return len(self.members)
+
+    def groups(self):
+        ng = C.c_int()
+        mg = (C.c_int * self.n)()
+        _check(lib().edus_batch_groups(self.h, C.byref(ng), mg))
+        return ng.value, list(mg)
+
+    def graph(self, steps_per_graph: int):
+        _check(lib().edus_batch_graph(self.h, int(steps_per_graph)))
+
+    def rk4_steps(self, dt: float, E) -> None:
+        """E[s][j][member][3] at t_s, t_s + dt/2, t_s + dt."""
+        e = np.ascontiguousarray(E, dtype=np.float64).reshape(-1, 3, self.n, 3)
+        _check(lib().edus_batch_rk4_steps(self.h, float(dt), e.shape[0], _dp(e)))
+
+    def store(self, i: int):
+        _check(lib().edus_batch_store(self.h, int(i)))
+
+    def load(self, i: int):
+        _check(lib().edus_batch_load(self.h, int(i)))
+
+    def observables(self, i: int):
+        nb = self.members[i].nb
+        pb, pw, v = np.empty(nb), np.empty(nb), np.empty(6)
+        _check(lib().edus_batch_observables(self.h, int(i), _dp(pb), _dp(pw), _dp(v)))
+        return pb, pw, v
+
+    def check(self):
+        t, h = C.c_double(), C.c_double()
+        _check(lib().edus_batch_check(self.h, C.byref(t), C.byref(h)))
+        return t.value, h.value
+
+    def synchronize(self):
+        _check(lib().edus_batch_synchronize(self.h))
 
 
 def _sim_params(p: dict, precision: int, layout: int) -> _SimParams:

@@ -87,6 +87,9 @@ typedef struct {
 
 /* ---- context ----------------------------------------------------------- */
 edus_ctx* edus_create(const edus_desc* desc);
+/* A new context with a device-side copy of src's set-up arrays and state (no
+ * counterpart in the reference; used to start many runs from one set-up). */
+edus_ctx* edus_clone(edus_ctx* src);
 int edus_destroy(edus_ctx* ctx);
 const char* edus_last_error(void);
 
@@ -122,6 +125,13 @@ int edus_check(edus_ctx* ctx, double* trace_dev, double* herm_dev);
 
 int edus_synchronize(edus_ctx* ctx);
 
+/* steps_per_graph > 0: edus_rk4_steps (and edus_sim stepping) replays hipGraphs
+ * of that many RK4 steps instead of launching every kernel; 0 (default):
+ * direct launches.  Same results: the captured kernels read the field from a
+ * device table.  Per-kernel timing (below) sees direct launches only.  No
+ * counterpart in the reference. */
+int edus_graph(edus_ctx* ctx, int steps_per_graph);
+
 /* Kernel timing with HIP events on the context stream.  When enabled, every
  * launch of the named kernel class is bracketed by events; edus_timing_get
  * returns the summed milliseconds and launch count since the last reset.
@@ -150,6 +160,33 @@ typedef struct {
 } edus_tuning;
 int edus_get_tuning(edus_ctx* ctx, edus_tuning* out);
 int edus_set_tuning(edus_ctx* ctx, const edus_tuning* in);
+
+/* ---- batches ------------------------------------------------------------
+ * Many contexts advanced together.  No counterpart in the reference (one
+ * simulation per process).  Members with the same nb, grid, layout,
+ * precision, exchange flag and filledbands are stacked into one group whose
+ * kernels advance all of them per launch; groups run on their own streams.
+ * The batch copies each member's set-up arrays and state at creation;
+ * members must outlive the batch (store/load copy states to and from them). */
+typedef struct edus_batch edus_batch;
+edus_batch* edus_batch_create(int n, edus_ctx* const* members);
+int edus_batch_destroy(edus_batch* batch);
+/* number of groups; member_group[n] (may be NULL) the group of each member */
+int edus_batch_groups(edus_batch* batch, int* ngroups, int* member_group);
+/* steps_per_graph > 0: capture that many RK4 steps of every group in one
+ * hipGraph and replay it; 0: direct launches (default) */
+int edus_batch_graph(edus_batch* batch, int steps_per_graph);
+/* nsteps RK4 steps of every member (DESolver.hpp:155-187), member i's field
+ * E[((s*3 + j)*n + i)*3 + d] at t_s, t_s + dt/2, t_s + dt (j = 0, 1, 2) */
+int edus_batch_rk4_steps(edus_batch* batch, double dt, int nsteps, const double* E);
+/* state of member i: batch -> member context (store) / member -> batch (load) */
+int edus_batch_store(edus_batch* batch, int i);
+int edus_batch_load(edus_batch* batch, int i);
+/* observables of member i's state in the batch (as edus_observables) */
+int edus_batch_observables(edus_batch* batch, int i, double* pop_bloch, double* pop_wannier, double* vel);
+/* invariant monitors over all members (as edus_check) */
+int edus_batch_check(edus_batch* batch, double* trace_dev, double* herm_dev);
+int edus_batch_synchronize(edus_batch* batch);
 
 /* ---- reference-format simulation (host set-up mirrored in C++) --------- */
 typedef struct {
