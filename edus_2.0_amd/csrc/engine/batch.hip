@@ -12,9 +12,9 @@
 // Graph mode captures S steps of every group -- each group on its own stream,
 // forked from and joined back to the batch stream -- into one hipGraph, which
 // is then replayed once per S steps after the next S steps' field tables are
-// copied in.  Captured kernels read every field component (no per-chunk
-// skipping of zero terms), which gives the same numbers: a zero field adds
-// exact zeros.
+// copied in.  As with direct launches, gradient axes and E.xi terms whose
+// field is zero throughout a call are skipped; skipping or not gives the same
+// numbers (a zero field adds exact zeros).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -25,57 +25,12 @@
 #include <tuple>
 #include <vector>
 
+#include "batch.hpp"
 #include "engine_internal.hpp"
 
 #define HIP_CHECK EDUS_HIP_CHECK
 
 namespace edus {
-
-struct BatchGroup {
-    Engine* e = nullptr;
-    std::unique_ptr<Engine> own;  // null when the group borrows a context's engine
-    std::vector<int> members;    // batch member index of each stacked instance
-    double* host_tab = nullptr;  // pinned staging: E then coef tables of one call
-    size_t host_cap = 0;         // doubles
-    size_t host_steps = 0;       // steps held by the staging of the current call
-    const double* cap_E = nullptr;  // device tables the graph was captured with
-};
-
-class Batch {
-  public:
-    Batch(int n, edus_ctx* const* ctxs);
-    explicit Batch(Engine* self);  // one member, the engine itself (graph replay of a context)
-    ~Batch();
-    Batch(const Batch&) = delete;
-    Batch& operator=(const Batch&) = delete;
-
-    int n;
-    std::vector<edus_ctx*> ctx;
-    std::vector<BatchGroup> groups;
-    std::vector<std::pair<int, int>> where;  // member -> (group, slot)
-    hipStream_t bs = nullptr;                // origin stream of graph launches
-    int graph_steps = 0;
-
-    void rk4_steps(double dt, int nsteps, const double* E);
-    void copy_state(int i, bool to_member);
-    void check(double* trace_dev, double* herm_dev);
-    void sync_all();
-
-  private:
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    double graph_dt = 0.0;
-    int graph_S = 0;
-    hipEvent_t fork = nullptr;
-    std::vector<hipEvent_t> joins;
-
-    void fill_tables(BatchGroup& g, int nsteps, const double* E, bool flags_axis[2], bool flags_xi[3]);
-    void upload_tables(BatchGroup& g, size_t h0, int cnt, hipStream_t st);
-    void capture(double dt, int S);
-    void fork_from(hipStream_t origin);
-    void join_to(hipStream_t origin);
-    void drop_graph();
-};
 
 Batch::Batch(int nmem, edus_ctx* const* ctxs) : n(nmem)
 {
@@ -128,6 +83,28 @@ Batch::Batch(Engine* self) : n(1)
     HIP_CHECK(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
     joins.resize(1);
     HIP_CHECK(hipEventCreateWithFlags(&joins[0], hipEventDisableTiming));
+}
+
+Batch::Batch(std::unique_ptr<Engine> group) : n(group->B)
+{
+    BatchGroup bg;
+    bg.own = std::move(group);
+    bg.e = bg.own.get();
+    for (int i = 0; i < n; ++i) {
+        bg.members.push_back(i);
+        where.push_back({0, i});
+    }
+    groups.push_back(std::move(bg));
+    HIP_CHECK(hipStreamCreateWithFlags(&bs, hipStreamNonBlocking));
+    HIP_CHECK(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
+    joins.resize(1);
+    HIP_CHECK(hipEventCreateWithFlags(&joins[0], hipEventDisableTiming));
+}
+
+double2* Batch::state(int i)
+{
+    const BatchGroup& g = groups[where[i].first];
+    return g.e->y + size_t(where[i].second) * g.e->inst_elems();
 }
 
 void delete_batch(Batch* b) { delete b; }
@@ -240,8 +217,9 @@ void Batch::join_to(hipStream_t origin)
     }
 }
 
-// One graph holding S RK4 steps of every group, table steps 0 .. S-1.
-void Batch::capture(double dt, int S)
+// One graph holding S RK4 steps of every group, table steps 0 .. S-1; the
+// gradient axes and E.xi terms in `flags` are the ones the graph evaluates.
+void Batch::capture(double dt, int S, const std::vector<std::array<bool, 5>>& flags)
 {
     drop_graph();
     std::vector<bool> timing(groups.size());
@@ -252,14 +230,14 @@ void Batch::capture(double dt, int S)
     }
     HIP_CHECK(hipStreamSynchronize(bs));
     for (auto& g : groups) HIP_CHECK(hipStreamSynchronize(g.e->s));
-    const bool all_axis[2] = {true, true};
     HIP_CHECK(hipStreamBeginCapture(bs, hipStreamCaptureModeThreadLocal));
     try {
         fork_from(bs);
         for (int st = 0; st < S; ++st)
-            for (auto& g : groups) {
-                const bool use_xi[3] = {g.e->xi[0] != nullptr, g.e->xi[1] != nullptr, g.e->xi[2] != nullptr};
-                g.e->rk4_step_tab(dt, st, all_axis, use_xi);
+            for (size_t gi = 0; gi < groups.size(); ++gi) {
+                const bool ua[2] = {flags[gi][0], flags[gi][1]};
+                const bool ux[3] = {flags[gi][2], flags[gi][3], flags[gi][4]};
+                groups[gi].e->rk4_step_tab(dt, st, ua, ux);
             }
         join_to(bs);
     } catch (...) {
@@ -274,6 +252,7 @@ void Batch::capture(double dt, int S)
     for (size_t i = 0; i < groups.size(); ++i) groups[i].e->timing = timing[i];
     graph_dt = dt;
     graph_S = S;
+    graph_flags = flags;
     for (auto& g : groups) g.cap_E = g.e->E_dev;
 }
 
@@ -289,9 +268,12 @@ void Batch::rk4_steps(double dt, int nsteps, const double* E)
     int done = 0;
     const int S = graph_steps;
     if (S > 0 && nsteps >= S) {
-        bool stale = exec == nullptr || graph_S != S || graph_dt != dt;
+        // the graph evaluates the gradient axes and E.xi terms that are non-zero
+        // in this call; a call that needs others (e.g. the pulse switching on)
+        // captures again
+        bool stale = exec == nullptr || graph_S != S || graph_dt != dt || graph_flags != flags;
         for (auto& g : groups) stale = stale || g.cap_E != g.e->E_dev;
-        if (stale) capture(dt, S);
+        if (stale) capture(dt, S, flags);
         for (; done + S <= nsteps; done += S) {
             // the graph reads table steps 0..S-1: copy this chunk's fields in
             // behind the previous replay on the same stream

@@ -86,7 +86,9 @@ EXPORTS = [
     "edus_rk4_steps", "edus_ab_steps", "edus_ab_reset", "edus_observables", "edus_check", "edus_synchronize", "edus_graph",
     "edus_timing_enable", "edus_timing_get", "edus_get_tuning", "edus_set_tuning", "edus_batch_create",
     "edus_batch_destroy", "edus_batch_groups", "edus_batch_graph", "edus_batch_rk4_steps", "edus_batch_store",
-    "edus_batch_load", "edus_batch_observables", "edus_batch_check", "edus_batch_synchronize", "edus_sim_create", "edus_sim_destroy", "edus_sim_info",
+    "edus_batch_load", "edus_batch_observables", "edus_batch_check", "edus_batch_synchronize", "edus_mlp_create",
+    "edus_mlp_destroy", "edus_mlp_num_params", "edus_mlp_set_params", "edus_mlp_get_params", "edus_mlp_forward",
+    "edus_mlp_gradients", "edus_mlp_train", "edus_sim_create", "edus_sim_destroy", "edus_sim_info",
     "edus_sim_engine", "edus_sim_field", "edus_sim_time", "edus_sim_propagate", "edus_sim_steps", "edus_sim_get",
 ]
 
@@ -117,6 +119,16 @@ def lib():
         L.edus_timing_enable.argtypes = [vp, C.c_int]
         L.edus_timing_get.argtypes = [vp, C.c_char_p, dp, lp]
         L.edus_get_tuning.argtypes = [vp, C.POINTER(Tuning)]
+        fp = C.POINTER(C.c_float)
+        L.edus_mlp_create.restype = vp
+        L.edus_mlp_create.argtypes = [C.c_int, C.c_int, C.c_int, C.c_ulonglong]
+        L.edus_mlp_destroy.argtypes = [vp]
+        L.edus_mlp_num_params.argtypes = [vp, lp]
+        L.edus_mlp_set_params.argtypes = [vp, fp]
+        L.edus_mlp_get_params.argtypes = [vp, fp]
+        L.edus_mlp_forward.argtypes = [vp, C.c_int, fp, fp]
+        L.edus_mlp_gradients.argtypes = [vp, C.c_int, fp, fp, fp, fp]
+        L.edus_mlp_train.argtypes = [vp, C.c_int, fp, fp, C.c_int, C.c_int, C.c_float, C.c_ulonglong, fp]
         L.edus_batch_create.restype = vp
         L.edus_batch_create.argtypes = [C.c_int, C.POINTER(vp)]
         L.edus_batch_destroy.argtypes = [vp]
@@ -280,6 +292,63 @@ class Engine:
 
     def set_tuning(self, t: Tuning) -> None:
         _check(lib().edus_set_tuning(self.h, C.byref(t)))
+
+
+def _fp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+class Mlp:
+    """edus_mlp_*: the per-k network of the learned coarse propagator."""
+
+    def __init__(self, din: int, dh: int, dout: int, seed: int = 0):
+        self.din, self.dh, self.dout = din, dh, dout
+        self.h = lib().edus_mlp_create(din, dh, dout, seed)
+        if not self.h:
+            raise EdusError(-1, lib().edus_last_error().decode())
+        n = C.c_long()
+        _check(lib().edus_mlp_num_params(self.h, C.byref(n)))
+        self.nparam = n.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().edus_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def get_params(self):
+        p = np.empty(self.nparam, dtype=np.float32)
+        _check(lib().edus_mlp_get_params(self.h, _fp(p)))
+        return p
+
+    def set_params(self, p):
+        a = np.ascontiguousarray(p, dtype=np.float32)
+        assert a.size == self.nparam
+        _check(lib().edus_mlp_set_params(self.h, _fp(a)))
+
+    def forward(self, X):
+        x = np.ascontiguousarray(X, dtype=np.float32)
+        y = np.empty((x.shape[0], self.dout), dtype=np.float32)
+        _check(lib().edus_mlp_forward(self.h, x.shape[0], _fp(x), _fp(y)))
+        return y
+
+    def gradients(self, X, T):
+        x = np.ascontiguousarray(X, dtype=np.float32)
+        t = np.ascontiguousarray(T, dtype=np.float32)
+        g = np.empty(self.nparam, dtype=np.float32)
+        loss = C.c_float()
+        _check(lib().edus_mlp_gradients(self.h, x.shape[0], _fp(x), _fp(t), _fp(g), C.byref(loss)))
+        return loss.value, g
+
+    def train(self, X, T, epochs: int, batch: int, lr: float, seed: int = 0):
+        x = np.ascontiguousarray(X, dtype=np.float32)
+        t = np.ascontiguousarray(T, dtype=np.float32)
+        hist = np.empty(max(epochs, 1), dtype=np.float32)
+        _check(lib().edus_mlp_train(self.h, x.shape[0], _fp(x), _fp(t), int(epochs), int(batch), float(lr),
+                                    int(seed), _fp(hist)))
+        return hist[:epochs]
 
 
 class Batch:

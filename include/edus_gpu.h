@@ -188,6 +188,75 @@ int edus_batch_observables(edus_batch* batch, int i, double* pop_bloch, double* 
 int edus_batch_check(edus_batch* batch, double* trace_dev, double* herm_dev);
 int edus_batch_synchronize(edus_batch* batch);
 
+/* ---- per-k MLP (learned coarse propagator) -------------------------------
+ * No counterpart in the reference.  Three dense layers din -> dh -> dh -> dout,
+ * tanh after the first two; BF16 weights and activations, FP32 accumulation
+ * on the matrix cores, FP32 master weights.  Parameters are one FP32 block:
+ * W1 [din][dh], b1 [dh], W2 [dh][dh], b2 [dh], W3 [dh][dout], b3 [dout].
+ * Inputs are rounded to BF16 (nearest even) on entry. */
+typedef struct edus_mlp edus_mlp;
+edus_mlp* edus_mlp_create(int din, int dh, int dout, unsigned long long seed);
+int edus_mlp_destroy(edus_mlp* mlp);
+int edus_mlp_num_params(edus_mlp* mlp, long* n);
+int edus_mlp_set_params(edus_mlp* mlp, const float* params);
+int edus_mlp_get_params(edus_mlp* mlp, float* params);
+/* Y [rows][dout] = net(X [rows][din]) */
+int edus_mlp_forward(edus_mlp* mlp, int rows, const float* X, float* Y);
+/* gradient of L = mean over rows and outputs of (net(X) - T)^2 w.r.t. the
+ * parameters (same layout), and L */
+int edus_mlp_gradients(edus_mlp* mlp, int rows, const float* X, const float* T, float* grads, float* loss);
+/* Adam (beta 0.9, 0.999, eps 1e-8) over shuffled mini-batches of `batch`
+ * rows for `epochs` passes; loss_per_epoch[epochs] (may be NULL) */
+int edus_mlp_train(edus_mlp* mlp, int rows, const float* X, const float* T, int epochs, int batch, float lr,
+                   unsigned long long seed, float* loss_per_epoch);
+
+/* ---- learned parareal ------------------------------------------------------
+ * No counterpart in the reference.  The run of nslices * fine_steps RK4 steps
+ * of dt from a context's state is cut into nslices slices; F = RK4 over a
+ * slice (fine_steps steps), G = RK4 with coarse_steps steps, plus the per-k
+ * network's correction when one is set.  Parareal iterations
+ *     U_{n+1} <- G(U_n new) + F(U_n old) - G(U_n old)
+ * run every slice's F as one stacked batch; they stop when the largest change
+ * of a slice boundary is <= tol, and after max_iter iterations without that
+ * the run is redone as sequential RK4 (fallback = 1). */
+typedef struct {
+    int nslices, fine_steps, coarse_steps;
+    double dt;        /* fine step; the coarse step is dt * fine_steps / coarse_steps */
+    double tol;       /* max |change of U_n| over slices and elements */
+    int max_iter;     /* parareal iterations before the sequential fallback */
+    int graph_steps;  /* > 0: hipGraph replay of the fine batch (the coarse RK4 replays its whole slice) */
+} edus_parareal_params;
+typedef struct {
+    int iterations, converged, fallback;
+    double last_update;  /* max change of the last iteration */
+    double seconds, fine_seconds, coarse_seconds;
+} edus_parareal_result;
+typedef struct edus_parareal edus_parareal;
+/* clones `base` (its set-up; the run starts from its state unless rho0 is given) */
+edus_parareal* edus_parareal_create(edus_ctx* base, const edus_parareal_params* params);
+int edus_parareal_destroy(edus_parareal* pr);
+/* fields: E_fine[((n*fine_steps + s)*3 + j)*3 + d] at t_s, t_s + dt/2, t_s + dt of
+ * slice n; E_coarse likewise with coarse_steps; E_feat[(n*field_samples + i)*2 + c]
+ * the x, y field samples of slice n given to the network */
+int edus_parareal_set_fields(edus_parareal* pr, const double* E_fine, const double* E_coarse, const float* E_feat,
+                             int field_samples);
+/* the network (din = 2 nb^2 + 2 field_samples + 4, dout = nb^2) with its input
+ * shift/scale mu, sig [din] and output scale/shift tsig, tmu [nb^2]; NULL:
+ * plain coarse RK4 */
+int edus_parareal_set_network(edus_parareal* pr, edus_mlp* net, const float* mu, const float* sig, const float* tmu,
+                              const float* tsig);
+int edus_parareal_run(edus_parareal* pr, const double* rho0, edus_parareal_result* result);
+/* slice boundary n = 0 .. nslices of the last run */
+int edus_parareal_get_state(edus_parareal* pr, int n, double* rho);
+/* training data for the network from `runs` runs with their own fields (run r:
+ * E_fine[r][...], E_coarse[r][...], E_feat[r][...] as above), started from
+ * base's state: for k_samples random k-points per run and slice, the features
+ * X[row][din] and targets T[row][nb^2] = param(F(U_n) - RK4_coarse(U_n)), rows
+ * ordered [run][slice][sample] */
+int edus_parareal_collect(edus_ctx* base, const edus_parareal_params* params, int runs, const double* E_fine,
+                          const double* E_coarse, const float* E_feat, int field_samples, int k_samples,
+                          unsigned long long seed, float* X, float* T);
+
 /* ---- reference-format simulation (host set-up mirrored in C++) --------- */
 typedef struct {
     double intensity; char intensity_units[32];
