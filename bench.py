@@ -26,14 +26,20 @@ METRIC = "k-point·RK4-steps/s (nb=14, 120² k); learned-parareal vs plain RK4 @
 HBM_PEAK_GBS = 8000.0
 
 
-def kernel_bytes(cls: str, npl: int, nk: int, nb: int, n_e: int) -> float:
-    """Algorithmic HBM bytes per launch: one array of npl planes is npl*nk*16 B.
-    grad_col reads rho, writes G; grad_row reads rho and G, writes D; comm
-    (RK4 stage average) reads rho_s, H0, n_E xi, D plus the stage's state
-    arrays, writes its outputs -- (6 + n_E) arrays on average over the four
-    stages; monitor reads the nb diagonal planes."""
+def kernel_bytes(cls: str, npl: int, nk: int, nb: int, n_e: int, solver: str = "RK", order: int = 4,
+                 coulomb: bool = False) -> float:
+    """Algorithmic HBM bytes per launch: one array of npl planes is
+    S = npl*nk*16 B.  grad_col reads rho, writes G (2S); grad_row reads rho
+    and G, writes D (3S); hf_row (forward or inverse row transform) 2S;
+    hf_col (column convolution with the real W) 2.5S.  comm reads rho_s, H0,
+    n_E xi, D (+ the exchange self energy) and the stage's state arrays and
+    writes its outputs: RK4 averages (6 + n_E) S over its four stages, AB of
+    order p reads y and p-1 history arrays and writes y and the new history
+    entry, (4 + n_E + p) S.  monitor reads the nb diagonal planes."""
     s = 16.0 * npl * nk
-    return {"grad_col": 2 * s, "grad_row": 3 * s, "comm": (6 + n_e) * s,
+    sig = 1 if coulomb else 0
+    comm = (6 + n_e + sig) if solver == "RK" else (4 + n_e + order + sig)
+    return {"grad_col": 2 * s, "grad_row": 3 * s, "comm": comm * s, "hf_row": 2 * s, "hf_col": 2.5 * s,
             "monitor": 16.0 * nb * nk}.get(cls, 0.0)
 
 
@@ -72,6 +78,9 @@ def main():
         print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported "
               "(multi-GPU sharding is described in DESIGN.md, not built)", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "hetero_batch64":
+        run_batch(args)
+        return
 
     import numpy as np  # noqa: F401
 
@@ -112,11 +121,17 @@ def main():
     eng.timing_enable(False)
 
     value = nk * args.steps / elapsed
-    n_e = int(sum(1 for x in params["lasers"][0]["polarization"] if x != 0))
+    integ = "RK4" if params["solver"] == "RK" else f"AB{params['order']}"
+    pol = np.zeros(3)
+    for las in params["lasers"]:
+        pol += np.abs(np.asarray(las["polarization"], dtype=float))
+    n_e = int(np.count_nonzero(pol))
+    solver, order, coul = params["solver"], int(params["order"]), bool(params["coulomb"])
+    kb = {c: kernel_bytes(c, npl, nk, nb, n_e, solver, order, coul) for c in classes}
     dom = max(classes, key=lambda c: times[c][0])
     ms, cnt = times[dom]
     avg_s = ms / 1e3 / max(cnt, 1)
-    byts = kernel_bytes(dom, npl, nk, nb, n_e)
+    byts = kb[dom]
     achieved = byts / avg_s / 1e9
     roofline = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": None, "bytes_per_launch": byts,
@@ -125,15 +140,15 @@ def main():
                 "direct_ms_per_step": elapsed_direct / args.steps * 1e3,
                 "kernel_share": {c: round(times[c][0] / 1e3 / elapsed_direct, 4) for c in classes if times[c][1]},
                 "avg_us": {c: round(times[c][0] * 1e3 / times[c][1], 2) for c in classes if times[c][1]},
-                "bytes_per_launch_all": {c: kernel_bytes(c, npl, nk, nb, n_e) for c in classes if times[c][1]}}
+                "bytes_per_launch_all": {c: kb[c] for c in classes if times[c][1]}}
 
     out = {
-        "metric": METRIC, "value": value, "unit": "k-point·RK4-steps/s", "n_gpus": 1, "steps": args.steps,
+        "metric": METRIC, "value": value, "unit": f"k-point·{integ}-steps/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": elapsed / args.steps * 1e3, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64" if args.precision == "fp64" else "f64 (f32 commutator)",
         "data": "synthetic",
         "config": {"workload": args.workload, "nb": nb, "grid": [sim.N0, sim.N1], "dt_au": sim.dt,
-                   "integrator": "RK4", "layout": args.layout, "precision": args.precision,
+                   "integrator": integ, "layout": args.layout, "precision": args.precision,
                    "graph_steps": args.graph,
                    "model": "seeded random tight-binding (edus.models)", "tuning": eng.get_tuning().as_dict()},
         "roofline": roofline,
@@ -141,6 +156,73 @@ def main():
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(params, args.cpu_steps)
+    print(json.dumps(out), flush=True)
+
+
+def run_batch(args):
+    """Heterogeneous batch: 64 pulse runs over nb in {2, 6, 14} and grids
+    48^2, 96^2, 120^2 (one seeded model per class), peak fields log-spaced
+    over 1e-3 .. 3e-2 a.u., CEP uniform in [0, 2 pi), photon energy uniform in
+    [0.8, 2.0] eV.  The same steps run once member by member (direct launches,
+    each context alone) and once as one batch (stacked groups, one hipGraph
+    per graph_steps steps); throughput = sum over members of Nk * steps / s."""
+    import numpy as np
+
+    import edus
+    from edus import models
+    from edus.parareal import Pulse, stage_table
+
+    rng = np.random.default_rng(7)
+    classes = [(w, g) for w in ("ipa2_square30", "ipa6_hex96", "ipa14_hex120") for g in (48, 96, 120)]
+    tmp = tempfile.mkdtemp(prefix="edus_batch_")
+    sims = {}
+    for w, g in classes:
+        p = models.workload(w, os.path.join(tmp, f"{w}_{g}"), grid=[g, g, 1])
+        sims[(w, g)] = edus.Simulation(p)
+    n = 64
+    e0 = np.logspace(np.log10(1e-3), np.log10(3e-2), n)
+    members, pulses, nks = [], [], []
+    for i in range(n):
+        w, g = classes[i % len(classes)]
+        members.append(sims[(w, g)].engine.clone())
+        pulses.append(Pulse(e0[i], rng.uniform(0.8, 2.0), 20.0, (1.0, 0.0, 0.0), rng.uniform(0, 2 * np.pi)))
+        nks.append(g * g)
+    dt, K, W = 0.05, args.steps, args.warmup
+    t_start = 200.0  # inside the pulses
+    E = np.stack([stage_table(pl, t_start, dt, 2 * W + 2 * K) for pl in pulses], axis=2)  # [s][j][i][3]
+    # member by member
+    for i, m in enumerate(members):
+        m.rk4_steps(dt, E[:W, :, i, :])
+        m.synchronize()
+    t0 = time.perf_counter()
+    for i, m in enumerate(members):
+        m.rk4_steps(dt, E[W:W + K, :, i, :])
+    for m in members:
+        m.synchronize()
+    t_seq = time.perf_counter() - t0
+    # as one batch, from where the members stand
+    batch = edus.Batch(members)
+    ng, _ = batch.groups()
+    batch.graph(args.graph)
+    batch.rk4_steps(dt, E[W + K:2 * W + K])
+    batch.synchronize()
+    t1 = time.perf_counter()
+    batch.rk4_steps(dt, E[2 * W + K:2 * W + 2 * K])
+    batch.synchronize()
+    t_batch = time.perf_counter() - t1
+    tr, herm = batch.check()
+    work = float(sum(nks)) * K
+    out = {
+        "metric": "k-point·RK4-steps/s, 64 heterogeneous pulse runs", "value": work / t_batch,
+        "unit": "k-point·RK4-steps/s", "n_gpus": 1, "steps": K, "warmup": W,
+        "ms_per_step": t_batch / K * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "hetero_batch64", "members": n, "groups": ng, "graph_steps": args.graph,
+                   "classes": [f"{w} {g}x{g}" for w, g in classes]},
+        "sequential": {"value": work / t_seq, "ms_per_step": t_seq / K * 1e3},
+        "batch_over_sequential": t_seq / t_batch,
+        "monitors": {"trace_dev": tr, "herm_dev": herm},
+    }
     print(json.dumps(out), flush=True)
 
 

@@ -83,9 +83,15 @@ constexpr int scalar_tile()
     return tk;
 }
 
-// MFMA engines: 8 k-points per tile (two per wave; 8 x 16 B = one 128-byte
-// line per plane), plane stride padded to 9 against LDS bank conflicts
-constexpr int kMfmaTile = 8;
+// MFMA engines: tile 0 holds 8 k-points (two per wave; 8 x 16 B = one
+// 128-byte line per plane), tile 1 holds 4 (one per wave; half the LDS, so
+// more work-groups per CU); plane stride padded by one against LDS bank
+// conflicts.  The per-k arithmetic does not depend on the tile.
+template <int NB, int LAYOUT, int KIND, int TILE>
+constexpr int tile_k()
+{
+    return KIND == kCommScalar ? scalar_tile<NB, LAYOUT>() : (TILE == 0 ? 8 : 4);
+}
 
 template <int KIND, int TK>
 constexpr int plane_stride()
@@ -93,10 +99,10 @@ constexpr int plane_stride()
     return KIND == kCommScalar ? TK : TK + 1;
 }
 
-template <int NB, int LAYOUT, int KIND>
+template <int NB, int LAYOUT, int KIND, int TILE>
 constexpr size_t comm_lds_bytes()
 {
-    constexpr int TK = KIND == kCommScalar ? scalar_tile<NB, LAYOUT>() : kMfmaTile;
+    constexpr int TK = tile_k<NB, LAYOUT, KIND, TILE>();
     // the monitor's block reduction reuses the tile: 2 doubles per thread
     return std::max(size_t(2) * npl_of<NB, LAYOUT>() * plane_stride<KIND, TK>() * sizeof(double2),
                     size_t(2) * kCommThreads * sizeof(double));
@@ -167,12 +173,12 @@ __device__ inline void mfma_commutator32(const double2* sH, const double2* sR, i
     }
 }
 
-template <int NB, int LAYOUT, int KIND>
+template <int NB, int LAYOUT, int KIND, int TILE>
 __global__ void __launch_bounds__(kCommThreads) comm_kernel(CommArgs a)
 {
     constexpr int NT = kCommThreads;
     constexpr int NPL = npl_of<NB, LAYOUT>();
-    constexpr int TK = KIND == kCommScalar ? scalar_tile<NB, LAYOUT>() : kMfmaTile;
+    constexpr int TK = tile_k<NB, LAYOUT, KIND, TILE>();
     constexpr int PS = plane_stride<KIND, TK>();
     constexpr int PER = (NPL * TK + NT - 1) / NT;  // elements per thread
     extern __shared__ double2 lds[];
@@ -395,12 +401,12 @@ __global__ void __launch_bounds__(kCommThreads) comm_kernel(CommArgs a)
     }
 }
 
-template <int NB, int LAYOUT, int KIND>
+template <int NB, int LAYOUT, int KIND, int TILE>
 void launch_kind(const CommArgs& a, hipStream_t s)
 {
-    constexpr int TK = KIND == kCommScalar ? scalar_tile<NB, LAYOUT>() : kMfmaTile;
-    constexpr size_t lds = comm_lds_bytes<NB, LAYOUT, KIND>();
-    auto kern = &comm_kernel<NB, LAYOUT, KIND>;
+    constexpr int TK = tile_k<NB, LAYOUT, KIND, TILE>();
+    constexpr size_t lds = comm_lds_bytes<NB, LAYOUT, KIND, TILE>();
+    auto kern = &comm_kernel<NB, LAYOUT, KIND, TILE>;
     static bool raised = false;
     if (!raised) {
         (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -410,47 +416,54 @@ void launch_kind(const CommArgs& a, hipStream_t s)
     hipLaunchKernelGGL(kern, dim3(((a.Nk + TK - 1) / TK) * std::max(a.B, 1)), dim3(kCommThreads), lds, s, a);
 }
 
-template <int NB>
-void launch_nb(const CommArgs& a, int layout, int kind, hipStream_t s)
+template <int NB, int LAYOUT>
+void launch_layout(const CommArgs& a, int kind, int tile, hipStream_t s)
 {
-    if (layout == kFull) {
-        if (kind == kCommMfma64)
-            launch_kind<NB, kFull, kCommMfma64>(a, s);
-        else if (kind == kCommMfma32)
-            launch_kind<NB, kFull, kCommMfma32>(a, s);
+    if (kind == kCommMfma64) {
+        if (tile == 1)
+            launch_kind<NB, LAYOUT, kCommMfma64, 1>(a, s);
         else
-            launch_kind<NB, kFull, kCommScalar>(a, s);
+            launch_kind<NB, LAYOUT, kCommMfma64, 0>(a, s);
+    } else if (kind == kCommMfma32) {
+        if (tile == 1)
+            launch_kind<NB, LAYOUT, kCommMfma32, 1>(a, s);
+        else
+            launch_kind<NB, LAYOUT, kCommMfma32, 0>(a, s);
     } else {
-        if (kind == kCommMfma64)
-            launch_kind<NB, kHermitian, kCommMfma64>(a, s);
-        else if (kind == kCommMfma32)
-            launch_kind<NB, kHermitian, kCommMfma32>(a, s);
-        else
-            launch_kind<NB, kHermitian, kCommScalar>(a, s);
+        launch_kind<NB, LAYOUT, kCommScalar, 0>(a, s);
     }
+}
+
+template <int NB>
+void launch_nb(const CommArgs& a, int layout, int kind, int tile, hipStream_t s)
+{
+    if (layout == kFull)
+        launch_layout<NB, kFull>(a, kind, tile, s);
+    else
+        launch_layout<NB, kHermitian>(a, kind, tile, s);
 }
 
 }  // namespace
 
-void launch_comm(int nb, int layout, int kind, const CommArgs& a, hipStream_t s)
+void launch_comm(int nb, int layout, int kind, int tile, const CommArgs& a, hipStream_t s)
 {
     switch (nb) {
-        case 1: launch_nb<1>(a, layout, kind, s); break;
-        case 2: launch_nb<2>(a, layout, kind, s); break;
-        case 3: launch_nb<3>(a, layout, kind, s); break;
-        case 4: launch_nb<4>(a, layout, kind, s); break;
-        case 5: launch_nb<5>(a, layout, kind, s); break;
-        case 6: launch_nb<6>(a, layout, kind, s); break;
-        case 7: launch_nb<7>(a, layout, kind, s); break;
-        case 8: launch_nb<8>(a, layout, kind, s); break;
-        case 9: launch_nb<9>(a, layout, kind, s); break;
-        case 10: launch_nb<10>(a, layout, kind, s); break;
-        case 11: launch_nb<11>(a, layout, kind, s); break;
-        case 12: launch_nb<12>(a, layout, kind, s); break;
-        case 13: launch_nb<13>(a, layout, kind, s); break;
-        case 14: launch_nb<14>(a, layout, kind, s); break;
-        case 15: launch_nb<15>(a, layout, kind, s); break;
-        case 16: launch_nb<16>(a, layout, kind, s); break;
+        case 1: launch_nb<1>(a, layout, kind, tile, s); break;
+        case 2: launch_nb<2>(a, layout, kind, tile, s); break;
+        case 3: launch_nb<3>(a, layout, kind, tile, s); break;
+        case 4: launch_nb<4>(a, layout, kind, tile, s); break;
+        case 5: launch_nb<5>(a, layout, kind, tile, s); break;
+        case 6: launch_nb<6>(a, layout, kind, tile, s); break;
+        case 7: launch_nb<7>(a, layout, kind, tile, s); break;
+        case 8: launch_nb<8>(a, layout, kind, tile, s); break;
+        case 9: launch_nb<9>(a, layout, kind, tile, s); break;
+        case 10: launch_nb<10>(a, layout, kind, tile, s); break;
+        case 11: launch_nb<11>(a, layout, kind, tile, s); break;
+        case 12: launch_nb<12>(a, layout, kind, tile, s); break;
+        case 13: launch_nb<13>(a, layout, kind, tile, s); break;
+        case 14: launch_nb<14>(a, layout, kind, tile, s); break;
+        case 15: launch_nb<15>(a, layout, kind, tile, s); break;
+        case 16: launch_nb<16>(a, layout, kind, tile, s); break;
         default: break;  // rejected at context creation
     }
 }

@@ -76,10 +76,10 @@ std::map<std::tuple<int, int, int, int>, FFTChoice>& tuned_shapes()
     return m;
 }
 
-// commutator engine per (nb, layout, Nk)
-std::map<std::tuple<int, int, int>, int>& tuned_comm()
+// commutator engine and tile per (nb, layout, k-points of all instances, precision)
+std::map<std::tuple<int, int, int, int>, std::pair<int, int>>& tuned_comm()
 {
-    static std::map<std::tuple<int, int, int>, int> m;
+    static std::map<std::tuple<int, int, int, int>, std::pair<int, int>> m;
     return m;
 }
 
@@ -204,7 +204,7 @@ void Engine::build_velocity(double det)
         c.up.x1 = f;
         c.up.out1 = vel[d];
         c.up.b1 = 1.0 / det;
-        launch_comm(nb, layout, comm_kind64, c, s);
+        launch_comm(nb, layout, comm_kind64, comm_tile64, c, s);
     }
     HIP_CHECK(hipMemsetAsync(aux, 0, pb, s));
     HIP_CHECK(hipStreamSynchronize(s));
@@ -435,42 +435,58 @@ void Engine::autotune()
         tuned_shapes()[key] = best_c;
     }
 
-    // FP64 commutator engine: FMAs or matrix cores, whichever is faster here
-    // (they differ in summation order only)
-    const auto ckey = std::make_tuple(nb, layout, Nk);
+    // commutator engine and tile, whichever is fastest here: for FP64 the FMA
+    // engine or the matrix cores (they differ in summation order only), for
+    // mixed precision the FP32 matrix cores; tiles give identical results
+    const auto ckey = std::make_tuple(nb, layout, Nk * B, prec);
     auto cit = tuned_comm().find(ckey);
     if (cit != tuned_comm().end()) {
-        comm_kind64 = cit->second;
+        comm_kind = cit->second.first;
+        comm_tile = cit->second.second;
     } else {
+        std::vector<std::pair<int, int>> cands;
+        if (prec == EDUS_PREC_MIXED)
+            cands = {{kCommMfma32, 0}, {kCommMfma32, 1}};
+        else
+            cands = {{kCommScalar, 0}, {kCommMfma64, 0}, {kCommMfma64, 1}};
         hipEvent_t e0, e1;
         HIP_CHECK(hipEventCreate(&e0));
         HIP_CHECK(hipEventCreate(&e1));
         CommArgs c{};
         c.Nk = Nk;
         c.P = npl;
+        c.B = B;
         c.rho = y;
         c.H0 = H0;
         c.up.out_f = f;
         float best = 1e30f;
-        for (int kind : {int(kCommScalar), int(kCommMfma64)}) {
-            launch_comm(nb, layout, kind, c, s);  // warm-up
+        for (const auto& kt : cands) {
+            launch_comm(nb, layout, kt.first, kt.second, c, s);  // warm-up
             HIP_CHECK(hipEventRecord(e0, s));
-            for (int r = 0; r < 3; ++r) launch_comm(nb, layout, kind, c, s);
+            for (int r = 0; r < 3; ++r) launch_comm(nb, layout, kt.first, kt.second, c, s);
             HIP_CHECK(hipEventRecord(e1, s));
             HIP_CHECK(hipEventSynchronize(e1));
             float ms = 0.f;
             HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
             if (ms < best) {
                 best = ms;
-                comm_kind64 = kind;
+                comm_kind = kt.first;
+                comm_tile = kt.second;
             }
         }
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         HIP_CHECK(hipGetLastError());
-        tuned_comm()[ckey] = comm_kind64;
+        tuned_comm()[ckey] = {comm_kind, comm_tile};
     }
-    comm_kind = prec == EDUS_PREC_MIXED ? int(kCommMfma32) : comm_kind64;
+    if (prec == EDUS_PREC_MIXED) {
+        // set-up products (the velocity operator) stay FP64
+        comm_kind64 = kCommMfma64;
+        comm_tile64 = 0;
+    } else {
+        comm_kind64 = comm_kind;
+        comm_tile64 = comm_tile;
+    }
 
     HIP_CHECK(hipMemsetAsync(G, 0, plane_bytes(), s));
     HIP_CHECK(hipMemsetAsync(f, 0, plane_bytes(), s));
@@ -528,7 +544,7 @@ void Engine::evaluate(const double2* in, const double E[3], const StageUpdate& u
     c.up = up;
     TimedLaunch t;
     tbegin("comm", t);
-    launch_comm(nb, layout, comm_kind, c, s);
+    launch_comm(nb, layout, comm_kind, comm_tile, c, s);
     tend(t);
 }
 
@@ -654,8 +670,6 @@ Engine::Engine(const std::vector<const Engine*>& members)
     }
     HIP_CHECK(hipStreamSynchronize(s));
     autotune();
-    comm_kind = m0.comm_kind;
-    comm_kind64 = m0.comm_kind64;
     HIP_CHECK(hipStreamSynchronize(s));
     HIP_CHECK(hipGetLastError());
 }
@@ -691,7 +705,7 @@ void Engine::evaluate_tab(const double2* in, int ts, int j, const bool use_axis[
     c.up = up;
     TimedLaunch t;
     tbegin("comm", t);
-    launch_comm(nb, layout, comm_kind, c, s);
+    launch_comm(nb, layout, comm_kind, comm_tile, c, s);
     tend(t);
 }
 
@@ -901,6 +915,7 @@ int edus_get_tuning(edus_ctx* ctx, edus_tuning* out)
         out->fft_lines[a] = e.fft_choice[a].L;
     }
     out->comm_kind = e.comm_kind;
+    out->comm_tile = e.comm_tile;
     return EDUS_OK;
 }
 
@@ -934,10 +949,16 @@ int edus_set_tuning(edus_ctx* ctx, const edus_tuning* in)
         if (e.prec == EDUS_PREC_MIXED ? in->comm_kind != edus::kCommMfma32
                                       : (in->comm_kind != edus::kCommScalar && in->comm_kind != edus::kCommMfma64))
             throw EngineError(EDUS_ERR_ARG, "comm_kind does not match the context precision");
+        if (in->comm_tile != 0 && (in->comm_tile != 1 || in->comm_kind == edus::kCommScalar))
+            throw EngineError(EDUS_ERR_ARG, "comm_tile must be 0, or 1 for the MFMA engines");
         e.fft_choice[0] = next[0];
         e.fft_choice[1] = next[1];
         e.comm_kind = in->comm_kind;
-        if (e.prec != EDUS_PREC_MIXED) e.comm_kind64 = in->comm_kind;
+        e.comm_tile = in->comm_tile;
+        if (e.prec != EDUS_PREC_MIXED) {
+            e.comm_kind64 = in->comm_kind;
+            e.comm_tile64 = in->comm_tile;
+        }
     });
 }
 

@@ -83,6 +83,7 @@ class Engine {
     FFTChoice fft_choice[2];  // autotuned kernel and tile per grid axis
     int comm_kind = kCommScalar;     // engine of the RHS commutator (precision-dependent)
     int comm_kind64 = kCommScalar;   // FP64 engine, also used for set-up products
+    int comm_tile = 0, comm_tile64 = 0;  // tile of each
 
     bool timing = false;
     std::vector<TimedLaunch> pending;

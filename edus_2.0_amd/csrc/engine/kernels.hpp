@@ -101,6 +101,7 @@ void launch_fft(const AxisFFTArgs& a, const FFTChoice& c, hipStream_t s);
 // Engines for [H, rho] in the commutator kernel: FP64 FMAs, FP64 MFMA, or
 // FP32 MFMA with FP32 accumulation (mixed precision; everything else FP64).
 enum CommKind { kCommScalar = 0, kCommMfma64 = 1, kCommMfma32 = 2 };
-void launch_comm(int nb, int layout, int kind, const CommArgs& a, hipStream_t s);
+// tile 0: 8 k-points per work-group, 1: 4 (MFMA engines; the FMA engine has one)
+void launch_comm(int nb, int layout, int kind, int tile, const CommArgs& a, hipStream_t s);
 
 }  // namespace edus

@@ -7,6 +7,7 @@ Workloads (dt = 0.05 a.u. throughout):
   ipa14_hex120      14-band hexagonal model, 120x120 k, RK4, 1.0 eV, 0.02 a.u., 20 fs
   hsex8_hex90       8-band hexagonal HSEX model, 90x90 k, AB4, circular 1.8 eV, 0.005 a.u., 15 fs
   hsex8_hex96       8-band hexagonal HSEX model, 96x96 k, RK4 (parareal workload)
+  ipa6_hex96        6-band hexagonal model, 96x96 k, RK4 (a batch member class)
 """
 from __future__ import annotations
 
@@ -148,6 +149,11 @@ def workload(name: str, outdir: str, grid=None, seed=None) -> dict:
         m = random_model(14, "hex", 3.0, 1.0, ons, 0.05, 0.05, 1 if seed is None else seed, soc=0.2)
         user = {"grid": grid or [120, 120, 1], "filledbands": 7, "finaltime": 20.0, "finaltime_units": "fs",
                 "lasers": [laser(0.02, 1.0, 20.0, (1, 0, 0))]}
+    elif name == "ipa6_hex96":
+        ons = list(np.linspace(-2.0, -1.0, 3)) + list(np.linspace(1.0, 2.0, 3))
+        m = random_model(6, "hex", 3.0, 1.0, ons, 0.05, 0.05, 5 if seed is None else seed)
+        user = {"grid": grid or [96, 96, 1], "filledbands": 3, "finaltime": 10.0, "finaltime_units": "fs",
+                "lasers": [laser(0.01, 1.2, 10.0, (1, 0, 0))]}
     elif name in ("hsex8_hex90", "hsex8_hex96"):
         ons = list(np.linspace(-3.0, -1.5, 4)) + list(np.linspace(1.5, 3.0, 4))
         a = 2.5

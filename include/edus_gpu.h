@@ -149,14 +149,18 @@ int edus_timing_get(edus_ctx* ctx, const char* kernel_class, double* total_ms, l
  *   fft_lines[a]   lines per work-group
  *   comm_kind      0: FP64 FMAs, 1: FP64 MFMA (EDUS_PREC_FP64);
  *                  2: FP32 MFMA (EDUS_PREC_MIXED, not settable)
- * Within one family, FFT tiles give bit-identical results; the two FP64
- * commutator engines differ in summation order only.
+ *   comm_tile      k-points per commutator work-group: 0: 8, 1: 4 (MFMA
+ *                  engines; the FMA engine takes 0)
+ * Within one family, FFT tiles give bit-identical results, and so do the
+ * commutator tiles; the two FP64 commutator engines differ in summation order
+ * only.
  * edus_set_tuning returns EDUS_ERR_ARG for a choice that does not fit. */
 typedef struct {
     int fft_family[2];
     int fft_cfg[2];
     int fft_lines[2];
     int comm_kind;
+    int comm_tile;
 } edus_tuning;
 int edus_get_tuning(edus_ctx* ctx, edus_tuning* out);
 int edus_set_tuning(edus_ctx* ctx, const edus_tuning* in);

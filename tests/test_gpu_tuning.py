@@ -72,6 +72,7 @@ def test_fft_tiles_give_identical_results(model_dir, name, grid, layout):
         t.fft_cfg[:] = [cfg0, cfg1]
         t.fft_lines[:] = [l0, l1]
         t.comm_kind = base.comm_kind
+        t.comm_tile = base.comm_tile
         try:
             eng.set_tuning(t)
         except edus.EdusError:
@@ -98,16 +99,18 @@ def test_fp64_commutator_engines(model_dir, name, grid, layout):
     f_ref = o.rhs_E(E, rho)
     scale = max(1.0, np.abs(f_ref).max())
     outs = {}
-    for kind in (0, 1):
+    for kind, tile in ((0, 0), (1, 0), (1, 1)):
         t = eng.get_tuning()
         t.comm_kind = kind
+        t.comm_tile = tile
         eng.set_tuning(t)
         f = eng.rhs(E, rho)
         err = np.abs(f - f_ref).max() if layout == FULL else np.abs(_upper(f) - _upper(f_ref)).max()
-        _say(name, grid, "layout", layout, "comm_kind", kind, "max|df| vs oracle", err, "scale", scale)
+        _say(name, grid, "layout", layout, "comm_kind", kind, "tile", tile, "max|df| vs oracle", err, "scale", scale)
         assert err <= 1e-11 * scale
-        outs[kind] = f
-    assert np.abs(outs[0] - outs[1]).max() <= 1e-13 * scale
+        outs[(kind, tile)] = f
+    assert np.abs(outs[(0, 0)] - outs[(1, 0)]).max() <= 1e-13 * scale
+    assert np.array_equal(outs[(1, 0)], outs[(1, 1)])  # tiles: same arithmetic per k-point
 
 
 @pytest.mark.parametrize("name,grid", [("ipa2_square30", [12, 15]), ("ipa14_hex120", [10, 9]),
