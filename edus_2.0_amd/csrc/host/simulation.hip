@@ -4,6 +4,9 @@
 // every per-step operation runs in the engine.
 #include <hip/hip_runtime.h>
 
+#include <fstream>
+#include <iomanip>
+
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -470,6 +473,44 @@ int edus_sim_propagate(edus_sim* sim, int nsteps, double* times, double* laser, 
         const int n = nsteps < 0 ? sim->s->nsteps : nsteps;
         const int k = sim->s->propagate(n, times, laser, pop_bloch, pop_wannier, vel);
         if (nprinted) *nprinted = k;
+    });
+}
+
+int edus_sim_write_outputs(edus_sim* sim, const char* dir, int nsteps, int* nprinted)
+{
+    if (!sim || !dir) { edus::set_error("null argument"); return EDUS_ERR_ARG; }
+    return guarded_sim([&] {
+        Simulation& S = *sim->s;
+        const int n = nsteps < 0 ? S.nsteps : nsteps;
+        const int nb = S.nb;
+        // at most one printed row per step
+        std::vector<double> t(n + 1), las(3 * size_t(n + 1)), pb(size_t(n + 1) * nb), pw(size_t(n + 1) * nb),
+            v(6 * size_t(n + 1));
+        const int k = S.propagate(n, t.data(), las.data(), pb.data(), pw.data(), v.data());
+        if (nprinted) *nprinted = k;
+        const std::string d(dir);
+        auto open = [&](const char* name) {
+            std::ofstream f(d + "/" + name);
+            if (!f) throw EngineError(EDUS_ERR_IO, "cannot write " + d + "/" + name);
+            return f;
+        };
+        // formats of Simulation.cpp:420-429 (Time: default stream, Laser: the
+        // Cartesian vector's setw 24 / 15 digits), :459-467 (populations: setw
+        // 30, 14 digits, blank after each band) and :614-620 (velocity: setw
+        // 20, 8 digits)
+        std::ofstream ft = open("Time.txt"), fl = open("Laser.txt"), fp = open("Population.txt"),
+                      fw = open("Population_wannier.txt"), fv = open("Velocity.txt");
+        for (int r = 0; r < k; ++r) {
+            ft << t[r] << std::endl;
+            for (int c = 0; c < 3; ++c) fl << std::setw(24) << std::setprecision(15) << las[3 * size_t(r) + c];
+            fl << std::endl;
+            for (int b = 0; b < nb; ++b) fp << std::setw(30) << std::setprecision(14) << pb[size_t(r) * nb + b] << " ";
+            fp << std::endl;
+            for (int b = 0; b < nb; ++b) fw << std::setw(30) << std::setprecision(14) << pw[size_t(r) * nb + b] << " ";
+            fw << std::endl;
+            for (int c = 0; c < 6; ++c) fv << std::setw(20) << std::setprecision(8) << v[6 * size_t(r) + c];
+            fv << std::endl;
+        }
     });
 }
 

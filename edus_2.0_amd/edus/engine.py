@@ -90,7 +90,7 @@ EXPORTS = [
     "edus_batch_load", "edus_batch_observables", "edus_batch_check", "edus_batch_synchronize", "edus_mlp_create",
     "edus_mlp_destroy", "edus_mlp_num_params", "edus_mlp_set_params", "edus_mlp_get_params", "edus_mlp_forward",
     "edus_mlp_gradients", "edus_mlp_train", "edus_sim_create", "edus_sim_destroy", "edus_sim_info",
-    "edus_sim_engine", "edus_sim_field", "edus_sim_time", "edus_sim_propagate", "edus_sim_steps", "edus_sim_get",
+    "edus_sim_engine", "edus_sim_field", "edus_sim_time", "edus_sim_propagate", "edus_sim_steps", "edus_sim_write_outputs", "edus_sim_get",
 ]
 
 
@@ -153,6 +153,7 @@ def lib():
         L.edus_sim_time.argtypes = [vp]
         L.edus_sim_propagate.argtypes = [vp, C.c_int, dp, dp, dp, dp, dp, ip]
         L.edus_sim_steps.argtypes = [vp, C.c_int]
+        L.edus_sim_write_outputs.argtypes = [vp, C.c_char_p, C.c_int, ip]
         L.edus_sim_get.restype = C.c_long
         L.edus_sim_get.argtypes = [vp, C.c_char_p, dp]
         _LIB = L
@@ -495,6 +496,13 @@ class Simulation:
 
     def steps(self, n: int):
         _check(lib().edus_sim_steps(self.h, int(n)))
+
+    def write_outputs(self, directory: str, nsteps: int | None = None) -> int:
+        """edus_sim_write_outputs: propagate and write the reference's text files."""
+        k = C.c_int()
+        _check(lib().edus_sim_write_outputs(self.h, directory.encode(), -1 if nsteps is None else int(nsteps),
+                                            C.byref(k)))
+        return k.value
 
     def propagate(self, nsteps: int | None = None) -> dict:
         n = self.nsteps if nsteps is None else int(nsteps)

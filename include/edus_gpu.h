@@ -307,6 +307,12 @@ int edus_sim_propagate(edus_sim* sim, int nsteps, double* times, double* laser, 
                        double* pop_wannier, double* vel, int* nprinted);
 /* Advance nsteps without observables (benchmark path). */
 int edus_sim_steps(edus_sim* sim, int nsteps);
+/* Propagate nsteps (< 0: the whole run) and write the reference's text
+ * outputs into directory dir: Time.txt, Laser.txt, Population.txt,
+ * Population_wannier.txt, Velocity.txt, one row per printed step in the
+ * reference's formats (Simulation.cpp:420-429,459-467,614-620).  Laser_A.txt
+ * and output.h5 are not written. */
+int edus_sim_write_outputs(edus_sim* sim, const char* dir, int nsteps, int* nprinted);
 /* Set-up arrays for inspection: "H0k", "rk0", "rk1", "rk2", "U", "Vel0",
  * "Vel1", "Vel2", "W", "rho0R", "hartree" ([nb][nb]), "eps" (real [Nk][nb]). */
 long edus_sim_get(edus_sim* sim, const char* name, double* out);

@@ -265,3 +265,19 @@ def test_error_paths():
     bad["tb_file"] = "/nonexistent/model"
     with pytest.raises(edus.EdusError):
         edus.Simulation(bad)
+
+
+def test_hbn_reference_output_files(tmp_path):
+    """The reference's output files written by the library, compared with its
+    own hBN fixtures column by column at compare.py's 1e-10."""
+    sim = edus.Simulation(hbn_params())
+    k = sim.write_outputs(str(tmp_path))
+    assert k == 102
+    for name in ("Time.txt", "Laser.txt", "Population.txt", "Velocity.txt"):
+        got = np.loadtxt(os.path.join(str(tmp_path), name))
+        gold = np.loadtxt(os.path.join(GOLDEN, "hBN", name))
+        assert got.shape == gold.shape, name
+        tol = 1e-10 if name != "Time.txt" else 5e-6 * np.abs(gold).max()  # Time.txt: 6 printed digits
+        _say(name, "max diff", np.abs(got - gold).max())
+        assert np.abs(got - gold).max() <= tol, name
+    assert np.loadtxt(os.path.join(str(tmp_path), "Population_wannier.txt")).shape == (102, 2)
