@@ -556,12 +556,16 @@ void Engine::monitor(const double2* rho)
     tend(t);
 }
 
-void Engine::rk4_step(double dt, const double* E3x3)
+void Engine::rk4_step(double dt, const double* E3x3, int record_order)
 {
     StageUpdate u{};
     // k1 = f(t, y);   aux = y + dt/2 k1;  acc = y + dt/6 k1      (DESolver.hpp:166-171)
     u.x1 = y; u.out1 = acc; u.b1 = dt / 6.;
     u.x2 = y; u.out2 = aux; u.b2 = dt / 2.;
+    if (record_order > 0) {
+        ensure_hist();
+        u.out_f = hist[abidx[0]];
+    }
     evaluate(y, E3x3 + 0, u);
     // k2 = f(t+dt/2, aux); aux = y + dt/2 k2; acc += dt/3 k2     (:174-177)
     u = StageUpdate{};
@@ -579,6 +583,11 @@ void Engine::rk4_step(double dt, const double* E3x3)
     u.x1 = acc; u.out1 = y; u.b1 = dt / 6.;
     u.mon = mon; u.mon_src = y; u.nf = double(nf);
     evaluate(aux, E3x3 + 6, u);
+    if (record_order > 0) {  // the same rotation as ab_step
+        const int tmp = abidx[record_order - 1];
+        for (int i = record_order - 1; i > 0; --i) abidx[i] = abidx[i - 1];
+        abidx[0] = tmp;
+    }
 }
 
 Engine::Engine(const std::vector<const Engine*>& members)
@@ -845,6 +854,19 @@ int edus_ab_steps(edus_ctx* ctx, double dt, int order, int nsteps, const double*
     if (nsteps < 0) { edus::set_error("nsteps < 0"); return EDUS_ERR_ARG; }
     return guarded([&] {
         for (int s = 0; s < nsteps; ++s) ctx->e->ab_step(dt, order, E + 3 * size_t(s));
+        HIP_CHECK(hipGetLastError());
+    });
+}
+
+int edus_ab_seed(edus_ctx* ctx, double dt, int order, const double* E)
+{
+    EDUS_REQUIRE(ctx && E);
+    if (order != 4 && order != 5) {
+        edus::set_error("order " + std::to_string(order) + "not implemented for AB");
+        return EDUS_ERR_UNSUPPORTED;
+    }
+    return guarded([&] {
+        for (int s = 0; s < order - 1; ++s) ctx->e->rk4_step(dt, E + 9 * size_t(s), order);
         HIP_CHECK(hipGetLastError());
     });
 }

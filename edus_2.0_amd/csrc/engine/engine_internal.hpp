@@ -94,7 +94,9 @@ class Engine {
     void upload(const double* host_full, double2* dst);       // [Nk][nb][nb] -> planes
     void download(const double2* src, double* host_full);     // planes -> [Nk][nb][nb]
     void evaluate(const double2* in, const double E[3], const StageUpdate& up);
-    void rk4_step(double dt, const double* E3x3);
+    // record_order > 0: stage 1's right-hand side f(t_n, y_n) is also stored as
+    // the newest Adams-Bashforth history entry of that order (RK4 start-up)
+    void rk4_step(double dt, const double* E3x3, int record_order = 0);
     // stacked instances: fields from the device tables at table step `ts`;
     // use_axis / use_xi switch off gradient axes and E.xi terms that are zero
     // for every instance of the chunk

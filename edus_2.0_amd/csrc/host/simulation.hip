@@ -67,8 +67,9 @@ void matmul(int n, const cplx* A, const cplx* B, cplx* C)
 class DESolver {
   public:
     enum SolverType { AB, RK };
-    void initialize(Engine* e, const SetOfLaser* l, SolverType t, int order)
+    void initialize(Engine* e, const SetOfLaser* l, SolverType t, int order, bool seed = false)
     {
+        seed_ = seed;
         engine_ = e;
         lasers_ = l;
         type_ = t;
@@ -105,7 +106,17 @@ class DESolver {
                 t += dt;
             }
             for (int s = 0; s < nstep; ++s) {
-                engine_->ab_step(dt, order_, &E[3 * size_t(s)]);
+                if (seed_ && taken_ < order_ - 1) {
+                    // RK4 start-up: its first-stage right-hand sides become the history
+                    double E3[9];
+                    lasers_->field(CurrentTime, E3);
+                    lasers_->field(CurrentTime + dt / 2., E3 + 3);
+                    lasers_->field(CurrentTime + dt, E3 + 6);
+                    engine_->rk4_step(dt, E3, order_);
+                } else {
+                    engine_->ab_step(dt, order_, &E[3 * size_t(s)]);
+                }
+                ++taken_;
                 CurrentTime += dt;
             }
         }
@@ -116,6 +127,8 @@ class DESolver {
     const SetOfLaser* lasers_ = nullptr;
     SolverType type_ = RK;
     int order_ = 4;
+    bool seed_ = false;  // RK4 start-up of AB (not the reference's behaviour)
+    long taken_ = 0;
     double InitialTime = 0., ResolutionTime = 0., CurrentTime = 0.;
 };
 
@@ -269,7 +282,8 @@ Simulation::Simulation(const edus_sim_params& p)
     engine->upload(reinterpret_cast<const double*>(rho0.data()), engine->y);
 
     const std::string solver = p.solver;
-    desolver.initialize(engine.get(), &setoflaser, solver == "AB" ? DESolver::AB : DESolver::RK, p.order);
+    desolver.initialize(engine.get(), &setoflaser, solver == "AB" ? DESolver::AB : DESolver::RK, p.order,
+                        p.ab_seed != 0);
     if (solver != "AB" && solver != "RK") throw EngineError(EDUS_ERR_ARG, "solver must be RK or AB");
     desolver.set_InitialTime(ti);
     desolver.set_ResolutionTime(dt);

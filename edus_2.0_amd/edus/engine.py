@@ -78,13 +78,13 @@ class _SimParams(C.Structure):
         ("nlasers", C.c_int),
         ("lasers", _Laser * 10),
         ("precision", C.c_int),
-        ("layout", C.c_int),
+        ("layout", C.c_int), ("ab_seed", C.c_int),
     ]
 
 
 EXPORTS = [
     "edus_create", "edus_clone", "edus_destroy", "edus_last_error", "edus_set_state", "edus_get_state", "edus_rhs",
-    "edus_rk4_steps", "edus_ab_steps", "edus_ab_reset", "edus_observables", "edus_check", "edus_synchronize", "edus_graph",
+    "edus_rk4_steps", "edus_ab_steps", "edus_ab_reset", "edus_ab_seed", "edus_observables", "edus_check", "edus_synchronize", "edus_graph",
     "edus_timing_enable", "edus_timing_get", "edus_get_tuning", "edus_set_tuning", "edus_batch_create",
     "edus_batch_destroy", "edus_batch_groups", "edus_batch_graph", "edus_batch_rk4_steps", "edus_batch_store",
     "edus_batch_load", "edus_batch_observables", "edus_batch_check", "edus_batch_synchronize", "edus_mlp_create",
@@ -113,6 +113,7 @@ def lib():
         L.edus_rk4_steps.argtypes = [vp, C.c_double, C.c_int, dp]
         L.edus_ab_steps.argtypes = [vp, C.c_double, C.c_int, C.c_int, dp]
         L.edus_ab_reset.argtypes = [vp]
+        L.edus_ab_seed.argtypes = [vp, C.c_double, C.c_int, dp]
         L.edus_observables.argtypes = [vp, dp, dp, dp]
         L.edus_check.argtypes = [vp, dp, dp]
         L.edus_synchronize.argtypes = [vp]
@@ -261,6 +262,11 @@ class Engine:
 
     def ab_reset(self):
         _check(lib().edus_ab_reset(self.h))
+
+    def ab_seed(self, dt: float, order: int, E) -> None:
+        """edus_ab_seed: order-1 RK4 steps (E [order-1][3][3]) that fill the AB history."""
+        e = np.ascontiguousarray(E, dtype=np.float64).reshape(-1, 3, 3)
+        _check(lib().edus_ab_seed(self.h, float(dt), int(order), _dp(e)))
 
     def observables(self):
         pb, pw, v = np.empty(self.nb), np.empty(self.nb), np.empty(6)
@@ -444,6 +450,7 @@ def _sim_params(p: dict, precision: int, layout: int) -> _SimParams:
         L.phase = float(las["phase"])
     s.precision = int(precision)
     s.layout = int(layout)
+    s.ab_seed = int(bool(p.get("ab_seed", False)))
     return s
 
 

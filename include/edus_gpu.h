@@ -111,6 +111,11 @@ int edus_rk4_steps(edus_ctx* ctx, double dt, int nsteps, const double* E);
 int edus_ab_steps(edus_ctx* ctx, double dt, int order, int nsteps, const double* E);
 /* Clear the Adams-Bashforth history (zero, as DESolver.cpp:26-29). */
 int edus_ab_reset(edus_ctx* ctx);
+/* Start-up of Adams-Bashforth by RK4 instead of the reference's zero history:
+ * order-1 RK4 steps (E as in edus_rk4_steps) whose first-stage right-hand
+ * sides become the history; edus_ab_steps continues from there.  No
+ * counterpart in the reference (it starts AB from zero history). */
+int edus_ab_seed(edus_ctx* ctx, double dt, int order, const double* E);
 
 /* Observables of the current state (Simulation.cpp:439-470, 583-622):
  * pop_bloch[nb] (filled bands as 1 - P_n), pop_wannier[nb], vel[6] =
@@ -290,6 +295,7 @@ typedef struct {
     edus_laser_params lasers[10];
     int precision;            /* EDUS_PREC_* */
     int layout;               /* EDUS_LAYOUT_* */
+    int ab_seed;              /* 1: start Adams-Bashforth with RK4 steps (edus_ab_seed), 0: zero history */
 } edus_sim_params;
 
 edus_sim* edus_sim_create(const edus_sim_params* p);

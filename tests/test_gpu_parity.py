@@ -281,3 +281,28 @@ def test_hbn_reference_output_files(tmp_path):
         _say(name, "max diff", np.abs(got - gold).max())
         assert np.abs(got - gold).max() <= tol, name
     assert np.loadtxt(os.path.join(str(tmp_path), "Population_wannier.txt")).shape == (102, 2)
+
+
+def test_ab4_seeded_by_rk4(model_dir):
+    """edus_ab_seed: the start-up steps are RK4 steps (bit for bit), and the
+    seeded AB4 run stays closer to fine RK4 than the reference's zero-history
+    start."""
+    p = _model("ipa2_square30", model_dir, [11, 13])
+    p["lasers"][0]["t0"] = -2.0
+    dt = 0.05
+    sim = edus.Simulation(p)
+    e0 = sim.engine
+    E9 = np.array([[sim.field(dt * (s + f)) for f in (0.0, 0.5, 1.0)] for s in range(400)])  # [400][3][3]
+    ref, seeded, zero = e0.clone(), e0.clone(), e0.clone()
+    rk3 = e0.clone()
+    rk3.rk4_steps(dt, E9[:3])
+    seeded.ab_seed(dt, 4, E9[:3])
+    assert np.array_equal(seeded.get_state(), rk3.get_state())
+    n = 200
+    seeded.ab_steps(dt, 4, E9[3:3 + n, 0, :])
+    zero.ab_steps(dt, 4, E9[:3 + n, 0, :])
+    ref.rk4_steps(dt, E9[:3 + n])
+    es = np.abs(seeded.get_state() - ref.get_state()).max()
+    ez = np.abs(zero.get_state() - ref.get_state()).max()
+    _say("AB4 vs RK4 after", 3 + n, "steps: seeded", es, "zero-history", ez)
+    assert es < ez
