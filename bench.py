@@ -71,6 +71,11 @@ def main():
     ap.add_argument("--graph", type=int, default=20,
                     help="RK4 steps per replayed hipGraph in the timed region (0: direct launches)")
     ap.add_argument("--cpu-steps", type=int, default=40)
+    ap.add_argument("--coarse-steps", type=int, default=10, help="parareal: coarse RK4 steps per slice")
+    ap.add_argument("--train-runs", type=int, default=64, help="parareal: random-pulse training runs")
+    ap.add_argument("--k-samples", type=int, default=256, help="parareal: k-points sampled per run and slice")
+    ap.add_argument("--epochs", type=int, default=20, help="parareal: training epochs")
+    ap.add_argument("--tol", type=float, default=1e-6, help="parareal: convergence tolerance")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
 
@@ -80,6 +85,9 @@ def main():
         sys.exit(2)
     if args.workload == "hetero_batch64":
         run_batch(args)
+        return
+    if args.workload == "parareal_hsex8_96":
+        run_parareal(args)
         return
 
     import numpy as np  # noqa: F401
@@ -228,3 +236,90 @@ def run_batch(args):
 
 if __name__ == "__main__":
     main()
+
+
+def run_parareal(args):
+    """Learned parareal on the 8-band HSEX model, 96^2 k: 32 slices of 200
+    fine RK4 steps (dt 0.05 a.u.) of the pulse of the hsex8_hex96 workload.
+    Training data: 64 runs with random pulses (2048 slice trajectories) of the
+    plain method; the network is trained on the GPU.  Reported: wall time of
+    sequential GPU RK4 (graph replay), of parareal with the coarse RK4 alone
+    (the method without the network) and with the network, iterations, and
+    max|d rho| of both against sequential RK4 at every slice boundary."""
+    import numpy as np
+
+    import edus
+    from edus import models
+    from edus import parareal as pr
+
+    N, FINE, COARSE, DT = 32, 200, args.coarse_steps, 0.05
+    tmp = tempfile.mkdtemp(prefix="edus_para_")
+    params = models.workload("hsex8_hex96", tmp)
+    sim = edus.Simulation(params)
+    eng = sim.engine
+
+    def field(t):
+        t = np.asarray(t, dtype=float)
+        return np.array([sim.field(x) for x in t.ravel()]).reshape(t.shape + (3,))
+
+    Ef, Ec, Es = pr.slice_tables(field, N, FINE, COARSE, DT)
+    # sequential GPU RK4
+    seq = eng.clone()
+    seq.graph(50)
+    seq.rk4_steps(DT, Ef[0][:50])  # capture outside the timed region
+    seq.set_state(eng.get_state())
+    seq.synchronize()
+    t0 = time.perf_counter()
+    bounds = [None] * (N + 1)
+    for n in range(N):
+        seq.rk4_steps(DT, Ef[n])
+    seq.synchronize()
+    t_seq = time.perf_counter() - t0
+    # slice boundaries of the sequential run (untimed)
+    chk = eng.clone()
+    bounds[0] = chk.get_state()
+    for n in range(N):
+        chk.rk4_steps(DT, Ef[n])
+        bounds[n + 1] = chk.get_state()
+    # training data and training (one-off costs, reported separately)
+    t1 = time.perf_counter()
+    pulses = pr.random_pulses(args.train_runs, seed=11)
+    tabs = [pr.slice_tables(pl, N, FINE, COARSE, DT) for pl in pulses]
+    P = pr.Params(N, FINE, COARSE, DT, args.tol, 6, 50)
+    X, T = pr.collect(eng, P, np.stack([t[0] for t in tabs]), np.stack([t[1] for t in tabs]),
+                      np.stack([t[2] for t in tabs]), k_samples=args.k_samples, seed=5)
+    t_collect = time.perf_counter() - t1
+    t2 = time.perf_counter()
+    stats = pr.normalisation(X, T)
+    net, hist = pr.train_network(X, T, stats, hidden=256, epochs=args.epochs, batch=1024, lr=1e-3, seed=7)
+    t_train = time.perf_counter() - t2
+
+    def run(with_net):
+        para = pr.Parareal(eng, N, FINE, COARSE, DT, tol=args.tol, max_iter=6, graph_steps=50)
+        para.set_fields(Ef, Ec, Es)
+        if with_net:
+            para.set_network(net, stats)
+        para.run()  # warm-up: graphs captured, tiles tuned
+        res = para.run()
+        err = max(float(np.abs(para.state(n) - bounds[n]).max()) for n in range(N + 1))
+        res["max_drho_vs_sequential"] = err
+        return res
+
+    plain = run(False)
+    learned = run(True)
+    out = {
+        "metric": "learned-parareal wall time vs plain GPU RK4 (8-band HSEX, 96^2 k, 32 x 200 steps)",
+        "value": learned["seconds"], "unit": "s", "n_gpus": 1, "steps": N * FINE, "warmup": 0,
+        "ms_per_step": learned["seconds"] / (N * FINE) * 1e3, "higher_is_better": False, "scaling": "strong",
+        "vs_baseline": None, "dtype": "f64 (network bf16/f32)", "data": "synthetic",
+        "config": {"workload": "parareal_hsex8_96", "slices": N, "fine_steps": FINE, "coarse_steps": COARSE,
+                   "dt_au": DT, "tol": args.tol, "max_iter": 6, "train_runs": args.train_runs,
+                   "k_samples": args.k_samples, "epochs": args.epochs, "network": f"{X.shape[1]}-256-256-{T.shape[1]}"},
+        "sequential_rk4_s": t_seq,
+        "parareal_plain": plain,
+        "parareal_learned": learned,
+        "speedup_learned_vs_sequential": t_seq / learned["seconds"],
+        "training": {"rows": int(X.shape[0]), "collect_s": t_collect, "train_s": t_train,
+                     "loss_first": float(hist[0]), "loss_last": float(hist[-1])},
+    }
+    print(json.dumps(out), flush=True)

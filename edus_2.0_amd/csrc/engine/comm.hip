@@ -185,10 +185,19 @@ __global__ void __launch_bounds__(kCommThreads) comm_kernel(CommArgs a)
     double2* sR = lds;
     double2* sH = lds + NPL * PS;
     const int Nk = a.Nk;
-    // blocks cover the tiles of instance 0, then instance 1, ...
+    // Work-groups are dealt to the 8 XCDs round-robin (block b runs on XCD
+    // b % 8); give each XCD a contiguous range of tiles so that tiles sharing
+    // a cache line (TK = 4) meet in the same L2.  t covers the tiles of
+    // instance 0, then instance 1, ...
     const int ntile = (Nk + TK - 1) / TK;
-    const int inst = blockIdx.x / ntile;
-    const int k0 = (blockIdx.x - inst * ntile) * TK;
+    int t;
+    {
+        const int nblk = gridDim.x, q = nblk / 8, r = nblk % 8;
+        const int x = blockIdx.x % 8, i = blockIdx.x / 8;
+        t = x * q + min(x, r) + i;
+    }
+    const int inst = t / ntile;
+    const int k0 = (t - inst * ntile) * TK;
     const long ioff = long(inst) * NPL * Nk;
     double Ev[3];
 #pragma unroll
