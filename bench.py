@@ -167,6 +167,11 @@ def main():
     print(json.dumps(out), flush=True)
 
 
+def _progress(*a):
+    """One line per finished phase, to stderr."""
+    print("bench:", *a, file=sys.stderr, flush=True)
+
+
 def run_batch(args):
     """Heterogeneous batch: 64 pulse runs over nb in {2, 6, 14} and grids
     48^2, 96^2, 120^2 (one seeded model per class), peak fields log-spaced
@@ -187,6 +192,7 @@ def run_batch(args):
     for w, g in classes:
         p = models.workload(w, os.path.join(tmp, f"{w}_{g}"), grid=[g, g, 1])
         sims[(w, g)] = edus.Simulation(p)
+        _progress("set up", w, g)
     n = 64
     e0 = np.logspace(np.log10(1e-3), np.log10(3e-2), n)
     members, pulses, nks = [], [], []
@@ -208,6 +214,7 @@ def run_batch(args):
     for m in members:
         m.synchronize()
     t_seq = time.perf_counter() - t0
+    _progress("member by member", t_seq, "s")
     # as one batch, from where the members stand
     batch = edus.Batch(members)
     ng, _ = batch.groups()
@@ -257,6 +264,7 @@ def run_parareal(args):
     params = models.workload("hsex8_hex96", tmp)
     sim = edus.Simulation(params)
     eng = sim.engine
+    _progress("set up")
 
     def field(t):
         t = np.asarray(t, dtype=float)
@@ -275,6 +283,7 @@ def run_parareal(args):
         seq.rk4_steps(DT, Ef[n])
     seq.synchronize()
     t_seq = time.perf_counter() - t0
+    _progress("sequential RK4", t_seq, "s")
     # slice boundaries of the sequential run (untimed)
     chk = eng.clone()
     bounds[0] = chk.get_state()
@@ -289,10 +298,12 @@ def run_parareal(args):
     X, T = pr.collect(eng, P, np.stack([t[0] for t in tabs]), np.stack([t[1] for t in tabs]),
                       np.stack([t[2] for t in tabs]), k_samples=args.k_samples, seed=5)
     t_collect = time.perf_counter() - t1
+    _progress("training data", X.shape, t_collect, "s")
     t2 = time.perf_counter()
     stats = pr.normalisation(X, T)
     net, hist = pr.train_network(X, T, stats, hidden=256, epochs=args.epochs, batch=1024, lr=1e-3, seed=7)
     t_train = time.perf_counter() - t2
+    _progress("training", t_train, "s, loss", float(hist[0]), "->", float(hist[-1]))
 
     def run(with_net):
         para = pr.Parareal(eng, N, FINE, COARSE, DT, tol=args.tol, max_iter=6, graph_steps=50)
@@ -303,6 +314,7 @@ def run_parareal(args):
         res = para.run()
         err = max(float(np.abs(para.state(n) - bounds[n]).max()) for n in range(N + 1))
         res["max_drho_vs_sequential"] = err
+        _progress("parareal", "with" if with_net else "without", "network", res)
         return res
 
     plain = run(False)
