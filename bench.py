@@ -241,9 +241,6 @@ def run_batch(args):
     print(json.dumps(out), flush=True)
 
 
-if __name__ == "__main__":
-    main()
-
 
 def run_parareal(args):
     """Learned parareal on the 8-band HSEX model, 96^2 k: 32 slices of 200
@@ -335,3 +332,7 @@ def run_parareal(args):
                      "loss_first": float(hist[0]), "loss_last": float(hist[-1])},
     }
     print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

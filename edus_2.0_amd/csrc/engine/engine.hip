@@ -102,6 +102,7 @@ Engine::Engine(const edus_desc& d, const EngineOptions& opt)
     prec = d.precision;
     nf = d.filledbands;
     coulomb = d.coulomb != 0;
+    peierls = d.peierls != 0;
     std::memcpy(A, d.A, sizeof(A));
     {
         std::array<double, 9> a;
@@ -157,6 +158,10 @@ Engine::Engine(const edus_desc& d, const EngineOptions& opt)
     }
 
     autotune();
+    if (peierls) {
+        alloc(reinterpret_cast<void**>(&Hp), pb);
+        alloc(reinterpret_cast<void**>(&Tq), pb);
+    }
     if (opt.velocity_det > 0.0) build_velocity(opt.velocity_det);
 
     if (coulomb) {
@@ -210,14 +215,15 @@ void Engine::build_velocity(double det)
     HIP_CHECK(hipStreamSynchronize(s));
 }
 
-void Engine::observables(double* pop_bloch, double* pop_wannier, double* vel_out, int inst)
+void Engine::observables(double* pop_bloch, double* pop_wannier, double* vel_out, int inst,
+                         const double2* rho_override)
 {
     const size_t io = size_t(inst) * inst_elems();
     ObsArgs a{};
     a.nb = nb;
     a.Nk = Nk;
     a.layout = layout;
-    a.rho = y + io;
+    a.rho = rho_override ? rho_override : y + io;
     a.U = U ? U + size_t(inst) * nb * nb * Nk : nullptr;
     for (int d = 0; d < 3; ++d) a.vel[d] = vel[d] ? vel[d] + io : nullptr;
     a.partial = obs_partial;
@@ -609,6 +615,7 @@ Engine::Engine(const std::vector<const Engine*>& members)
         if (m->nb != nb || m->N0 != N0 || m->N1 != N1 || m->layout != layout || m->prec != prec ||
             m->coulomb != coulomb || m->nf != nf || m->B != 1)
             throw EngineError(EDUS_ERR_ARG, "batch group members differ in shape");
+        if (m->peierls) throw EngineError(EDUS_ERR_UNSUPPORTED, "Peierls-gauge contexts cannot be batched");
         std::array<double, 9> a;
         std::memcpy(a.data(), m->A, sizeof(m->A));
         Ainst.push_back(a);
@@ -737,6 +744,105 @@ void Engine::rk4_step_tab(double dt, int ts, const bool use_axis[2], const bool 
     u.x1 = acc; u.out1 = y; u.b1 = dt / 6.;
     u.mon = mon; u.mon_src = y; u.nf = double(nf);
     evaluate_tab(aux, ts, 2, use_axis, use_xi, u);
+}
+
+// F^-1 diag(exp(i sign A.R)) F applied to (in + sum_d addc_d add_d), one
+// axis after the other: A.R = R0 (A.a0) + R1 (A.a1) on the Gamma-centred grid
+// is a sum, so the phase factorises (Simulation.cpp:876-899 applies it to the
+// whole R-space array at once).
+void Engine::peierls_sandwich(const double2* in, const double2* const add[3], const double addc[3],
+                              const double Av[3], int sign, double2* out)
+{
+    double th[2];
+    for (int j = 0; j < 2; ++j) th[j] = sign * (Av[0] * A[0][j] + Av[1] * A[1][j] + Av[2] * A[2][j]);
+    const double2* src = in;
+    const bool two = N0 > 1 && N1 > 1;
+    for (int axis = 0; axis < 2; ++axis) {
+        const int n = axis == 0 ? N0 : N1;
+        if (n < 2) continue;
+        AxisFFTArgs a = axis_args(axis, fft_choice[axis], 4, src, nullptr, (two && axis == 0) ? Tq : out, th[axis],
+                                  1.0 / n, nullptr);
+        if (src == in)
+            for (int d = 0; d < 3; ++d) {
+                a.in_add[d] = add ? add[d] : nullptr;
+                a.in_add_c[d] = addc ? addc[d] : 0.0;
+            }
+        TimedLaunch t;
+        tbegin(axis == 0 ? "peierls_col" : "peierls_row", t);
+        launch_fft(a, fft_choice[axis], s);
+        tend(t);
+        src = (two && axis == 0) ? Tq : out;
+    }
+    if (N0 < 2 && N1 < 2) {  // one k-point: the phase at R = 0 is 1
+        HIP_CHECK(hipMemcpyAsync(out, in, plane_bytes(), hipMemcpyDeviceToDevice, s));
+        for (int d = 0; d < 3; ++d)
+            if (add && add[d] && addc[d] != 0.0) launch_axpy(out, add[d], addc[d], long(npl) * Nk, s);
+    }
+}
+
+void Engine::evaluate_peierls(const double2* in, const double E[3], const double Av[3], const StageUpdate& up)
+{
+    if (!peierls) throw EngineError(EDUS_ERR_STATE, "context was not created for the Peierls gauge");
+    // H(k) = S_A[H0 + E.xi] (+ exchange of the current state, whose phases
+    // cancel; H0 already holds the rho0 terms, inside the phase as the
+    // reference has them)
+    const double2* add[3] = {xi[0], xi[1], xi[2]};
+    peierls_sandwich(H0, add, E, Av, +1, Hp);
+    if (coulomb) fock(in);
+    CommArgs c{};
+    c.Nk = Nk;
+    c.P = npl;
+    c.rho = in;
+    c.D = nullptr;
+    c.H0 = Hp;
+    c.sigma = coulomb ? T : nullptr;
+    c.hdiag = coulomb ? hdiag : nullptr;
+    c.up = up;
+    TimedLaunch t;
+    tbegin("comm", t);
+    launch_comm(nb, layout, comm_kind, comm_tile, c, s);
+    tend(t);
+}
+
+void Engine::rk4_step_peierls(double dt, const double* E3x3, const double* A3x3)
+{
+    StageUpdate u{};
+    u.x1 = y; u.out1 = acc; u.b1 = dt / 6.;
+    u.x2 = y; u.out2 = aux; u.b2 = dt / 2.;
+    evaluate_peierls(y, E3x3 + 0, A3x3 + 0, u);
+    u = StageUpdate{};
+    u.x1 = acc; u.out1 = acc; u.b1 = dt / 3.;
+    u.x2 = y; u.out2 = aux; u.b2 = dt / 2.;
+    evaluate_peierls(aux, E3x3 + 3, A3x3 + 3, u);
+    u = StageUpdate{};
+    u.x1 = acc; u.out1 = acc; u.b1 = dt / 3.;
+    u.x2 = y; u.out2 = aux; u.b2 = dt;
+    evaluate_peierls(aux, E3x3 + 3, A3x3 + 3, u);
+    u = StageUpdate{};
+    u.x1 = acc; u.out1 = y; u.b1 = dt / 6.;
+    u.mon = mon; u.mon_src = y; u.nf = double(nf);
+    evaluate_peierls(aux, E3x3 + 6, A3x3 + 6, u);
+}
+
+void Engine::ab_step_peierls(double dt, int order, const double* E3, const double* A3)
+{
+    ensure_hist();
+    static const double b4[5] = {55. / 24., -59. / 24., 37. / 24., -3. / 8., 0.};
+    static const double b5[5] = {1901. / 720., -2774. / 720., 2616. / 720., -1274. / 720., 251. / 720.};
+    const double* beta = order == 4 ? b4 : b5;
+    StageUpdate u{};
+    u.out_f = hist[abidx[0]];
+    u.ab_order = order;
+    u.y = y;
+    for (int i = 0; i < order; ++i) {
+        u.ab_c[i] = dt * beta[i];
+        u.hist[i] = hist[abidx[i]];
+    }
+    u.mon = mon; u.mon_src = y; u.nf = double(nf);
+    evaluate_peierls(y, E3, A3, u);
+    const int tmp = abidx[order - 1];
+    for (int i = order - 1; i > 0; --i) abidx[i] = abidx[i - 1];
+    abidx[0] = tmp;
 }
 
 void Engine::ensure_hist()
@@ -868,6 +974,55 @@ int edus_ab_seed(edus_ctx* ctx, double dt, int order, const double* E)
     return guarded([&] {
         for (int s = 0; s < order - 1; ++s) ctx->e->rk4_step(dt, E + 9 * size_t(s), order);
         HIP_CHECK(hipGetLastError());
+    });
+}
+
+int edus_rhs_peierls(edus_ctx* ctx, const double E[3], const double Av[3], const double* in, double* out)
+{
+    EDUS_REQUIRE(ctx && E && Av && in && out);
+    return guarded([&] {
+        Engine& e = *ctx->e;
+        e.upload(in, e.aux);
+        edus::StageUpdate u{};
+        u.out_f = e.f;
+        e.evaluate_peierls(e.aux, E, Av, u);
+        e.download(e.f, out);
+    });
+}
+
+int edus_rk4_steps_peierls(edus_ctx* ctx, double dt, int nsteps, const double* E, const double* Av)
+{
+    EDUS_REQUIRE(ctx && (nsteps <= 0 || (E && Av)));
+    if (nsteps < 0) { edus::set_error("nsteps < 0"); return EDUS_ERR_ARG; }
+    return guarded([&] {
+        for (int s = 0; s < nsteps; ++s) ctx->e->rk4_step_peierls(dt, E + 9 * size_t(s), Av + 9 * size_t(s));
+        HIP_CHECK(hipGetLastError());
+    });
+}
+
+int edus_ab_steps_peierls(edus_ctx* ctx, double dt, int order, int nsteps, const double* E, const double* Av)
+{
+    EDUS_REQUIRE(ctx && (nsteps <= 0 || (E && Av)));
+    if (order != 4 && order != 5) {
+        edus::set_error("order " + std::to_string(order) + "not implemented for AB");
+        return EDUS_ERR_UNSUPPORTED;
+    }
+    if (nsteps < 0) { edus::set_error("nsteps < 0"); return EDUS_ERR_ARG; }
+    return guarded([&] {
+        for (int s = 0; s < nsteps; ++s) ctx->e->ab_step_peierls(dt, order, E + 3 * size_t(s), Av + 3 * size_t(s));
+        HIP_CHECK(hipGetLastError());
+    });
+}
+
+int edus_observables_peierls(edus_ctx* ctx, const double Av[3], double* pop_bloch, double* pop_wannier,
+                             double* vel)
+{
+    EDUS_REQUIRE(ctx && Av);
+    return guarded([&] {
+        Engine& e = *ctx->e;
+        if (!e.peierls) throw EngineError(EDUS_ERR_STATE, "context was not created for the Peierls gauge");
+        e.peierls_sandwich(e.y, nullptr, nullptr, Av, -1, e.G);  // G is free between steps
+        e.observables(pop_bloch, pop_wannier, vel, 0, e.G);
     });
 }
 

@@ -58,6 +58,8 @@ class Engine {
     int nb, N0, N1, Nk, npl, layout, prec, nf;
     int B = 1;  // stacked instances
     bool coulomb;
+    bool peierls = false;
+    double2 *Hp = nullptr, *Tq = nullptr;  // Peierls: phased H(k) and its axis-0 half
     double A[3][3];
     std::vector<std::array<double, 9>> Ainst;  // lattice of each stacked instance
 
@@ -94,6 +96,13 @@ class Engine {
     void upload(const double* host_full, double2* dst);       // [Nk][nb][nb] -> planes
     void download(const double2* src, double* host_full);     // planes -> [Nk][nb][nb]
     void evaluate(const double2* in, const double E[3], const StageUpdate& up);
+    // Peierls gauge: the same stage with field E and vector potential Av
+    void evaluate_peierls(const double2* in, const double E[3], const double Av[3], const StageUpdate& up);
+    void rk4_step_peierls(double dt, const double* E3x3, const double* A3x3);
+    void ab_step_peierls(double dt, int order, const double* E3, const double* A3);
+    // out = planes of the phase sandwich F^-1 exp(i sign A.R) F in (both axes)
+    void peierls_sandwich(const double2* in, const double2* const add[3], const double addc[3], const double Av[3],
+                          int sign, double2* out);
     // record_order > 0: stage 1's right-hand side f(t_n, y_n) is also stored as
     // the newest Adams-Bashforth history entry of that order (RK4 start-up)
     void rk4_step(double dt, const double* E3x3, int record_order = 0);
@@ -111,7 +120,8 @@ class Engine {
     Batch* self_batch = nullptr;
     void ab_step(double dt, int order, const double* E3);
     void monitor(const double2* rho);
-    void observables(double* pop_bloch, double* pop_wannier, double* vel, int inst = 0);
+    void observables(double* pop_bloch, double* pop_wannier, double* vel, int inst = 0,
+                     const double2* rho_override = nullptr);
     void collect_timing();
 
   private:

@@ -182,6 +182,7 @@ __device__ double2* run_fft(double2* a, double2* b, int lines, int n, const FFTF
 // mode 1: forward     out = scale * F in          (sign -1)
 // mode 2: inverse     out = scale * F^-1 in       (sign +1)
 // mode 3: convolution out = F^-1 diag(scale * mult(R)) F in
+// mode 4: phase       out = F^-1 diag(scale exp(i coef R)) F (in + sum_d c_d in_add_d)
 template <int NT, int MAXV>
 __global__ void __launch_bounds__(NT) axis_fft_kernel(AxisFFTArgs a)
 {
@@ -224,7 +225,19 @@ __global__ void __launch_bounds__(NT) axis_fft_kernel(AxisFFTArgs a)
             const int idx = threadIdx.x + e * NT;
             const int c = a.axis == 1 ? idx / n : idx % L;
             const int j = a.axis == 1 ? idx - c * n : idx / L;
-            if (idx < span && c < nlines) v[e] = a.in[base + c * cstride + long(j) * rowstride];
+            if (idx < span && c < nlines) {
+                const long o = base + c * cstride + long(j) * rowstride;
+                double2 x = a.in[o];
+                if (a.mode == 4) {  // input: a sum of fields
+                    for (int d = 0; d < 3; ++d)
+                        if (a.in_add[d]) {
+                            const double2 w = a.in_add[d][o];
+                            x.x += a.in_add_c[d] * w.x;
+                            x.y += a.in_add_c[d] * w.y;
+                        }
+                }
+                v[e] = x;
+            }
         }
         for (int i = threadIdx.x; i < n; i += NT) twl[i] = a.tw[i];
 #pragma unroll
@@ -238,9 +251,16 @@ __global__ void __launch_bounds__(NT) axis_fft_kernel(AxisFFTArgs a)
     __syncthreads();
 
     double2* res;
-    if (a.mode == 0 || a.mode == 3) {
+    if (a.mode == 0 || a.mode == 3 || a.mode == 4) {
         double2* x = run_fft<NT>(bufA, bufB, nlines, n, a.fac, twl, -1.0);
-        if (a.mode == 0) {
+        if (a.mode == 4) {  // scale exp(i coef R)
+            for (int idx = threadIdx.x; idx < nlines * n; idx += NT) {
+                const int j = idx % n;
+                double sn, cs;
+                sincos(a.coef * double(gamma_centred(j, n)), &sn, &cs);
+                x[idx] = cmul(x[idx], make_double2(a.scale * cs, a.scale * sn));
+            }
+        } else if (a.mode == 0) {
             for (int idx = threadIdx.x; idx < nlines * n; idx += NT) {
                 const int c = idx / n;
                 const int j = idx - c * n;

@@ -12,8 +12,10 @@
 // The derivative sandwich (mode 0) multiplies by i coef R(k) and runs the
 // inverse steps B^-1 (same thread, no exchange), LDS, A^-1, so one line costs
 // two barriers and two LDS round trips.  Mode 3 multiplies by a real field
-// instead (the Hartree-Fock convolution along this axis), mode 1 stops after
-// step B (forward transform), mode 2 starts at step B^-1 (inverse).
+// instead (the Hartree-Fock convolution along this axis), mode 4 by
+// scale exp(i coef R) (the Peierls phase along this axis, its input a sum
+// in + sum_d c_d in_add_d), mode 1 stops after step B (forward transform),
+// mode 2 starts at step B^-1 (inverse).
 //
 // Every thread owns at most one A task and one B task (L*max(RA,RB) <= NT,
 // checked on the host), so each thread's global loads are all issued before
@@ -32,6 +34,31 @@ namespace {
 using reg::sfor;
 
 constexpr int kTwoStepThreads = 256;
+
+// the kernel's input value at global offset o (mode 4: a sum of fields)
+template <int MODE>
+__device__ inline double2 load_in(const AxisFFTArgs& a, long o)
+{
+    double2 v = a.in[o];
+    if constexpr (MODE == 4) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+            if (a.in_add[d]) {
+                const double2 w = a.in_add[d][o];
+                v.x += a.in_add_c[d] * w.x;
+                v.y += a.in_add_c[d] * w.y;
+            }
+    }
+    return v;
+}
+
+// mode 4 multiplier scale * exp(i coef R)
+__device__ inline double2 phase(const AxisFFTArgs& a, int R)
+{
+    double sn, cs;
+    sincos(a.coef * double(R), &sn, &cs);
+    return make_double2(a.scale * cs, a.scale * sn);
+}
 
 template <int RA, int RB>
 struct TwoStepShape {
@@ -92,7 +119,7 @@ __global__ void __launch_bounds__(NT) two_step_kernel(AxisFFTArgs a)
     const long gA = base + lineA * cstride;
     const long gB = base + lineB * cstride;
     constexpr int mode = MODE;
-    constexpr bool sandwich = mode == 0 || mode == 3;
+    constexpr bool sandwich = mode == 0 || mode == 3 || mode == 4;
 
     if constexpr (RB == 1) {
         // the whole line in one thread: no exchange
@@ -100,7 +127,7 @@ __global__ void __launch_bounds__(NT) two_step_kernel(AxisFFTArgs a)
         double2 v[RA];
         sfor<RA>([&](auto j_) {
             constexpr int j = decltype(j_)::value;
-            v[j] = a.in[gA + long(j) * rowstride];
+            v[j] = load_in<mode>(a, gA + long(j) * rowstride);
         });
         double mul[RA];
         if constexpr (mode == 3) {
@@ -124,6 +151,11 @@ __global__ void __launch_bounds__(NT) two_step_kernel(AxisFFTArgs a)
                     constexpr int k = decltype(k_)::value;
                     v[k] = cscale(v[k], a.scale * mul[k]);
                 });
+            } else if constexpr (mode == 4) {
+                sfor<RA>([&](auto k_) {
+                    constexpr int k = decltype(k_)::value;
+                    v[k] = cmul(v[k], phase(a, gamma_centred(k, RA)));
+                });
             }
             if constexpr (sandwich) reg::rdft<RA, 1>(v);
         }
@@ -146,7 +178,7 @@ __global__ void __launch_bounds__(NT) two_step_kernel(AxisFFTArgs a)
             if (doA) {
                 sfor<RA>([&](auto j_) {
                     constexpr int j = decltype(j_)::value;
-                    v[j] = a.in[gA + long(j * RB) * rowstride + long(jb) * rowstride];
+                    v[j] = load_in<mode>(a, gA + long(j * RB) * rowstride + long(jb) * rowstride);
                 });
             }
             for (int i = tid; i < n; i += NT) twl[i] = a.tw[i];
@@ -228,6 +260,11 @@ __global__ void __launch_bounds__(NT) two_step_kernel(AxisFFTArgs a)
                             constexpr int k = decltype(k_)::value;
                             u[k] = cmul_i(u[k], coef * double(gamma_centred(ka + RA * k, n)));
                         });
+                    } else if constexpr (mode == 4) {
+                        sfor<RB>([&](auto k_) {
+                            constexpr int k = decltype(k_)::value;
+                            u[k] = cmul(u[k], phase(a, gamma_centred(ka + RA * k, n)));
+                        });
                     } else {
                         sfor<RB>([&](auto k_) {
                             constexpr int k = decltype(k_)::value;
@@ -302,7 +339,8 @@ void launch_two(const AxisFFTArgs& a, int variant, hipStream_t s)
             break;
         case 1: launch_mode<RA, RB, 1, false>(a, blocks, lds, s); break;
         case 2: launch_mode<RA, RB, 2, false>(a, blocks, lds, s); break;
-        default: launch_mode<RA, RB, 3, false>(a, blocks, lds, s); break;
+        case 3: launch_mode<RA, RB, 3, false>(a, blocks, lds, s); break;
+        default: launch_mode<RA, RB, 4, false>(a, blocks, lds, s); break;
     }
 }
 

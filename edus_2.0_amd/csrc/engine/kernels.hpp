@@ -19,7 +19,7 @@ struct AxisFFTArgs {
     int axis;               // 0: columns (length N0), 1: rows (length N1)
     int n;                  // line length
     int L;                  // lines per work-group
-    int mode;               // 0 derivative, 1 forward, 2 inverse, 3 convolution
+    int mode;               // 0 derivative, 1 forward, 2 inverse, 3 convolution, 4 phase
     double coef;            // derivative: c_axis / n
     // derivative of stacked instances: plane p uses coef_tab[(p / ppi) * coef_stride]
     // (instance p / ppi, ppi = planes per instance) instead of coef
@@ -29,6 +29,9 @@ struct AxisFFTArgs {
     const double* mult;     // convolution: real multiplier field, triangle planes [P][N0][N1]
     FFTFactors fac;
     const double2* tw;      // exp(-2 pi i j / n), j < n
+    // mode 4 input: in + sum_d in_add_c[d] * in_add[d] (null terms skipped)
+    const double2* in_add[3];
+    double in_add_c[3];
 };
 
 // Per-stage update rule of the fused commutator kernel.  With f the

@@ -246,6 +246,23 @@ void SetOfLaser::field(double t, double E[3]) const
     }
 }
 
+void SetOfLaser::vector_potential(double t, double A[3])
+{
+    A[0] = A[1] = A[2] = 0.;
+    for (const Laser& L : lasers) {
+        const double dT = t - A_prev;
+        if (!(dT < 1.e-07)) {
+            double E1[3], E2[3], E3[3];
+            L.field(A_prev, E1);
+            L.field(A_prev + dT / 2., E2);
+            L.field(t, E3);
+            for (int d = 0; d < 3; ++d) Astate[d] -= dT / 6. * (E1[d] + 4. * E2[d] + E3[d]);
+            A_prev = t;
+        }
+        for (int d = 0; d < 3; ++d) A[d] += Astate[d];
+    }
+}
+
 // ------------------------------------------------------------ struve H0
 // Zhang & Jin, "Computation of Special Functions" (1996), STVH0: power series
 // for x <= 20, asymptotic series plus Y0 expansion above (the form the

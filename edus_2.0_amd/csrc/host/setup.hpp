@@ -63,6 +63,13 @@ struct Laser {
 struct SetOfLaser {
     std::vector<Laser> lasers;
     void field(double t, double E[3]) const;
+    // SetOfLaser::VectorPotential (Laser.hpp:152-158) over Laser::VectorPotential
+    // (Laser.cpp:136-152).  The reference keeps A and the previous call time in
+    // function-level statics shared by every laser: the first laser integrates
+    // its own field, A -= dT/6 (E1 + 4 E2 + E3), from the previous call time,
+    // and every laser adds the same A.  Here the state lives in this object.
+    void vector_potential(double t, double A[3]);
+    double Astate[3] = {0, 0, 0}, A_prev = 0.0;
 };
 
 // src/ModelCoulomb/ModelCoulomb.cpp

@@ -67,9 +67,10 @@ void matmul(int n, const cplx* A, const cplx* B, cplx* C)
 class DESolver {
   public:
     enum SolverType { AB, RK };
-    void initialize(Engine* e, const SetOfLaser* l, SolverType t, int order, bool seed = false)
+    void initialize(Engine* e, SetOfLaser* l, SolverType t, int order, bool seed = false, bool peierls = false)
     {
         seed_ = seed;
+        peierls_ = peierls;
         engine_ = e;
         lasers_ = l;
         type_ = t;
@@ -88,15 +89,26 @@ class DESolver {
         if (nstep <= 0) return;
         const double dt = ResolutionTime;
         if (type_ == RK) {
-            std::vector<double> E(size_t(nstep) * 9);
+            std::vector<double> E(size_t(nstep) * 9), A;
             double t = CurrentTime;
+            if (peierls_) A.resize(E.size());
             for (int s = 0; s < nstep; ++s) {
                 lasers_->field(t, &E[9 * size_t(s)]);
                 lasers_->field(t + dt / 2., &E[9 * size_t(s) + 3]);
                 lasers_->field(t + dt, &E[9 * size_t(s) + 6]);
+                if (peierls_) {
+                    // SourceTerm asks for A at every stage time, in this order
+                    // (the third call repeats t + dt/2 and changes nothing)
+                    lasers_->vector_potential(t, &A[9 * size_t(s)]);
+                    lasers_->vector_potential(t + dt / 2., &A[9 * size_t(s) + 3]);
+                    lasers_->vector_potential(t + dt, &A[9 * size_t(s) + 6]);
+                }
                 t += dt;
             }
-            engine_->rk4_steps(dt, nstep, E.data());
+            if (peierls_)
+                for (int s = 0; s < nstep; ++s) engine_->rk4_step_peierls(dt, &E[9 * size_t(s)], &A[9 * size_t(s)]);
+            else
+                engine_->rk4_steps(dt, nstep, E.data());
             for (int s = 0; s < nstep; ++s) CurrentTime += dt;
         } else {
             std::vector<double> E(size_t(nstep) * 3);
@@ -112,7 +124,13 @@ class DESolver {
                     lasers_->field(CurrentTime, E3);
                     lasers_->field(CurrentTime + dt / 2., E3 + 3);
                     lasers_->field(CurrentTime + dt, E3 + 6);
+                    if (peierls_)
+                        throw EngineError(EDUS_ERR_UNSUPPORTED, "RK4 start-up of AB in the Peierls gauge");
                     engine_->rk4_step(dt, E3, order_);
+                } else if (peierls_) {
+                    double A3[3];
+                    lasers_->vector_potential(CurrentTime, A3);
+                    engine_->ab_step_peierls(dt, order_, &E[3 * size_t(s)], A3);
                 } else {
                     engine_->ab_step(dt, order_, &E[3 * size_t(s)]);
                 }
@@ -124,7 +142,8 @@ class DESolver {
 
   private:
     Engine* engine_ = nullptr;
-    const SetOfLaser* lasers_ = nullptr;
+    SetOfLaser* lasers_ = nullptr;
+    bool peierls_ = false;
     SolverType type_ = RK;
     int order_ = 4;
     bool seed_ = false;  // RK4 start-up of AB (not the reference's behaviour)
@@ -136,6 +155,7 @@ class Simulation {
   public:
     explicit Simulation(const edus_sim_params& p);
     int nb, N0, N1, Nk, nsteps, filled, print_stride;
+    bool peierls = false;
     double dt;
     Material material;
     SetOfLaser setoflaser;
@@ -276,6 +296,8 @@ Simulation::Simulation(const edus_sim_params& p)
     }
     d.U = reinterpret_cast<const double*>(U.data());
     d.filledbands = filled;
+    d.peierls = p.peierls;
+    peierls = p.peierls != 0;
     EngineOptions opt;
     opt.velocity_det = jacobian();  // velocity operator built on the device (Simulation.cpp:523-576)
     engine = std::make_unique<Engine>(d, opt);
@@ -283,7 +305,7 @@ Simulation::Simulation(const edus_sim_params& p)
 
     const std::string solver = p.solver;
     desolver.initialize(engine.get(), &setoflaser, solver == "AB" ? DESolver::AB : DESolver::RK, p.order,
-                        p.ab_seed != 0);
+                        p.ab_seed != 0, p.peierls != 0);
     if (solver != "AB" && solver != "RK") throw EngineError(EDUS_ERR_ARG, "solver must be RK or AB");
     desolver.set_InitialTime(ti);
     desolver.set_ResolutionTime(dt);
@@ -390,8 +412,15 @@ int Simulation::propagate(int n, double* times, double* laser, double* popb, dou
             setoflaser.field(t, E);
             if (laser)
                 for (int d = 0; d < 3; ++d) laser[3 * printed + d] = E[d];
+            const double2* rho_obs = nullptr;
+            if (peierls) {  // the printed observables read rho(R) exp(-i A.R) (Simulation.cpp:447-449,593-595)
+                double A[3];
+                setoflaser.vector_potential(t, A);
+                engine->peierls_sandwich(engine->y, nullptr, nullptr, A, -1, engine->G);
+                rho_obs = engine->G;
+            }
             engine->observables(popb ? popb + size_t(printed) * nb : nullptr, popw ? popw + size_t(printed) * nb : nullptr,
-                                vel ? vel + 6 * size_t(printed) : nullptr);
+                                vel ? vel + 6 * size_t(printed) : nullptr, 0, rho_obs);
             ++printed;
         }
         // advance to the next printed step in one batch

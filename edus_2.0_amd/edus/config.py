@@ -62,8 +62,6 @@ def compose(user: dict, base_dir: str | None = None) -> dict:
         out["tb_file"] = os.path.join(base_dir, out["tb_file"])
     if out["gradient_space"] != "R":
         raise NotImplementedError("only gradient_space='R' (the reference default) is supported")
-    if out["peierls"]:
-        raise NotImplementedError("the Peierls-gauge branch is not implemented")
     if len(out["lasers"]) > 10:
         raise ValueError("at most 10 lasers (input_schema.json maxItems)")
     return out

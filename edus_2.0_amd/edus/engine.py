@@ -35,7 +35,7 @@ class _Desc(C.Structure):
         ("A", (C.c_double * 3) * 3),
         ("coulomb", C.c_int), ("W", C.c_void_p), ("hartree", C.c_void_p), ("rho0k", C.c_void_p),
         ("U", C.c_void_p), ("vel", C.c_void_p * 3),
-        ("filledbands", C.c_int),
+        ("filledbands", C.c_int), ("peierls", C.c_int),
     ]
 
 
@@ -78,13 +78,14 @@ class _SimParams(C.Structure):
         ("nlasers", C.c_int),
         ("lasers", _Laser * 10),
         ("precision", C.c_int),
-        ("layout", C.c_int), ("ab_seed", C.c_int),
+        ("layout", C.c_int), ("ab_seed", C.c_int), ("peierls", C.c_int),
     ]
 
 
 EXPORTS = [
     "edus_create", "edus_clone", "edus_destroy", "edus_last_error", "edus_set_state", "edus_get_state", "edus_rhs",
-    "edus_rk4_steps", "edus_ab_steps", "edus_ab_reset", "edus_ab_seed", "edus_observables", "edus_check", "edus_synchronize", "edus_graph",
+    "edus_rk4_steps", "edus_ab_steps", "edus_ab_reset", "edus_ab_seed", "edus_rhs_peierls",
+    "edus_rk4_steps_peierls", "edus_ab_steps_peierls", "edus_observables_peierls", "edus_observables", "edus_check", "edus_synchronize", "edus_graph",
     "edus_timing_enable", "edus_timing_get", "edus_get_tuning", "edus_set_tuning", "edus_batch_create",
     "edus_batch_destroy", "edus_batch_groups", "edus_batch_graph", "edus_batch_rk4_steps", "edus_batch_store",
     "edus_batch_load", "edus_batch_observables", "edus_batch_check", "edus_batch_synchronize", "edus_mlp_create",
@@ -114,6 +115,10 @@ def lib():
         L.edus_ab_steps.argtypes = [vp, C.c_double, C.c_int, C.c_int, dp]
         L.edus_ab_reset.argtypes = [vp]
         L.edus_ab_seed.argtypes = [vp, C.c_double, C.c_int, dp]
+        L.edus_rhs_peierls.argtypes = [vp, dp, dp, dp, dp]
+        L.edus_rk4_steps_peierls.argtypes = [vp, C.c_double, C.c_int, dp, dp]
+        L.edus_ab_steps_peierls.argtypes = [vp, C.c_double, C.c_int, C.c_int, dp, dp]
+        L.edus_observables_peierls.argtypes = [vp, dp, dp, dp, dp]
         L.edus_observables.argtypes = [vp, dp, dp, dp]
         L.edus_check.argtypes = [vp, dp, dp]
         L.edus_synchronize.argtypes = [vp]
@@ -181,7 +186,7 @@ class Engine:
 
     def __init__(self, nb, N0, N1, H0k, xik=(None, None, None), A=None, filledbands=0, U=None,
                  vel=(None, None, None), coulomb=False, W=None, hartree=None, rho0k=None,
-                 precision=PREC_FP64, layout=LAYOUT_FULL, _handle=None, _owner=None):
+                 precision=PREC_FP64, layout=LAYOUT_FULL, peierls=False, _handle=None, _owner=None):
         self.nb, self.N0, self.N1 = nb, N0, N1
         self.Nk = N0 * N1
         self._owner = _owner
@@ -211,6 +216,7 @@ class Engine:
         d.coulomb = int(bool(coulomb))
         d.W, d.hartree, d.rho0k, d.U = ptr(W), ptr(hartree), ptr(rho0k), ptr(U)
         d.filledbands = int(filledbands)
+        d.peierls = int(bool(peierls))
         self.h = lib().edus_create(C.byref(d))
         if not self.h:
             raise EdusError(-1, lib().edus_last_error().decode())
@@ -251,6 +257,20 @@ class Engine:
         out = np.empty_like(a)
         _check(lib().edus_rhs(self.h, _dp(e), _dp(a.view(np.float64)), _dp(out.view(np.float64))))
         return out
+
+    def rhs_peierls(self, E, Av, rho) -> np.ndarray:
+        a = _c128(rho).reshape(self._shape())
+        e = np.ascontiguousarray(E, dtype=np.float64)
+        av = np.ascontiguousarray(Av, dtype=np.float64)
+        out = np.empty_like(a)
+        _check(lib().edus_rhs_peierls(self.h, _dp(e), _dp(av), _dp(a.view(np.float64)), _dp(out.view(np.float64))))
+        return out
+
+    def observables_peierls(self, Av):
+        pb, pw, v = np.empty(self.nb), np.empty(self.nb), np.empty(6)
+        av = np.ascontiguousarray(Av, dtype=np.float64)
+        _check(lib().edus_observables_peierls(self.h, _dp(av), _dp(pb), _dp(pw), _dp(v)))
+        return pb, pw, v
 
     def rk4_steps(self, dt: float, E) -> None:
         e = np.ascontiguousarray(E, dtype=np.float64).reshape(-1, 3, 3)
@@ -451,6 +471,7 @@ def _sim_params(p: dict, precision: int, layout: int) -> _SimParams:
     s.precision = int(precision)
     s.layout = int(layout)
     s.ab_seed = int(bool(p.get("ab_seed", False)))
+    s.peierls = int(bool(p.get("peierls", False)))
     return s
 
 

@@ -83,6 +83,7 @@ typedef struct {
     const double* U;       /* eigenvectors per k, columns [Nk][nb][nb] complex; NULL = no Bloch populations */
     const double* vel[3];  /* velocity operator V_d(k) [Nk][nb][nb]; NULL = no current */
     int filledbands;
+    int peierls;           /* 1: Peierls gauge (Functional_SourceTerm.hpp:20-44); step with the *_peierls calls */
 } edus_desc;
 
 /* ---- context ----------------------------------------------------------- */
@@ -109,6 +110,18 @@ int edus_rk4_steps(edus_ctx* ctx, double dt, int nsteps, const double* E);
 /* nsteps Adams-Bashforth steps of order 4 or 5 (DESolver.hpp:202-220), history
  * starting at zero as in the reference.  E[s*3 + d] at t_s. */
 int edus_ab_steps(edus_ctx* ctx, double dt, int order, int nsteps, const double* E);
+/* Peierls gauge (contexts created with peierls = 1), the SourceTerm branch of
+ * Functional_SourceTerm.hpp:20-44: no E.grad_k term; H(R) = H0(R) + E.r(R) +
+ * H_ee[rho(R) exp(-i A.R)] multiplied by exp(+i A.R) before the commutator.
+ * A is the vector potential at the same times as E (the reference's
+ * SetOfLaser::VectorPotential, Laser.hpp:152-158); observables read
+ * rho(R) exp(-i A.R) (Simulation.cpp:447-449,593-595). */
+int edus_rhs_peierls(edus_ctx* ctx, const double E[3], const double A[3], const double* in, double* out);
+int edus_rk4_steps_peierls(edus_ctx* ctx, double dt, int nsteps, const double* E, const double* A);
+int edus_ab_steps_peierls(edus_ctx* ctx, double dt, int order, int nsteps, const double* E, const double* A);
+int edus_observables_peierls(edus_ctx* ctx, const double A[3], double* pop_bloch, double* pop_wannier,
+                             double* vel);
+
 /* Clear the Adams-Bashforth history (zero, as DESolver.cpp:26-29). */
 int edus_ab_reset(edus_ctx* ctx);
 /* Start-up of Adams-Bashforth by RK4 instead of the reference's zero history:
@@ -296,6 +309,7 @@ typedef struct {
     int precision;            /* EDUS_PREC_* */
     int layout;               /* EDUS_LAYOUT_* */
     int ab_seed;              /* 1: start Adams-Bashforth with RK4 steps (edus_ab_seed), 0: zero history */
+    int peierls;              /* the reference's "peierls" key */
 } edus_sim_params;
 
 edus_sim* edus_sim_create(const edus_sim_params* p);

@@ -590,12 +590,80 @@ void Simulation::E_of_t(double t, double E[3]) const
     }
 }
 
+void Simulation::vector_potential(double t, double out[3])
+{
+    out[0] = out[1] = out[2] = 0.;
+    for (auto& L : lasers) {
+        const double dT = t - A_prev;
+        if (!(dT < 1.e-07)) {
+            double E1[3], E2[3], E3[3];
+            L.E(A_prev, E1);
+            L.E(A_prev + dT / 2., E2);
+            L.E(t, E3);
+            for (int d = 0; d < 3; ++d) Apot[d] -= dT / 6. * (E1[d] + 4. * E2[d] + E3[d]);
+            A_prev = t;
+        }
+        for (int d = 0; d < 3; ++d) out[d] += Apot[d];
+    }
+}
+
+// Simulation::Apply_Peierls_phase (Simulation.cpp:876-899)
+void Simulation::peierls_phase(std::vector<cplx>& aR, const double Av[3], int sign) const
+{
+#pragma omp parallel for schedule(static)
+    for (int iR = 0; iR < Nk; ++iR) {
+        double Rcr[3] = {Rgc[0][iR], Rgc[1][iR], Rgc[2][iR]};
+        double Rc[3] = {0, 0, 0};
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) Rc[i] += A[i][j] * Rcr[j];
+        const cplx ph = std::exp(cplx(0, 1) * double(sign) * dot3(Av, Rc));
+        for (int h = 0; h < M; ++h) aR[size_t(iR) * M + h] *= ph;
+    }
+}
+
 // SourceTerm (Functional_SourceTerm.hpp:11-57)
 void Simulation::rhs(std::vector<cplx>& out, double t, const std::vector<cplx>& in)
 {
     double E[3];
     E_of_t(t, E);
+    if (par.peierls) {
+        double Av[3];
+        vector_potential(t, Av);
+        rhs_EA(out, E, Av, in);
+        return;
+    }
     rhs_E(out, E, in);
+}
+
+// Peierls branch of SourceTerm: no gradient term; the exchange input is
+// rho(R) exp(-i A.R); the whole H(R) is multiplied by exp(+i A.R) before
+// going to k (Functional_SourceTerm.hpp:20-44)
+void Simulation::rhs_EA(std::vector<cplx>& out, const double E[3], const double Av[3], const std::vector<cplx>& in)
+{
+    std::vector<cplx> HR(size_t(Nk) * M);
+#pragma omp parallel for schedule(static)
+    for (size_t o = 0; o < HR.size(); ++o) HR[o] = 0. + (H0R[o] + E[0] * rR[0][o] + E[1] * rR[1][o] + E[2] * rR[2][o]);
+    if (par.coulomb) {
+        std::vector<cplx> pR = in;
+        fft_planes(pR, -1);
+        peierls_phase(pR, Av, -1);
+        for (int a = 0; a < nb; ++a)
+            for (int b = 0; b < nb; ++b)
+                HR[a * nb + a] += hartree[a * nb + b] * (pR[b * nb + b] - rho0R[b * nb + b]);
+#pragma omp parallel for schedule(static)
+        for (size_t o = 0; o < HR.size(); ++o) HR[o] -= Wscr[o] * (pR[o] - rho0R[o]);
+    }
+    peierls_phase(HR, Av, +1);
+    fft_planes(HR, +1);
+    out.assign(size_t(Nk) * M, 0.);
+#pragma omp parallel for schedule(static)
+    for (int ik = 0; ik < Nk; ++ik) {
+        const cplx* h = &HR[size_t(ik) * M];
+        const cplx* r = &in[size_t(ik) * M];
+        cplx* o = &out[size_t(ik) * M];
+        zgemm(nb, cplx(0, -1), h, r, 1., o);
+        zgemm(nb, cplx(0, 1), r, h, 1., o);
+    }
 }
 
 void Simulation::rhs_E(std::vector<cplx>& out, const double E[3], const std::vector<cplx>& in)
@@ -689,11 +757,12 @@ void Simulation::populations(std::vector<double>& bloch, std::vector<double>& wa
     std::vector<cplx> sb(nb, 0.), sw(nb, 0.);
     std::vector<cplx> tmp(M), rb(M);
     for (int ik = 0; ik < Nk; ++ik) {
-        zgemm(nb, 1., &Udag[size_t(ik) * M], &rho[size_t(ik) * M], 0., tmp.data());
+        const std::vector<cplx>& r = par.peierls ? rho_obs : rho;
+        zgemm(nb, 1., &Udag[size_t(ik) * M], &r[size_t(ik) * M], 0., tmp.data());
         zgemm(nb, 1., tmp.data(), &U[size_t(ik) * M], 0., rb.data());
         for (int n = 0; n < nb; ++n) {
             sb[n] += rb[n * nb + n];
-            sw[n] += rho[size_t(ik) * M + n * nb + n];
+            sw[n] += r[size_t(ik) * M + n * nb + n];
         }
     }
     bloch.resize(nb);
@@ -712,7 +781,7 @@ void Simulation::velocity(double v[6]) const
         cplx acc = 0.;
         for (int ik = 0; ik < Nk; ++ik) {
             const cplx* V = &Vel[d][size_t(ik) * M];
-            const cplx* r = &rho[size_t(ik) * M];
+            const cplx* r = &(par.peierls ? rho_obs : rho)[size_t(ik) * M];
             for (int n = 0; n < nb; ++n)
                 for (int l = 0; l < nb; ++l) acc += V[n * nb + l] * r[l * nb + n];
         }
@@ -733,6 +802,15 @@ void Simulation::propagate(Observables* obs, int nsteps_override)
     int n = nsteps_override >= 0 ? nsteps_override : nsteps;
     for (int it = 0; it < n; ++it) {
         if (obs && print_now(t_current)) {  // Simulation.cpp:420-429
+            if (par.peierls) {
+                // the printed observables read rho phased by exp(-i A.R) (Simulation.cpp:447-449,593-595)
+                double Av[3];
+                vector_potential(t_current, Av);
+                rho_obs = rho;
+                fft_planes(rho_obs, -1);
+                peierls_phase(rho_obs, Av, -1);
+                fft_planes(rho_obs, +1);
+            }
             obs->time.push_back(t_current);
             double E[3];
             E_of_t(t_current, E);

@@ -106,6 +106,7 @@ struct SimParams {
     bool coulomb = false; double epsilon = 1.0;
     std::vector<double> r0 = {10.0}; std::string r0_units = "angstrom";
     bool coulomb_center_quirk = true;          // ModelCoulomb.cpp:76-79 behaviour
+    bool peierls = false;                      // Peierls gauge (Functional_SourceTerm.hpp:20-44)
     std::vector<LaserParams> lasers;
 };
 
@@ -142,6 +143,17 @@ class Simulation {
     double beta[5] = {0, 0, 0, 0, 0};
 
     void E_of_t(double t, double E[3]) const;
+    // SetOfLaser::VectorPotential (Laser.hpp:152-158) over Laser::VectorPotential
+    // (Laser.cpp:136-152): one integration state shared by every laser (the
+    // reference keeps it in function-level statics): A -= dT/6 (E1 + 4 E2 + E3)
+    // of the first laser from the previous call time, then every laser adds A.
+    void vector_potential(double t, double out[3]);
+    double Apot[3] = {0, 0, 0}, A_prev = 0.0;
+    // Peierls-gauge right-hand side with field E and vector potential Av
+    void rhs_EA(std::vector<cplx>& out, const double E[3], const double Av[3], const std::vector<cplx>& in);
+    // O(R) *= exp(i sign Av.R) for every plane of an R-space array
+    void peierls_phase(std::vector<cplx>& aR, const double Av[3], int sign) const;
+    std::vector<cplx> rho_obs;  // the density matrix the observables read (phased in the Peierls gauge)
     void rhs(std::vector<cplx>& out, double t, const std::vector<cplx>& in);
     // same right-hand side with the field given instead of evaluated at t
     void rhs_E(std::vector<cplx>& out, const double E[3], const std::vector<cplx>& in);
@@ -151,6 +163,7 @@ class Simulation {
     void velocity(double v[6]) const;
     void rho_R(std::vector<cplx>& out) const;       // FFT k->R of rho (1/N)
     bool print_now(double t) const;
+    void fft_planes_public(std::vector<cplx>& a, int sign) const { fft_planes(a, sign); }
 
   private:
     FFTPlan1D plan[2];

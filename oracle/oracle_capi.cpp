@@ -36,6 +36,7 @@ struct oracle_params {
     int coulomb_center_quirk;
     int nlasers;
     oracle_laser lasers[10];
+    int peierls;
 };
 
 static thread_local std::string g_err;
@@ -57,6 +58,7 @@ void* oracle_create(const oracle_params* p)
         s.r0.assign(p->r0, p->r0 + (p->nr0 > 0 ? p->nr0 : 1));
         s.r0_units = p->r0_units;
         s.coulomb_center_quirk = p->coulomb_center_quirk != 0;
+        s.peierls = p->peierls != 0;
         for (int i = 0; i < p->nlasers; ++i) {
             const oracle_laser& l = p->lasers[i];
             LaserParams lp;
@@ -145,6 +147,20 @@ void oracle_rhs_E(void* h, const double* E, const double* in, double* out)
     std::memcpy(out, vout.data(), vout.size() * sizeof(cplx));
 }
 
+void oracle_rhs_EA(void* h, const double* E, const double* Av, const double* in, double* out)
+{
+    auto* s = static_cast<Simulation*>(h);
+    std::vector<cplx> vin(size_t(s->Nk) * s->M), vout;
+    std::memcpy(vin.data(), in, vin.size() * sizeof(cplx));
+    s->rhs_EA(vout, E, Av, vin);
+    std::memcpy(out, vout.data(), vout.size() * sizeof(cplx));
+}
+
+void oracle_vector_potential(void* h, double t, double* out)
+{
+    static_cast<Simulation*>(h)->vector_potential(t, out);
+}
+
 void oracle_lattice(void* h, double* A)
 {
     auto* s = static_cast<Simulation*>(h);
@@ -163,6 +179,13 @@ void oracle_E(void* h, double t, double* E) { static_cast<Simulation*>(h)->E_of_
 void oracle_observables(void* h, double* popb, double* popw, double* vel6)
 {
     auto* s = static_cast<Simulation*>(h);
+    if (s->par.peierls) {  // as at a print step of propagate
+        double Av[3];
+        s->vector_potential(s->t_current, Av);
+        s->rho_R(s->rho_obs);
+        s->peierls_phase(s->rho_obs, Av, -1);
+        s->fft_planes_public(s->rho_obs, +1);
+    }
     std::vector<double> pb, pw;
     s->populations(pb, pw);
     for (int n = 0; n < s->nb; ++n) { popb[n] = pb[n]; popw[n] = pw[n]; }

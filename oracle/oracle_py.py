@@ -41,6 +41,7 @@ class _Params(C.Structure):
         ("coulomb_center_quirk", C.c_int),
         ("nlasers", C.c_int),
         ("lasers", _Laser * 10),
+        ("peierls", C.c_int),
     ]
 
 
@@ -65,6 +66,8 @@ def lib():
         L.oracle_eigenvalues.argtypes = [vp, dp]
         L.oracle_rhs.argtypes = [vp, C.c_double, dp, dp]
         L.oracle_rhs_E.argtypes = [vp, dp, dp, dp]
+        L.oracle_rhs_EA.argtypes = [vp, dp, dp, dp, dp]
+        L.oracle_vector_potential.argtypes = [vp, C.c_double, dp]
         L.oracle_lattice.argtypes = [vp, dp]
         L.oracle_step.argtypes = [vp, C.c_int]
         L.oracle_E.argtypes = [vp, C.c_double, dp]
@@ -105,6 +108,7 @@ def make_params(p: dict) -> _Params:
         s.r0[i] = float(v)
     s.r0_units = p["r0_units"].encode()
     s.coulomb_center_quirk = int(bool(p.get("coulomb_center_quirk", True)))
+    s.peierls = int(bool(p.get("peierls", False)))
     s.nlasers = len(p["lasers"])
     for i, las in enumerate(p["lasers"]):
         L = s.lasers[i]
@@ -178,6 +182,20 @@ class Oracle:
         e = np.ascontiguousarray(E, dtype=np.float64)
         out = np.empty_like(a)
         lib().oracle_rhs_E(self.h, _ptr(e), _ptr(a.view(np.float64)), _ptr(out.view(np.float64)))
+        return out
+
+    def rhs_EA(self, E, Av, rho: np.ndarray) -> np.ndarray:
+        """Peierls-gauge right-hand side with field E and vector potential Av."""
+        a = np.ascontiguousarray(rho, dtype=np.complex128)
+        e = np.ascontiguousarray(E, dtype=np.float64)
+        av = np.ascontiguousarray(Av, dtype=np.float64)
+        out = np.empty_like(a)
+        lib().oracle_rhs_EA(self.h, _ptr(e), _ptr(av), _ptr(a.view(np.float64)), _ptr(out.view(np.float64)))
+        return out
+
+    def vector_potential(self, t: float) -> np.ndarray:
+        out = np.empty(3)
+        lib().oracle_vector_potential(self.h, float(t), _ptr(out))
         return out
 
     def lattice(self) -> np.ndarray:

@@ -39,5 +39,6 @@ def test_config_defaults_and_unit_quirk():
     assert p["lasers"][0]["intensity_units"] == "Wcm2"
     assert p["r0"] == [10.0]
     assert config.compose({"r0": 40})["r0"] == [40.0]
+    assert config.compose({"peierls": True})["peierls"] is True
     with pytest.raises(NotImplementedError):
-        config.compose({"peierls": True})
+        config.compose({"gradient_space": "k"})
