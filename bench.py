@@ -77,6 +77,8 @@ def main():
     ap.add_argument("--epochs", type=int, default=20, help="parareal: training epochs")
     ap.add_argument("--tol", type=float, default=1e-6, help="parareal: convergence tolerance")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--tuning", default=None,
+                    help="JSON of edus_tuning fields to force after the autotuner (e.g. for counter runs)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -103,6 +105,15 @@ def main():
     eng = sim.engine
     nb, nk = sim.nb, sim.Nk
     npl = nb * nb if layout == edus.LAYOUT_FULL else nb * (nb + 1) // 2
+    if args.tuning:
+        tun = eng.get_tuning()
+        for key, val in json.loads(args.tuning).items():
+            if isinstance(val, list):
+                for i, v in enumerate(val):
+                    getattr(tun, key)[i] = int(v)
+            else:
+                setattr(tun, key, int(val))
+        eng.set_tuning(tun)
 
     # timed region: K steps, graph replay unless --graph 0
     eng.graph(args.graph)

@@ -324,12 +324,8 @@ void Batch::check(double* trace_dev, double* herm_dev)
     sync_all();
     double mt = 0.0, mh = 0.0;
     for (auto& g : groups) {
-        unsigned long long m[2];
-        HIP_CHECK(hipMemcpy(m, g.e->mon, sizeof(m), hipMemcpyDeviceToHost));
-        HIP_CHECK(hipMemsetAsync(g.e->mon, 0, sizeof(m), g.e->s));
-        HIP_CHECK(hipStreamSynchronize(g.e->s));
         double v[2];
-        std::memcpy(v, m, sizeof(v));
+        g.e->read_monitor(v);
         mt = std::max(mt, v[0]);
         mh = std::max(mh, v[1]);
     }

@@ -99,12 +99,21 @@ constexpr int plane_stride()
     return KIND == kCommScalar ? TK : TK + 1;
 }
 
+// Slots of one LDS tile array: NPL planes of PS slots, rounded up to whole
+// 64-slot chunks (one chunk = one wave-wide direct load).
+template <int NB, int LAYOUT, int KIND, int TILE>
+constexpr int tile_slots()
+{
+    constexpr int TK = tile_k<NB, LAYOUT, KIND, TILE>();
+    return (npl_of<NB, LAYOUT>() * plane_stride<KIND, TK>() + 63) / 64 * 64;
+}
+
 template <int NB, int LAYOUT, int KIND, int TILE>
 constexpr size_t comm_lds_bytes()
 {
-    constexpr int TK = tile_k<NB, LAYOUT, KIND, TILE>();
-    // the monitor's block reduction reuses the tile: 2 doubles per thread
-    return std::max(size_t(2) * npl_of<NB, LAYOUT>() * plane_stride<KIND, TK>() * sizeof(double2),
+    // rho and H tiles plus the Hartree diagonal; the monitor's block
+    // reduction reuses the tile: 2 doubles per thread
+    return std::max((size_t(2) * tile_slots<NB, LAYOUT, KIND, TILE>() + kMaxNb) * sizeof(double2),
                     size_t(2) * kCommThreads * sizeof(double));
 }
 
@@ -173,7 +182,9 @@ __device__ inline void mfma_commutator32(const double2* sH, const double2* sR, i
     }
 }
 
-template <int NB, int LAYOUT, int KIND, int TILE>
+// AB: the stage is an Adams-Bashforth step (its history is prefetched, so the
+// register budget of the other stages does not carry four history arrays).
+template <int NB, int LAYOUT, int KIND, int TILE, bool AB>
 __global__ void __launch_bounds__(kCommThreads) comm_kernel(CommArgs a)
 {
     constexpr int NT = kCommThreads;
@@ -181,9 +192,11 @@ __global__ void __launch_bounds__(kCommThreads) comm_kernel(CommArgs a)
     constexpr int TK = tile_k<NB, LAYOUT, KIND, TILE>();
     constexpr int PS = plane_stride<KIND, TK>();
     constexpr int PER = (NPL * TK + NT - 1) / NT;  // elements per thread
+    constexpr int NSLOT = tile_slots<NB, LAYOUT, KIND, TILE>();
     extern __shared__ double2 lds[];
     double2* sR = lds;
-    double2* sH = lds + NPL * PS;
+    double2* sH = lds + NSLOT;
+    double2* sHd = lds + 2 * NSLOT;
     const int Nk = a.Nk;
     // Work-groups are dealt to the 8 XCDs round-robin (block b runs on XCD
     // b % 8); give each XCD a contiguous range of tiles so that tiles sharing
@@ -204,63 +217,165 @@ __global__ void __launch_bounds__(kCommThreads) comm_kernel(CommArgs a)
     for (int dd = 0; dd < 3; ++dd) Ev[dd] = a.E_tab ? a.E_tab[inst * 3 + dd] : a.E[dd];
     const StageUpdate& u = a.up;
 
-    // Load phase.  Element (p, kk) is loaded and later produced by the same
-    // thread, so the operands of its update (D and the stage arrays) are
-    // fetched here too: every global load of the tile is in flight before the
-    // first barrier.
-    const bool x1_in = (u.x1 == a.rho), x2_in = (u.x2 == a.rho), y_in = (u.y == a.rho);
-    double2 dv[PER], x1v[PER], x2v[PER];
+    // Load phase.  Element (p, kk) of the tile is owned by one thread, which
+    // forms its H, later produces it and applies its update.
+    //  1. rho and H0 go straight from HBM into the LDS tiles
+    //     (global_load_lds_dwordx4: lane l of a wave writes 16 bytes at the
+    //     wave-uniform base + 16 l, so one wave instruction fills 64
+    //     consecutive slots; slot q holds element (q / PS, q % PS), the
+    //     padding column kk = TK is not loaded).  No registers are held.
+    //  2. The addends of H (E_d xi_d for the fields present, Sigma) come into
+    //     registers two arrays at a time, issued with step 1; after one wait
+    //     and a barrier every thread adds them to its elements of the H tile.
+    //  3. The operands of the element's update (D, the stage arrays, the
+    //     Adams-Bashforth history) are issued next and stay in flight through
+    //     the commutator.
+    // x1v holds the first stage operand (RK: x1, AB: y); ex holds the second
+    // RK operand x2 in ex[0], or the AB history f_{n-1..n-4}.
+    constexpr bool ab = AB;
+    constexpr int NEX = AB ? 4 : 1;
+    const double2* x1p = ab ? u.y : u.x1;
+    const bool ld_x1 = (ab || u.out1) && x1p != a.rho;
+    const bool ld_x2 = !ab && u.out2 && u.x2 != a.rho;
+    const double2 zero = make_double2(0., 0.);
+    {
+        const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+        for (int c = wave; c < NSLOT / 64; c += NT / 64) {
+            const int q = c * 64 + lane;
+            const int p = q / PS;
+            const int kk = q - p * PS;
+            if (kk < TK && p < NPL && k0 + kk < Nk) {
+                const long o = ioff + long(p) * Nk + (k0 + kk);
+                __builtin_amdgcn_global_load_lds(a.rho + o, sR + c * 64, 16, 0, 0);
+                __builtin_amdgcn_global_load_lds(a.H0 + o, sH + c * 64, 16, 0, 0);
+            }
+        }
+    }
+    long off[PER];
+    bool in_tile[PER], ok[PER];
+    int slot_of[PER];
 #pragma unroll
     for (int j = 0; j < PER; ++j) {
         const int idx = threadIdx.x + j * NT;
-        if (idx >= NPL * TK) break;
         const int p = idx / TK;
         const int kk = idx - p * TK;
-        const int k = k0 + kk;
-        double2 r = make_double2(0., 0.), h = r, d = r, x1 = r, x2 = r;
-        if (k < Nk) {
-            const long o = ioff + long(p) * Nk + k;
-            r = a.rho[o];
-            h = a.H0[o];
+        in_tile[j] = idx < NPL * TK;
+        ok[j] = in_tile[j] && k0 + kk < Nk;
+        off[j] = ioff + long(p) * Nk + (k0 + kk);
+        slot_of[j] = p * PS + kk;
+    }
+    // addend coefficients: E_d of the instance for xi_d, 1 for Sigma
+    double cf[4];
 #pragma unroll
-            for (int dd = 0; dd < 3; ++dd)
-                if (a.xi[dd]) {
-                    const double2 x = a.xi[dd][o];
-                    h.x += Ev[dd] * x.x;
-                    h.y += Ev[dd] * x.y;
-                }
-            if (a.sigma) h = cadd(h, a.sigma[o]);
-            if (a.hdiag) {
-                int n, m;
-                decode_plane<NB, LAYOUT>(p, n, m);
-                if (n == m) h = cadd(h, a.hdiag[inst * NB + n]);
+    for (int r = 0; r < 4; ++r) {
+        const int dd = a.add_dd[r];
+        cf[r] = dd == 0 ? Ev[0] : dd == 1 ? Ev[1] : dd == 2 ? Ev[2] : 1.0;
+    }
+    double2 dv[PER], x1v[PER], ex[NEX][PER];
+    {
+        double2 xa[PER], xb[PER];
+#pragma unroll
+        for (int j = 0; j < PER; ++j) xa[j] = xb[j] = zero;
+        if (a.n_add > 0) {
+#pragma unroll
+            for (int j = 0; j < PER; ++j)
+                if (ok[j]) xa[j] = a.add[0][off[j]];
+        }
+        if (a.n_add > 1) {
+#pragma unroll
+            for (int j = 0; j < PER; ++j)
+                if (ok[j]) xb[j] = a.add[1][off[j]];
+        }
+        if (a.hdiag && threadIdx.x < NB) sHd[threadIdx.x] = a.hdiag[inst * NB + threadIdx.x];
+        __builtin_amdgcn_s_waitcnt(0);  // this wave's direct loads are in LDS
+        __syncthreads();                // and every other wave's
+        // H = H0 + sum of the addends (in the order xi_0, xi_1, xi_2, Sigma)
+        // + Hartree diagonal, in place in the H tile
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+            if (!in_tile[j]) continue;
+            const int sl = slot_of[j];
+            double2 h = sH[sl];
+            if (a.n_add > 0) {
+                h.x += cf[0] * xa[j].x;
+                h.y += cf[0] * xa[j].y;
             }
-            if (a.D) d = a.D[o];
+            if (a.n_add > 1) {
+                h.x += cf[1] * xb[j].x;
+                h.y += cf[1] * xb[j].y;
+            }
+            sH[sl] = h;
+            // stage operands that are the stage input itself
             // (if/else rather than ?: on double2 keeps the values out of scratch)
-            if (u.out1) {
-                if (x1_in)
-                    x1 = r;
-                else
-                    x1 = u.x1[o];
+            x1v[j] = zero;
+            ex[0][j] = zero;
+            if (!ld_x1) x1v[j] = sR[sl];
+            if (!ab && !ld_x2) ex[0][j] = sR[sl];
+        }
+        if (a.n_add > 2) {  // third and fourth addend: a second round
+#pragma unroll
+            for (int j = 0; j < PER; ++j) {
+                xa[j] = xb[j] = zero;
+                if (ok[j]) xa[j] = a.add[2][off[j]];
             }
-            if (u.out2) {
-                if (x2_in)
-                    x2 = r;
-                else
-                    x2 = u.x2[o];
+            if (a.n_add > 3) {
+#pragma unroll
+                for (int j = 0; j < PER; ++j)
+                    if (ok[j]) xb[j] = a.add[3][off[j]];
             }
-            if (u.ab_order) {
-                if (y_in)
-                    x1 = r;
-                else
-                    x1 = u.y[o];
+#pragma unroll
+            for (int j = 0; j < PER; ++j) {
+                if (!in_tile[j]) continue;
+                const int sl = slot_of[j];
+                double2 h = sH[sl];
+                h.x += cf[2] * xa[j].x;
+                h.y += cf[2] * xa[j].y;
+                if (a.n_add > 3) {
+                    h.x += cf[3] * xb[j].x;
+                    h.y += cf[3] * xb[j].y;
+                }
+                sH[sl] = h;
             }
         }
-        sR[p * PS + kk] = r;
-        sH[p * PS + kk] = h;
-        dv[j] = d;
-        x1v[j] = x1;
-        x2v[j] = x2;
+        if (a.hdiag) {
+#pragma unroll
+            for (int j = 0; j < PER; ++j) {
+                if (!ok[j]) continue;
+                int n, m;
+                decode_plane<NB, LAYOUT>((threadIdx.x + j * NT) / TK, n, m);
+                if (n == m) sH[slot_of[j]] = cadd(sH[slot_of[j]], sHd[n]);
+            }
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+        dv[j] = zero;
+#pragma unroll
+        for (int e = 1; e < NEX; ++e) ex[e][j] = zero;
+    }
+    if (a.D) {
+#pragma unroll
+        for (int j = 0; j < PER; ++j)
+            if (ok[j]) dv[j] = a.D[off[j]];
+    }
+    if (ld_x1) {
+#pragma unroll
+        for (int j = 0; j < PER; ++j)
+            if (ok[j]) x1v[j] = x1p[off[j]];
+    }
+    if (ld_x2) {
+#pragma unroll
+        for (int j = 0; j < PER; ++j)
+            if (ok[j]) ex[0][j] = u.x2[off[j]];
+    }
+    if constexpr (AB) {
+#pragma unroll
+        for (int i = 1; i < 5; ++i)  // constant indices: the pointer table stays in SGPRs
+            if (i < u.ab_order) {
+#pragma unroll
+                for (int j = 0; j < PER; ++j)
+                    if (ok[j]) ex[i - 1][j] = u.hist[i][off[j]];
+            }
     }
     __syncthreads();
 
@@ -301,12 +416,10 @@ __global__ void __launch_bounds__(kCommThreads) comm_kernel(CommArgs a)
 
 #pragma unroll
     for (int j = 0; j < PER; ++j) {
+        if (!ok[j]) continue;
         const int o = threadIdx.x + j * NT;
-        if (o >= NPL * TK) break;
         const int p = o / TK;
         const int kk = o - p * TK;
-        const int k = k0 + kk;
-        if (k >= Nk) continue;
         double2 acc;
         if constexpr (KIND == kCommScalar) {
             int n, m;
@@ -327,27 +440,27 @@ __global__ void __launch_bounds__(kCommThreads) comm_kernel(CommArgs a)
         } else {
             acc = sR[p * PS + kk];
         }
-        const long og = ioff + long(p) * Nk + k;
+        const long og = off[j];
         // f = D - i * [H, rho]
         const double2 f = make_double2(dv[j].x + acc.y, dv[j].y - acc.x);
 
         if (u.out_f) u.out_f[og] = f;
-        if (u.out1) u.out1[og] = make_double2(x1v[j].x + u.b1 * f.x, x1v[j].y + u.b1 * f.y);
-        if (u.out2) u.out2[og] = make_double2(x2v[j].x + u.b2 * f.x, x2v[j].y + u.b2 * f.y);
-        if (u.ab_order) {
+        if constexpr (AB) {
             // y <- y + dt beta_0 f, then + dt beta_i f_{n-i} in the reference's order
             double2 yv = x1v[j];
             yv.x += u.ab_c[0] * f.x;
             yv.y += u.ab_c[0] * f.y;
 #pragma unroll
-            for (int i = 1; i < 5; ++i) {  // constant indices: the pointer table stays in SGPRs
+            for (int i = 1; i < 5; ++i) {
                 if (i < u.ab_order) {
-                    const double2 h = u.hist[i][og];
-                    yv.x += u.ab_c[i] * h.x;
-                    yv.y += u.ab_c[i] * h.y;
+                    yv.x += u.ab_c[i] * ex[i - 1][j].x;
+                    yv.y += u.ab_c[i] * ex[i - 1][j].y;
                 }
             }
             u.y[og] = yv;
+        } else {
+            if (u.out1) u.out1[og] = make_double2(x1v[j].x + u.b1 * f.x, x1v[j].y + u.b1 * f.y);
+            if (u.out2) u.out2[og] = make_double2(ex[0][j].x + u.b2 * f.x, ex[0][j].y + u.b2 * f.y);
         }
     }
 
@@ -404,18 +517,17 @@ __global__ void __launch_bounds__(kCommThreads) comm_kernel(CommArgs a)
             __syncthreads();
         }
         if (threadIdx.x == 0) {
-            atomicMax(&u.mon[0], static_cast<unsigned long long>(__double_as_longlong(sred[0])));
-            atomicMax(&u.mon[1], static_cast<unsigned long long>(__double_as_longlong(sred[NT])));
+            mon_slot(u.mon, blockIdx.x, sred[0], sred[NT]);
         }
     }
 }
 
-template <int NB, int LAYOUT, int KIND, int TILE>
-void launch_kind(const CommArgs& a, hipStream_t s)
+template <int NB, int LAYOUT, int KIND, int TILE, bool AB>
+void launch_one(const CommArgs& a, hipStream_t s)
 {
     constexpr int TK = tile_k<NB, LAYOUT, KIND, TILE>();
     constexpr size_t lds = comm_lds_bytes<NB, LAYOUT, KIND, TILE>();
-    auto kern = &comm_kernel<NB, LAYOUT, KIND, TILE>;
+    auto kern = &comm_kernel<NB, LAYOUT, KIND, TILE, AB>;
     static bool raised = false;
     if (!raised) {
         (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -423,6 +535,15 @@ void launch_kind(const CommArgs& a, hipStream_t s)
         raised = true;
     }
     hipLaunchKernelGGL(kern, dim3(((a.Nk + TK - 1) / TK) * std::max(a.B, 1)), dim3(kCommThreads), lds, s, a);
+}
+
+template <int NB, int LAYOUT, int KIND, int TILE>
+void launch_kind(const CommArgs& a, hipStream_t s)
+{
+    if (a.up.ab_order > 0)
+        launch_one<NB, LAYOUT, KIND, TILE, true>(a, s);
+    else
+        launch_one<NB, LAYOUT, KIND, TILE, false>(a, s);
 }
 
 template <int NB, int LAYOUT>
@@ -454,8 +575,20 @@ void launch_nb(const CommArgs& a, int layout, int kind, int tile, hipStream_t s)
 
 }  // namespace
 
-void launch_comm(int nb, int layout, int kind, int tile, const CommArgs& a, hipStream_t s)
+void launch_comm(int nb, int layout, int kind, int tile, const CommArgs& args, hipStream_t s)
 {
+    CommArgs a = args;
+    a.n_add = 0;
+    for (int r = 0; r < 4; ++r) {
+        a.add[r] = nullptr;
+        a.add_dd[r] = -1;
+    }
+    for (int d = 0; d < 3; ++d)
+        if (args.xi[d]) {
+            a.add[a.n_add] = args.xi[d];
+            a.add_dd[a.n_add++] = d;
+        }
+    if (args.sigma) a.add[a.n_add++] = args.sigma;
     switch (nb) {
         case 1: launch_nb<1>(a, layout, kind, tile, s); break;
         case 2: launch_nb<2>(a, layout, kind, tile, s); break;

@@ -56,4 +56,19 @@ __device__ inline double2 element(const double2* __restrict__ f, int r, int c, i
 // (MeshGrid.cpp:458-486 with reduce() at 345-371: j >= n/2 maps to j - n).
 __host__ __device__ inline int gamma_centred(int j, int n) { return (2 * j >= n) ? j - n : j; }
 
+// Monitor maxima (bit patterns of non-negative doubles, combined with integer
+// atomicMax) are spread over kMonSlots slots, each on its own 128-byte line, so
+// that the work-groups of one launch do not queue on one address: slot s holds
+// the trace defect at mon[kMonStride s] and the Hermiticity defect at
+// mon[kMonHalf + kMonStride s].  The reader takes the maximum over all slots.
+constexpr int kMonSlots = 256, kMonStride = 16, kMonHalf = kMonSlots * kMonStride;
+constexpr int kMonWords = 2 * kMonHalf;
+
+__device__ inline void mon_slot(unsigned long long* mon, unsigned block, double trace_dev, double herm_dev)
+{
+    const int o = int(block % kMonSlots) * kMonStride;
+    atomicMax(&mon[o], static_cast<unsigned long long>(__double_as_longlong(trace_dev)));
+    atomicMax(&mon[kMonHalf + o], static_cast<unsigned long long>(__double_as_longlong(herm_dev)));
+}
+
 }  // namespace edus

@@ -77,9 +77,9 @@ std::map<std::tuple<int, int, int, int>, FFTChoice>& tuned_shapes()
 }
 
 // commutator engine and tile per (nb, layout, k-points of all instances, precision)
-std::map<std::tuple<int, int, int, int>, std::pair<int, int>>& tuned_comm()
+std::map<std::tuple<int, int, int, int, int>, std::tuple<int, int, int>>& tuned_comm()
 {
-    static std::map<std::tuple<int, int, int, int>, std::pair<int, int>> m;
+    static std::map<std::tuple<int, int, int, int, int>, std::tuple<int, int, int>> m;
     return m;
 }
 
@@ -124,7 +124,7 @@ Engine::Engine(const edus_desc& d, const EngineOptions& opt)
     const size_t full = size_t(nb) * nb * Nk * sizeof(double2);
     for (double2** p : {&y, &acc, &aux, &G, &D, &f, &H0}) alloc(reinterpret_cast<void**>(p), pb);
     alloc(reinterpret_cast<void**>(&stage), full);
-    alloc(reinterpret_cast<void**>(&mon), 2 * sizeof(unsigned long long));
+    alloc(reinterpret_cast<void**>(&mon), kMonWords * sizeof(unsigned long long));
     const int nv = 2 * nb + 6;
     alloc(reinterpret_cast<void**>(&obs_partial), size_t(observables_blocks(Nk)) * nv * sizeof(double));
     alloc(reinterpret_cast<void**>(&obs_out), size_t(nv) * sizeof(double));
@@ -443,47 +443,94 @@ void Engine::autotune()
 
     // commutator engine and tile, whichever is fastest here: for FP64 the FMA
     // engine or the matrix cores (they differ in summation order only), for
-    // mixed precision the FP32 matrix cores; tiles give identical results
-    const auto ckey = std::make_tuple(nb, layout, Nk * B, prec);
+    // mixed precision the FP32 matrix cores; tiles give identical results.
+    // The Adams-Bashforth kernel (history prefetched, more registers) gets a
+    // tile of its own.
+    const auto ckey = std::make_tuple(nb, layout, Nk * B, prec, force_kind);
     auto cit = tuned_comm().find(ckey);
     if (cit != tuned_comm().end()) {
-        comm_kind = cit->second.first;
-        comm_tile = cit->second.second;
+        comm_kind = std::get<0>(cit->second);
+        comm_tile = std::get<1>(cit->second);
+        comm_tile_ab = std::get<2>(cit->second);
     } else {
         std::vector<std::pair<int, int>> cands;
         if (prec == EDUS_PREC_MIXED)
             cands = {{kCommMfma32, 0}, {kCommMfma32, 1}};
         else
             cands = {{kCommScalar, 0}, {kCommMfma64, 0}, {kCommMfma64, 1}};
+        if (force_kind >= 0) {  // keep the members' engine: only the tile is free
+            std::vector<std::pair<int, int>> keep;
+            for (const auto& kt : cands)
+                if (kt.first == force_kind) keep.push_back(kt);
+            cands = keep.empty() ? std::vector<std::pair<int, int>>{{force_kind, 0}} : keep;
+        }
         hipEvent_t e0, e1;
         HIP_CHECK(hipEventCreate(&e0));
         HIP_CHECK(hipEventCreate(&e1));
-        CommArgs c{};
-        c.Nk = Nk;
-        c.P = npl;
-        c.B = B;
-        c.rho = y;
-        c.H0 = H0;
-        c.up.out_f = f;
-        float best = 1e30f;
-        for (const auto& kt : cands) {
-            launch_comm(nb, layout, kt.first, kt.second, c, s);  // warm-up
+        auto time_of = [&](int kind, int tile, const CommArgs& c) {
+            launch_comm(nb, layout, kind, tile, c, s);  // warm-up
             HIP_CHECK(hipEventRecord(e0, s));
-            for (int r = 0; r < 3; ++r) launch_comm(nb, layout, kt.first, kt.second, c, s);
+            for (int r = 0; r < 3; ++r) launch_comm(nb, layout, kind, tile, c, s);
             HIP_CHECK(hipEventRecord(e1, s));
             HIP_CHECK(hipEventSynchronize(e1));
             float ms = 0.f;
             HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+            return ms;
+        };
+        // the operand sets of the hot uses, so that the candidates see the
+        // step's real HBM traffic, not a cache-resident subset (the arrays
+        // hold no state yet; every coefficient is 0).  RK4 middle stage:
+        // stage input, H0, one xi, D and two stage arrays read, two written.
+        CommArgs c{};
+        c.Nk = Nk;
+        c.P = npl;
+        c.B = B;
+        c.rho = aux;
+        c.H0 = H0;
+        for (int d = 0; d < 3; ++d)
+            if (xi[d]) {
+                c.xi[d] = xi[d];
+                break;
+            }
+        c.D = D;
+        c.up.x1 = acc;
+        c.up.out1 = acc;
+        c.up.x2 = y;
+        c.up.out2 = f;
+        // Adams-Bashforth step of order 4: state (read and written in place),
+        // H0, one xi, D and three history arrays read, the new history entry
+        // written (over one of them: each element is read and written by the
+        // same thread)
+        CommArgs cab = c;
+        cab.up = StageUpdate{};
+        cab.up.ab_order = 4;
+        cab.up.y = aux;
+        cab.up.out_f = G;
+        cab.up.hist[1] = acc;
+        cab.up.hist[2] = G;
+        cab.up.hist[3] = f;
+        float best = 1e30f;
+        for (const auto& kt : cands) {
+            const float ms = time_of(kt.first, kt.second, c);
             if (ms < best) {
                 best = ms;
                 comm_kind = kt.first;
                 comm_tile = kt.second;
             }
         }
+        best = 1e30f;
+        for (const auto& kt : cands) {
+            if (kt.first != comm_kind) continue;
+            const float ms = time_of(kt.first, kt.second, cab);
+            if (ms < best) {
+                best = ms;
+                comm_tile_ab = kt.second;
+            }
+        }
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         HIP_CHECK(hipGetLastError());
-        tuned_comm()[ckey] = {comm_kind, comm_tile};
+        tuned_comm()[ckey] = std::make_tuple(comm_kind, comm_tile, comm_tile_ab);
     }
     if (prec == EDUS_PREC_MIXED) {
         // set-up products (the velocity operator) stay FP64
@@ -494,8 +541,7 @@ void Engine::autotune()
         comm_tile64 = comm_tile;
     }
 
-    HIP_CHECK(hipMemsetAsync(G, 0, plane_bytes(), s));
-    HIP_CHECK(hipMemsetAsync(f, 0, plane_bytes(), s));
+    for (double2* p : {G, f, acc, aux}) HIP_CHECK(hipMemsetAsync(p, 0, plane_bytes(), s));
     HIP_CHECK(hipStreamSynchronize(s));
 }
 
@@ -550,7 +596,7 @@ void Engine::evaluate(const double2* in, const double E[3], const StageUpdate& u
     c.up = up;
     TimedLaunch t;
     tbegin("comm", t);
-    launch_comm(nb, layout, comm_kind, comm_tile, c, s);
+    launch_comm(nb, layout, comm_kind, tile_for(c.up), c, s);
     tend(t);
 }
 
@@ -628,7 +674,7 @@ Engine::Engine(const std::vector<const Engine*>& members)
     const size_t full = size_t(nb) * nb * Nk * sizeof(double2);
     for (double2** p : {&y, &acc, &aux, &G, &D, &f, &H0}) alloc(reinterpret_cast<void**>(p), pb);
     alloc(reinterpret_cast<void**>(&stage), full);
-    alloc(reinterpret_cast<void**>(&mon), 2 * sizeof(unsigned long long));
+    alloc(reinterpret_cast<void**>(&mon), kMonWords * sizeof(unsigned long long));
     const int nv = 2 * nb + 6;
     alloc(reinterpret_cast<void**>(&obs_partial), size_t(observables_blocks(Nk)) * nv * sizeof(double));
     alloc(reinterpret_cast<void**>(&obs_out), size_t(nv) * sizeof(double));
@@ -685,6 +731,9 @@ Engine::Engine(const std::vector<const Engine*>& members)
         }
     }
     HIP_CHECK(hipStreamSynchronize(s));
+    // the commutator engines differ in summation order: a group uses its
+    // members' engine so that every member ends exactly where it ends alone
+    force_kind = m0.comm_kind;
     autotune();
     HIP_CHECK(hipStreamSynchronize(s));
     HIP_CHECK(hipGetLastError());
@@ -721,7 +770,7 @@ void Engine::evaluate_tab(const double2* in, int ts, int j, const bool use_axis[
     c.up = up;
     TimedLaunch t;
     tbegin("comm", t);
-    launch_comm(nb, layout, comm_kind, comm_tile, c, s);
+    launch_comm(nb, layout, comm_kind, tile_for(c.up), c, s);
     tend(t);
 }
 
@@ -800,7 +849,7 @@ void Engine::evaluate_peierls(const double2* in, const double E[3], const double
     c.up = up;
     TimedLaunch t;
     tbegin("comm", t);
-    launch_comm(nb, layout, comm_kind, comm_tile, c, s);
+    launch_comm(nb, layout, comm_kind, tile_for(c.up), c, s);
     tend(t);
 }
 
@@ -871,6 +920,19 @@ void Engine::ab_step(double dt, int order, const double* E3)
     const int tmp = abidx[order - 1];
     for (int i = order - 1; i > 0; --i) abidx[i] = abidx[i - 1];
     abidx[0] = tmp;
+}
+
+// maxima over every monitor slot since the last read; the slots are cleared
+void Engine::read_monitor(double out[2])
+{
+    std::vector<unsigned long long> m(kMonWords);
+    HIP_CHECK(hipMemcpyAsync(m.data(), mon, kMonWords * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemsetAsync(mon, 0, kMonWords * sizeof(unsigned long long), s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    unsigned long long best[2] = {0, 0};
+    for (int h = 0; h < 2; ++h)
+        for (int i = 0; i < kMonSlots; ++i) best[h] = std::max(best[h], m[size_t(h) * kMonHalf + size_t(i) * kMonStride]);
+    std::memcpy(out, best, sizeof(best));
 }
 
 }  // namespace edus
@@ -1049,13 +1111,8 @@ int edus_check(edus_ctx* ctx, double* trace_dev, double* herm_dev)
 {
     EDUS_REQUIRE(ctx);
     return guarded([&] {
-        Engine& e = *ctx->e;
-        unsigned long long m[2];
-        HIP_CHECK(hipMemcpyAsync(m, e.mon, sizeof(m), hipMemcpyDeviceToHost, e.s));
-        HIP_CHECK(hipMemsetAsync(e.mon, 0, sizeof(m), e.s));
-        HIP_CHECK(hipStreamSynchronize(e.s));
         double v[2];
-        std::memcpy(v, m, sizeof(v));
+        ctx->e->read_monitor(v);
         if (trace_dev) *trace_dev = v[0];
         if (herm_dev) *herm_dev = v[1];
     });
@@ -1093,6 +1150,7 @@ int edus_get_tuning(edus_ctx* ctx, edus_tuning* out)
     }
     out->comm_kind = e.comm_kind;
     out->comm_tile = e.comm_tile;
+    out->comm_tile_ab = e.comm_tile_ab;
     return EDUS_OK;
 }
 
@@ -1126,12 +1184,14 @@ int edus_set_tuning(edus_ctx* ctx, const edus_tuning* in)
         if (e.prec == EDUS_PREC_MIXED ? in->comm_kind != edus::kCommMfma32
                                       : (in->comm_kind != edus::kCommScalar && in->comm_kind != edus::kCommMfma64))
             throw EngineError(EDUS_ERR_ARG, "comm_kind does not match the context precision");
-        if (in->comm_tile != 0 && (in->comm_tile != 1 || in->comm_kind == edus::kCommScalar))
-            throw EngineError(EDUS_ERR_ARG, "comm_tile must be 0, or 1 for the MFMA engines");
+        for (int t : {in->comm_tile, in->comm_tile_ab})
+            if (t != 0 && (t != 1 || in->comm_kind == edus::kCommScalar))
+                throw EngineError(EDUS_ERR_ARG, "comm_tile and comm_tile_ab must be 0, or 1 for the MFMA engines");
         e.fft_choice[0] = next[0];
         e.fft_choice[1] = next[1];
         e.comm_kind = in->comm_kind;
         e.comm_tile = in->comm_tile;
+        e.comm_tile_ab = in->comm_tile_ab;
         if (e.prec != EDUS_PREC_MIXED) {
             e.comm_kind64 = in->comm_kind;
             e.comm_tile64 = in->comm_tile;

@@ -76,6 +76,7 @@ class Engine {
     double2 *U = nullptr, *T = nullptr, *stage = nullptr;
     double *W = nullptr, *hart = nullptr;
     double2 *hdiag = nullptr, *diag_partial = nullptr;
+    int force_kind = -1;  // >= 0: the autotuner keeps this commutator engine (batch groups)
     double *obs_partial = nullptr, *obs_out = nullptr;
     unsigned long long* mon = nullptr;
     double2* hist[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
@@ -86,6 +87,9 @@ class Engine {
     int comm_kind = kCommScalar;     // engine of the RHS commutator (precision-dependent)
     int comm_kind64 = kCommScalar;   // FP64 engine, also used for set-up products
     int comm_tile = 0, comm_tile64 = 0;  // tile of each
+    int comm_tile_ab = 0;                 // tile of the Adams-Bashforth kernel
+    void read_monitor(double out[2]);
+    int tile_for(const StageUpdate& up) const { return up.ab_order > 0 ? comm_tile_ab : comm_tile; }
 
     bool timing = false;
     std::vector<TimedLaunch> pending;

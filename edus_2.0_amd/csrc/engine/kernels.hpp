@@ -49,7 +49,8 @@ struct StageUpdate {
     double ab_c[5];
     double2* y;               // AB target
     // invariant monitor on the array this stage wrote (mon_src): max over the
-    // tile of |Tr - nf| into mon[0], Hermiticity defect into mon[1]
+    // tile of |Tr - nf| and of the Hermiticity defect into the work-group's
+    // monitor slots (mon_slot)
     unsigned long long* mon;
     const double2* mon_src;
     double nf;
@@ -67,6 +68,12 @@ struct CommArgs {
     const double2* sigma;     // Hartree-Fock self energy in k (may be null)
     const double2* hdiag;     // Hartree diagonal shift, [B][nb] complex on device (may be null)
     StageUpdate up;
+    // filled by launch_comm from xi and sigma: the addends of H in the order
+    // xi_0, xi_1, xi_2, Sigma (present ones only); add_dd is the field
+    // component of each (-1: Sigma, coefficient 1)
+    const double2* add[4];
+    int add_dd[4];
+    int n_add;
 };
 
 // derivative coefficient of the line in plane `plane`

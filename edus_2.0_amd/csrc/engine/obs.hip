@@ -162,26 +162,47 @@ __global__ void __launch_bounds__(kObsBlock) diag_sum_kernel(const double2* __re
 
 // hdiag_n = sum_m Hartree_nm * rho_mm(R=0), rho_mm(R=0) = mean_k rho_mm(k)
 // (Coulomb.cpp:150-159; the rho0 part is folded into H0 at set-up).
-// One block per instance; partials [B][nb][nblk], hart [B][nb][nb], hdiag [B][nb].
-__global__ void hartree_kernel(const double2* __restrict__ partial, int nblk, const double* __restrict__ hart,
-                               int nb, double invNk, double2* __restrict__ hdiag)
+// One block per instance; partials [B][nb][nblk] (nblk <= 64), hart
+// [B][nb][nb], hdiag [B][nb].  All partials are fetched at once (one memory
+// round trip), then each diagonal element adds its nblk partials in block
+// order.
+constexpr int kMaxDiagBlocks = 64;
+constexpr int kHartreeLoads = (kMaxNb * kMaxDiagBlocks + kObsBlock - 1) / kObsBlock;
+
+__global__ void __launch_bounds__(kObsBlock) hartree_kernel(const double2* __restrict__ partial, int nblk,
+                                                            const double* __restrict__ hart, int nb, double invNk,
+                                                            double2* __restrict__ hdiag)
 {
+    __shared__ double2 items[kMaxNb * kMaxDiagBlocks];
     __shared__ double2 mean[kMaxNb];
     const int t = threadIdx.x;
     const int inst = blockIdx.x;
     partial += long(inst) * nb * nblk;
-    hart += long(inst) * nb * nb;
-    hdiag += long(inst) * nb;
+    const int total = nb * nblk;
+    double2 v[kHartreeLoads];
+#pragma unroll
+    for (int r = 0; r < kHartreeLoads; ++r) {
+        const int i = t + r * kObsBlock;
+        v[r] = make_double2(0., 0.);
+        if (i < total) v[r] = partial[i];
+    }
+#pragma unroll
+    for (int r = 0; r < kHartreeLoads; ++r) {
+        const int i = t + r * kObsBlock;
+        if (i < total) items[i] = v[r];
+    }
+    __syncthreads();
     if (t < nb) {
         double2 s = make_double2(0., 0.);
-        for (int b = 0; b < nblk; ++b) s = cadd(s, partial[long(t) * nblk + b]);
+        for (int b = 0; b < nblk; ++b) s = cadd(s, items[t * nblk + b]);
         mean[t] = cscale(s, invNk);
     }
     __syncthreads();
     if (t < nb) {
+        const double* hrow = hart + (long(inst) * nb + t) * nb;
         double2 h = make_double2(0., 0.);
-        for (int m = 0; m < nb; ++m) h = cadd(h, cscale(mean[m], hart[t * nb + m]));
-        hdiag[t] = h;
+        for (int m = 0; m < nb; ++m) h = cadd(h, cscale(mean[m], hrow[m]));
+        hdiag[long(inst) * nb + t] = h;
     }
 }
 
@@ -225,8 +246,7 @@ __global__ void __launch_bounds__(kObsBlock) monitor_kernel(const double2* __res
         double a = s_tr[0], b = s_h[0];
         if (a != a) a = __builtin_huge_val();  // NaN trips the monitor
         if (b != b) b = __builtin_huge_val();
-        atomicMax(&mon[0], static_cast<unsigned long long>(__double_as_longlong(a)));
-        atomicMax(&mon[1], static_cast<unsigned long long>(__double_as_longlong(b)));
+        mon_slot(mon, blockIdx.x, a, b);
     }
 }
 
@@ -282,13 +302,14 @@ void launch_observables(const ObsArgs& a, double* out, hipStream_t s)
     hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(64), 0, s, a.partial, blocks, nv, 1.0 / double(a.Nk),
                        out);
 }
-int diag_blocks(int Nk) { return int(std::min<long>(grid1d(Nk, kObsBlock), 64)); }
+int diag_blocks(int Nk) { return int(std::min<long>(grid1d(Nk, kObsBlock), kMaxDiagBlocks)); }
 void launch_hartree(const double2* rho, int nb, int Nk, int layout, int P, int B, const double* hart,
                     double2* partial, double2* hdiag, hipStream_t s)
 {
     const int nblk = diag_blocks(Nk);
     hipLaunchKernelGGL(diag_sum_kernel, dim3(nblk, nb, B), dim3(kObsBlock), 0, s, rho, nb, Nk, layout, P, partial);
-    hipLaunchKernelGGL(hartree_kernel, dim3(B), dim3(64), 0, s, partial, nblk, hart, nb, 1.0 / double(Nk), hdiag);
+    hipLaunchKernelGGL(hartree_kernel, dim3(B), dim3(kObsBlock), 0, s, partial, nblk, hart, nb, 1.0 / double(Nk),
+                       hdiag);
 }
 void launch_monitor(const double2* rho, int nb, int Nk, int layout, double nf, unsigned long long* mon, hipStream_t s)
 {

@@ -169,6 +169,7 @@ int edus_timing_get(edus_ctx* ctx, const char* kernel_class, double* total_ms, l
  *                  2: FP32 MFMA (EDUS_PREC_MIXED, not settable)
  *   comm_tile      k-points per commutator work-group: 0: 8, 1: 4 (MFMA
  *                  engines; the FMA engine takes 0)
+ *   comm_tile_ab   the same for the Adams-Bashforth steps
  * Within one family, FFT tiles give bit-identical results, and so do the
  * commutator tiles; the two FP64 commutator engines differ in summation order
  * only.
@@ -179,6 +180,7 @@ typedef struct {
     int fft_lines[2];
     int comm_kind;
     int comm_tile;
+    int comm_tile_ab;
 } edus_tuning;
 int edus_get_tuning(edus_ctx* ctx, edus_tuning* out);
 int edus_set_tuning(edus_ctx* ctx, const edus_tuning* in);

@@ -73,6 +73,7 @@ def test_fft_tiles_give_identical_results(model_dir, name, grid, layout):
         t.fft_lines[:] = [l0, l1]
         t.comm_kind = base.comm_kind
         t.comm_tile = base.comm_tile
+        t.comm_tile_ab = base.comm_tile_ab
         try:
             eng.set_tuning(t)
         except edus.EdusError:
@@ -103,6 +104,7 @@ def test_fp64_commutator_engines(model_dir, name, grid, layout):
         t = eng.get_tuning()
         t.comm_kind = kind
         t.comm_tile = tile
+        t.comm_tile_ab = tile
         eng.set_tuning(t)
         f = eng.rhs(E, rho)
         err = np.abs(f - f_ref).max() if layout == FULL else np.abs(_upper(f) - _upper(f_ref)).max()
@@ -111,6 +113,33 @@ def test_fp64_commutator_engines(model_dir, name, grid, layout):
         outs[(kind, tile)] = f
     assert np.abs(outs[(0, 0)] - outs[(1, 0)]).max() <= 1e-13 * scale
     assert np.array_equal(outs[(1, 0)], outs[(1, 1)])  # tiles: same arithmetic per k-point
+
+
+@pytest.mark.parametrize("name,grid", [("hsex8_hex90", [9, 11]), ("ipa14_hex120", [10, 9])])
+@pytest.mark.parametrize("order", [4, 5])
+def test_adams_bashforth_tiles_identical(model_dir, name, grid, order):
+    """The Adams-Bashforth kernel (history prefetched into registers) gives
+    bit-identical states with either tile (its parity with the oracle is
+    checked in test_gpu_parity)."""
+    p = _model(model_dir, name, grid)
+    p["solver"], p["order"] = "AB", order
+    o = oracle_py.Oracle(p)
+    coul = bool(p["coulomb"])
+    states = []
+    for tile in (0, 1):
+        eng = _engine(o, FULL, p["filledbands"], coulomb=coul)
+        eng.set_state(o.rho)
+        t = eng.get_tuning()
+        t.comm_kind = 1
+        t.comm_tile_ab = tile
+        eng.set_tuning(t)
+        rng = np.random.default_rng(4)
+        E = rng.uniform(-4e-3, 4e-3, size=(12, 3)) * np.array([1.0, 1.0, 0.0])
+        eng.ab_steps(0.05, order, E)
+        states.append(eng.get_state())
+        tr, herm = eng.check()
+        _say(name, grid, "AB", order, "tile", tile, "monitors", tr, herm)
+    assert np.array_equal(states[0], states[1])
 
 
 @pytest.mark.parametrize("name,grid", [("ipa2_square30", [12, 15]), ("ipa14_hex120", [10, 9]),
