@@ -43,7 +43,10 @@ def kernel_bytes(cls: str, npl: int, nk: int, nb: int, n_e: int, solver: str = "
             "monitor": 16.0 * nb * nk}.get(cls, 0.0)
 
 
-def cpu_baseline(params: dict, steps: int):
+def cpu_baseline(params: dict, steps: int, integ: str, target_s: float = 15.0):
+    """The FP64 CPU restatement (oracle/) on this host's cores, same workload
+    and grid.  steps <= 0: as many steps as fill about target_s seconds
+    (timed from one step after the warm-up step)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle_py
 
@@ -51,12 +54,35 @@ def cpu_baseline(params: dict, steps: int):
     os.environ["OMP_NUM_THREADS"] = str(threads)
     o = oracle_py.Oracle(params)
     o.step(1)  # warm-up
+    if steps <= 0:
+        t0 = time.perf_counter()
+        o.step(1)
+        one = time.perf_counter() - t0
+        steps = int(max(3, min(2000, target_s / max(one, 1e-6))))
     t0 = time.perf_counter()
     o.step(steps)
     dt = time.perf_counter() - t0
-    return {"value": o.Nk * steps / dt, "unit": "k-point·RK4-steps/s", "cores": threads, "kind": "port",
+    return {"value": o.Nk * steps / dt, "unit": f"k-point·{integ}-steps/s", "cores": threads, "kind": "port",
             "sample": f"oracle/ FP64 restatement (3 FFTs + 2 zgemm per k per RHS), same workload and grid, "
-                      f"{steps} RK4 steps after 1 warm-up step, {dt:.2f} s"}
+                      f"{steps} {integ} steps after warm-up, {dt:.2f} s"}
+
+
+def measured_traffic(cls: str, workload: str, layout: str, precision: str):
+    """HBM bytes per launch of a kernel class from the committed rocprofv3
+    counter summaries under profiles/ (FETCH_SIZE and WRITE_SIZE runs scaled
+    by the copy-kernel calibration); None when no summary covers the case."""
+    import glob
+
+    want = {"class": cls, "workload": workload, "layout": layout, "precision": precision}
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.txt"))):
+        with open(path) as f:
+            for line in f:
+                if not line.startswith("traffic:"):
+                    continue
+                kv = dict(x.split("=", 1) for x in line.split()[1:])
+                if all(kv.get(k) == v for k, v in want.items()):
+                    return float(kv["bytes_per_launch"]), os.path.relpath(path, ROOT)
+    return None, None
 
 
 def main():
@@ -70,7 +96,7 @@ def main():
                     help="mixed: FP32-MFMA commutator with FP64 state, gradient and updates")
     ap.add_argument("--graph", type=int, default=20,
                     help="RK4 steps per replayed hipGraph in the timed region (0: direct launches)")
-    ap.add_argument("--cpu-steps", type=int, default=40)
+    ap.add_argument("--cpu-steps", type=int, default=0, help="CPU baseline steps (0: about 15 s worth)")
     ap.add_argument("--coarse-steps", type=int, default=10, help="parareal: coarse RK4 steps per slice")
     ap.add_argument("--train-runs", type=int, default=64, help="parareal: random-pulse training runs")
     ap.add_argument("--k-samples", type=int, default=256, help="parareal: k-points sampled per run and slice")
@@ -152,8 +178,10 @@ def main():
     avg_s = ms / 1e3 / max(cnt, 1)
     byts = kb[dom]
     achieved = byts / avg_s / 1e9
+    traffic, traffic_src = measured_traffic(dom, args.workload, args.layout, args.precision)
     roofline = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": None, "bytes_per_launch": byts,
+                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
+                "bytes_per_launch": byts,
                 "avg_launch_us": avg_s * 1e6,
                 "timed_over": f"{args.steps} direct-launch steps after the timed region (same kernels)",
                 "direct_ms_per_step": elapsed_direct / args.steps * 1e3,
@@ -174,7 +202,7 @@ def main():
         "monitors": {"trace_dev": trace_dev, "herm_dev": herm_dev},
     }
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(params, args.cpu_steps)
+        out["cpu_baseline"] = cpu_baseline(params, args.cpu_steps, integ)
     print(json.dumps(out), flush=True)
 
 
