@@ -103,6 +103,8 @@ Engine::Engine(const edus_desc& d, const EngineOptions& opt)
     nf = d.filledbands;
     coulomb = d.coulomb != 0;
     peierls = d.peierls != 0;
+    if (d.gradient_space != 0 && d.gradient_space != 1) throw EngineError(EDUS_ERR_ARG, "gradient_space must be 0 (R) or 1 (k)");
+    kgrad = d.gradient_space == 1;
     std::memcpy(A, d.A, sizeof(A));
     {
         std::array<double, 9> a;
@@ -147,6 +149,16 @@ Engine::Engine(const edus_desc& d, const EngineOptions& opt)
         HIP_CHECK(hipStreamSynchronize(s));
     }
 
+    if (kgrad) {
+        try {
+            kst = kgradient_stencil(A, N0, N1);
+        } catch (const std::exception& e) {
+            throw EngineError(EDUS_ERR_UNSUPPORTED, e.what());
+        }
+        alloc(reinterpret_cast<void**>(&kst_dev), kst.size() * sizeof(KStencilPoint));
+        HIP_CHECK(hipMemcpyAsync(kst_dev, kst.data(), kst.size() * sizeof(KStencilPoint), hipMemcpyHostToDevice, s));
+        HIP_CHECK(hipStreamSynchronize(s));
+    }
     fac[0] = factorize(N0);
     fac[1] = factorize(N1);
     for (int ax = 0; ax < 2; ++ax) {
@@ -198,7 +210,8 @@ void Engine::build_velocity(double det)
         HIP_CHECK(hipMemsetAsync(aux, 0, pb, s));
         if (xi[d]) launch_axpy(aux, xi[d], -1.0, long(npl) * Nk, s);  // aux = -r_d
         const double cu[2] = {A[d][0] / N0, A[d][1] / N1};  // unit field along d
-        const double2* Dp = gradient(H0, cu, nullptr, cu[0] != 0.0, cu[1] != 0.0);
+        const double ud[3] = {double(d == 0), double(d == 1), double(d == 2)};
+        const double2* Dp = kgrad ? kgradient(H0, ud, nullptr) : gradient(H0, cu, nullptr, cu[0] != 0.0, cu[1] != 0.0);
         HIP_CHECK(hipMemsetAsync(f, 0, pb, s));  // zero base for the scaled store
         CommArgs c{};
         c.Nk = Nk;
@@ -566,6 +579,15 @@ const double2* Engine::gradient(const double2* in, const double c[2], const doub
     return D;
 }
 
+const double2* Engine::kgradient(const double2* in, const double E[3], const double* E_tab)
+{
+    TimedLaunch t;
+    tbegin("kgrad", t);
+    launch_kgrad(in, D, N0, N1, npl, B, kst_dev, int(kst.size()), E, E_tab, s);
+    tend(t);
+    return D;
+}
+
 // T = F^-1[ -W o (F rho)/N ]  (screened exchange in k),  hdiag = Hartree shift
 void Engine::fock(const double2* in)
 {
@@ -579,7 +601,8 @@ void Engine::evaluate(const double2* in, const double E[3], const StageUpdate& u
 {
     double cg[2];
     for (int j = 0; j < 2; ++j) cg[j] = (A[0][j] * E[0] + A[1][j] * E[1] + A[2][j] * E[2]) / (j == 0 ? N0 : N1);
-    const double2* Dp = gradient(in, cg, nullptr, cg[0] != 0.0, cg[1] != 0.0);
+    const double2* Dp = kgrad ? ((cg[0] != 0.0 || cg[1] != 0.0) ? kgradient(in, E, nullptr) : nullptr)
+                              : gradient(in, cg, nullptr, cg[0] != 0.0, cg[1] != 0.0);
     if (coulomb) fock(in);
     CommArgs c{};
     c.Nk = Nk;
@@ -662,6 +685,15 @@ Engine::Engine(const std::vector<const Engine*>& members)
             m->coulomb != coulomb || m->nf != nf || m->B != 1)
             throw EngineError(EDUS_ERR_ARG, "batch group members differ in shape");
         if (m->peierls) throw EngineError(EDUS_ERR_UNSUPPORTED, "Peierls-gauge contexts cannot be batched");
+        if (m->kgrad != m0.kgrad) throw EngineError(EDUS_ERR_ARG, "batch group members differ in gradient space");
+        if (m->kgrad) {
+            bool same = m->kst.size() == m0.kst.size();
+            for (size_t i = 0; same && i < m->kst.size(); ++i)
+                same = std::memcmp(&m->kst[i], &m0.kst[i], sizeof(KStencilPoint)) == 0;
+            if (!same)
+                throw EngineError(EDUS_ERR_UNSUPPORTED,
+                                  "k-space gradient contexts with different stencils (lattices) cannot share a group");
+        }
         std::array<double, 9> a;
         std::memcpy(a.data(), m->A, sizeof(m->A));
         Ainst.push_back(a);
@@ -710,6 +742,12 @@ Engine::Engine(const std::vector<const Engine*>& members)
         alloc(reinterpret_cast<void**>(&U), size_t(B) * full);
         stack(U, [](const Engine* m) { return m->U; }, full);
     }
+    kgrad = m0.kgrad;
+    if (kgrad) {
+        kst = m0.kst;
+        alloc(reinterpret_cast<void**>(&kst_dev), kst.size() * sizeof(KStencilPoint));
+        HIP_CHECK(hipMemcpyAsync(kst_dev, kst.data(), kst.size() * sizeof(KStencilPoint), hipMemcpyHostToDevice, s));
+    }
     fac[0] = m0.fac[0];
     fac[1] = m0.fac[1];
     for (int ax = 0; ax < 2; ++ax) {
@@ -754,7 +792,8 @@ void Engine::evaluate_tab(const double2* in, int ts, int j, const bool use_axis[
 {
     const size_t row = size_t(ts) * 3 + j;
     const double c0[2] = {0.0, 0.0};
-    const double2* Dp = gradient(in, c0, coef_dev + row * B * 2, use_axis[0], use_axis[1]);
+    const double2* Dp = kgrad ? ((use_axis[0] || use_axis[1]) ? kgradient(in, nullptr, E_dev + row * B * 3) : nullptr)
+                              : gradient(in, c0, coef_dev + row * B * 2, use_axis[0], use_axis[1]);
     if (coulomb) fock(in);
     CommArgs c{};
     c.Nk = Nk;

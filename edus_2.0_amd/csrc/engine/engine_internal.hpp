@@ -59,6 +59,12 @@ class Engine {
     int B = 1;  // stacked instances
     bool coulomb;
     bool peierls = false;
+    // gradient_space "k": finite-difference stencil (host copy and device table)
+    bool kgrad = false;
+    std::vector<KStencilPoint> kst;
+    KStencilPoint* kst_dev = nullptr;
+    // E.grad_k in by the stencil into D (E or, for tables, E_tab on the device)
+    const double2* kgradient(const double2* in, const double E[3], const double* E_tab);
     double2 *Hp = nullptr, *Tq = nullptr;  // Peierls: phased H(k) and its axis-0 half
     double A[3][3];
     std::vector<std::array<double, 9>> Ainst;  // lattice of each stacked instance

@@ -2,6 +2,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 #include "common.hpp"
 
 namespace edus {
@@ -113,5 +115,19 @@ void launch_fft(const AxisFFTArgs& a, const FFTChoice& c, hipStream_t s);
 enum CommKind { kCommScalar = 0, kCommMfma64 = 1, kCommMfma32 = 2 };
 // tile 0: 8 k-points per work-group, 1: 4 (MFMA engines; the FMA engine has one)
 void launch_comm(int nb, int layout, int kind, int tile, const CommArgs& a, hipStream_t s);
+
+// k-space finite-difference gradient (gradient_space "k"): one stencil point
+// per mesh vector b of the shells in use: grid offset, shell weight, Cartesian b
+struct KStencilPoint {
+    int m0, m1;
+    double w;
+    double b[3];
+    int shell;
+};
+std::vector<KStencilPoint> kgradient_stencil(const double A[3][3], int N0, int N1);
+// out = sum_b w_b (b.E) in(k + b) for every plane of B stacked instances;
+// E_tab ([B][3] on the device) replaces E when set
+void launch_kgrad(const double2* in, double2* out, int N0, int N1, int P, int B, const KStencilPoint* st, int nst,
+                  const double E[3], const double* E_tab, hipStream_t s);
 
 }  // namespace edus

@@ -297,6 +297,7 @@ Simulation::Simulation(const edus_sim_params& p)
     d.U = reinterpret_cast<const double*>(U.data());
     d.filledbands = filled;
     d.peierls = p.peierls;
+    d.gradient_space = p.gradient_space;
     peierls = p.peierls != 0;
     EngineOptions opt;
     opt.velocity_det = jacobian();  // velocity operator built on the device (Simulation.cpp:523-576)

@@ -60,8 +60,8 @@ def compose(user: dict, base_dir: str | None = None) -> dict:
         out["r0"] = [float(out["r0"])]
     if base_dir and out["tb_file"] and not os.path.isabs(out["tb_file"]):
         out["tb_file"] = os.path.join(base_dir, out["tb_file"])
-    if out["gradient_space"] != "R":
-        raise NotImplementedError("only gradient_space='R' (the reference default) is supported")
+    if out["gradient_space"] not in ("R", "k"):
+        raise ValueError("gradient_space must be 'R' or 'k'")
     if len(out["lasers"]) > 10:
         raise ValueError("at most 10 lasers (input_schema.json maxItems)")
     return out

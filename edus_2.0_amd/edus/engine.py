@@ -35,7 +35,7 @@ class _Desc(C.Structure):
         ("A", (C.c_double * 3) * 3),
         ("coulomb", C.c_int), ("W", C.c_void_p), ("hartree", C.c_void_p), ("rho0k", C.c_void_p),
         ("U", C.c_void_p), ("vel", C.c_void_p * 3),
-        ("filledbands", C.c_int), ("peierls", C.c_int),
+        ("filledbands", C.c_int), ("peierls", C.c_int), ("gradient_space", C.c_int),
     ]
 
 
@@ -78,7 +78,7 @@ class _SimParams(C.Structure):
         ("nlasers", C.c_int),
         ("lasers", _Laser * 10),
         ("precision", C.c_int),
-        ("layout", C.c_int), ("ab_seed", C.c_int), ("peierls", C.c_int),
+        ("layout", C.c_int), ("ab_seed", C.c_int), ("peierls", C.c_int), ("gradient_space", C.c_int),
     ]
 
 
@@ -86,7 +86,8 @@ EXPORTS = [
     "edus_create", "edus_clone", "edus_destroy", "edus_last_error", "edus_set_state", "edus_get_state", "edus_rhs",
     "edus_rk4_steps", "edus_ab_steps", "edus_ab_reset", "edus_ab_seed", "edus_rhs_peierls",
     "edus_rk4_steps_peierls", "edus_ab_steps_peierls", "edus_observables_peierls", "edus_observables", "edus_check", "edus_synchronize", "edus_graph",
-    "edus_timing_enable", "edus_timing_get", "edus_get_tuning", "edus_set_tuning", "edus_batch_create",
+    "edus_timing_enable", "edus_timing_get", "edus_get_tuning", "edus_set_tuning", "edus_kgrad_stencil",
+    "edus_batch_create",
     "edus_batch_destroy", "edus_batch_groups", "edus_batch_graph", "edus_batch_rk4_steps", "edus_batch_store",
     "edus_batch_load", "edus_batch_observables", "edus_batch_check", "edus_batch_synchronize", "edus_mlp_create",
     "edus_mlp_destroy", "edus_mlp_num_params", "edus_mlp_set_params", "edus_mlp_get_params", "edus_mlp_forward",
@@ -148,6 +149,8 @@ def lib():
         L.edus_batch_check.argtypes = [vp, dp, dp]
         L.edus_batch_synchronize.argtypes = [vp]
         L.edus_set_tuning.argtypes = [vp, C.POINTER(Tuning)]
+        L.edus_kgrad_stencil.argtypes = [C.POINTER(C.c_double * 9), C.c_int, C.c_int, dp, C.c_int]
+        L.edus_kgrad_stencil.restype = C.c_int
         L.edus_sim_create.restype = vp
         L.edus_sim_create.argtypes = [C.POINTER(_SimParams)]
         L.edus_sim_destroy.argtypes = [vp]
@@ -186,7 +189,8 @@ class Engine:
 
     def __init__(self, nb, N0, N1, H0k, xik=(None, None, None), A=None, filledbands=0, U=None,
                  vel=(None, None, None), coulomb=False, W=None, hartree=None, rho0k=None,
-                 precision=PREC_FP64, layout=LAYOUT_FULL, peierls=False, _handle=None, _owner=None):
+                 precision=PREC_FP64, layout=LAYOUT_FULL, peierls=False, gradient_space="R", _handle=None,
+                 _owner=None):
         self.nb, self.N0, self.N1 = nb, N0, N1
         self.Nk = N0 * N1
         self._owner = _owner
@@ -217,6 +221,9 @@ class Engine:
         d.W, d.hartree, d.rho0k, d.U = ptr(W), ptr(hartree), ptr(rho0k), ptr(U)
         d.filledbands = int(filledbands)
         d.peierls = int(bool(peierls))
+        if gradient_space not in ("R", "k"):
+            raise ValueError("gradient_space must be 'R' or 'k'")
+        d.gradient_space = 1 if gradient_space == "k" else 0
         self.h = lib().edus_create(C.byref(d))
         if not self.h:
             raise EdusError(-1, lib().edus_last_error().decode())
@@ -320,6 +327,19 @@ class Engine:
 
     def set_tuning(self, t: Tuning) -> None:
         _check(lib().edus_set_tuning(self.h, C.byref(t)))
+
+
+def kgrad_stencil(A, N0: int, N1: int) -> np.ndarray:
+    """edus_kgrad_stencil: the finite-difference stencil of gradient_space "k"
+    for lattice A (columns: lattice vectors, bohr) and an N0 x N1 mesh, rows
+    (shell, m0, m1, bx, by, bz, w).  Host only."""
+    a = (C.c_double * 9)(*np.asarray(A, dtype=float).ravel())
+    n = lib().edus_kgrad_stencil(C.byref(a), int(N0), int(N1), None, 0)
+    if n < 0:
+        raise EdusError(n, "no finite-difference stencil for this lattice and mesh")
+    out = np.empty((n, 7))
+    lib().edus_kgrad_stencil(C.byref(a), int(N0), int(N1), _dp(out), n)
+    return out
 
 
 def _fp(a):
@@ -472,6 +492,7 @@ def _sim_params(p: dict, precision: int, layout: int) -> _SimParams:
     s.layout = int(layout)
     s.ab_seed = int(bool(p.get("ab_seed", False)))
     s.peierls = int(bool(p.get("peierls", False)))
+    s.gradient_space = 1 if p.get("gradient_space", "R") == "k" else 0
     return s
 
 

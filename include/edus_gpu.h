@@ -84,6 +84,8 @@ typedef struct {
     const double* vel[3];  /* velocity operator V_d(k) [Nk][nb][nb]; NULL = no current */
     int filledbands;
     int peierls;           /* 1: Peierls gauge (Functional_SourceTerm.hpp:20-44); step with the *_peierls calls */
+    int gradient_space;    /* 0: "R" spectral derivative (default); 1: "k" finite differences over the
+                              k-mesh shells (kGradient.cpp:25-210, kGradient_definitions.hpp:19-31) */
 } edus_desc;
 
 /* ---- context ----------------------------------------------------------- */
@@ -183,6 +185,14 @@ typedef struct {
     int comm_tile_ab;
 } edus_tuning;
 int edus_get_tuning(edus_ctx* ctx, edus_tuning* out);
+
+/* The finite-difference stencil that gradient_space "k" uses for a lattice A
+ * (A[i][j]: component i of lattice vector j, bohr) and an N0 x N1 k mesh: rows
+ * (shell, m0, m1, b_x, b_y, b_z, w_shell), one per mesh vector b of the shells
+ * in use, in the reference's order (kGradient.cpp:44-210).  Host only (no GPU
+ * needed).  Returns the number of rows, or a negative EDUS_ERR_* code; rows
+ * are written only if cap is large enough. */
+int edus_kgrad_stencil(const double A[3][3], int N0, int N1, double* rows, int cap);
 int edus_set_tuning(edus_ctx* ctx, const edus_tuning* in);
 
 /* ---- batches ------------------------------------------------------------
@@ -312,6 +322,7 @@ typedef struct {
     int layout;               /* EDUS_LAYOUT_* */
     int ab_seed;              /* 1: start Adams-Bashforth with RK4 steps (edus_ab_seed), 0: zero history */
     int peierls;              /* the reference's "peierls" key */
+    int gradient_space;       /* the reference's "gradient_space" key: 0 "R", 1 "k" */
 } edus_sim_params;
 
 edus_sim* edus_sim_create(const edus_sim_params* p);

@@ -351,6 +351,9 @@ Simulation::Simulation(const SimParams& p) : par(p)
             for (int d = 0; d < 3; ++d) Rgc[d][ik] = reduce_coord(c[d], -N[d] / 2., N[d] / 2., N[d]);
         }
 
+    if (p.gradient_space == "k") setup_kgradient();
+    else if (p.gradient_space != "R") throw std::runtime_error("gradient_space must be R or k");
+
     // explicit DFT H(R)->H(k), sign +1, over the Gamma-centred tb R list (fftPair.cpp:110-143)
     std::vector<std::array<double, 3>> Rtb(tb.nR);
     for (int iR = 0; iR < tb.nR; ++iR)
@@ -472,6 +475,12 @@ Simulation::Simulation(const SimParams& p) : par(p)
             zgemm(nb, cplx(0, -1), r, h, 0., v);
             zgemm(nb, cplx(0, 1), h, r, 1., v);
         }
+        if (kgrad) {  // V += grad_k H0 along d, in k (Simulation.cpp:554-556)
+            const double u[3] = {double(d == 0), double(d == 1), double(d == 2)};
+            kgradient(Vel[d], H0k, u);
+            for (auto& v : Vel[d]) v /= det;
+            continue;
+        }
         fft_planes(Vel[d], -1);
         for (int iR = 0; iR < Nk; ++iR) {
             double Rc[3] = {0, 0, 0};
@@ -559,6 +568,139 @@ void Simulation::setup_coulomb(const TBModel& tb)
             hartree[a * nb + b] = v;
             hartree[b * nb + a] = v;
         }
+}
+
+// ------------------------------------------------------------------ k-space gradient
+// Pseudo-inverse of a real m x n matrix (row-major) as Matrix<double>::pseudoinv
+// (Geometry/Matrix.cpp:130-161): V S^+ U^T, where singular values <= 1e-8 are
+// multiplied by s instead of 1/s.  Here through the eigen-decomposition of
+// A^T A (eigenvalues s^2, eigenvectors V) and U = A V / s.
+static std::vector<double> pseudoinverse(const std::vector<double>& A, int m, int n)
+{
+    std::vector<cplx> AtA(size_t(n) * n, 0.);
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) {
+            double acc = 0.;
+            for (int r = 0; r < m; ++r) acc += A[r * n + i] * A[r * n + j];
+            AtA[i * n + j] = acc;
+        }
+    std::vector<double> w(n);
+    std::vector<cplx> V(size_t(n) * n);
+    eigh(n, AtA.data(), w.data(), V.data());
+    std::vector<double> P(size_t(n) * m, 0.);
+    for (int c = 0; c < n; ++c) {
+        const double sv = std::sqrt(std::max(w[c], 0.));
+        if (sv <= 1.e-8) continue;  // U undefined; its term is O(s) <= 1e-8
+        // u_c = A v_c / s, P += v_c (1/s) u_c^T
+        std::vector<double> u(m, 0.);
+        for (int r = 0; r < m; ++r)
+            for (int i = 0; i < n; ++i) u[r] += A[r * n + i] * V[i * n + c].real();
+        for (int r = 0; r < m; ++r) u[r] /= sv;
+        for (int i = 0; i < n; ++i)
+            for (int r = 0; r < m; ++r) P[i * m + r] += V[i * n + c].real() / sv * u[r];
+    }
+    return P;
+}
+
+// kGradient::initialize for a k mesh (kGradient.cpp:25-41): shells of the
+// Gamma-centred mesh by Cartesian norm (SortInShells, :44-85), the smallest
+// number of shells whose weights w = A^+ q satisfy A w = q (B1 condition,
+// Calculate_nshellsAndweights :88-125, GradientMatrix :161-180), and the
+// k+b neighbours (Find_kpb :182-210) as grid offsets.
+void Simulation::setup_kgradient()
+{
+    kgrad = true;
+    // reciprocal basis B = 2 pi (A^-1)^T, columns b_j (Model.hpp:46-47)
+    const double det = A[0][0] * (A[1][1] * A[2][2] - A[1][2] * A[2][1]) - A[0][1] * (A[1][0] * A[2][2] - A[1][2] * A[2][0]) +
+                       A[0][2] * (A[1][0] * A[2][1] - A[1][1] * A[2][0]);
+    double inv[3][3];
+    inv[0][0] = (A[1][1] * A[2][2] - A[1][2] * A[2][1]) / det;
+    inv[0][1] = (A[0][2] * A[2][1] - A[0][1] * A[2][2]) / det;
+    inv[0][2] = (A[0][1] * A[1][2] - A[0][2] * A[1][1]) / det;
+    inv[1][0] = (A[1][2] * A[2][0] - A[1][0] * A[2][2]) / det;
+    inv[1][1] = (A[0][0] * A[2][2] - A[0][2] * A[2][0]) / det;
+    inv[1][2] = (A[0][2] * A[1][0] - A[0][0] * A[1][2]) / det;
+    inv[2][0] = (A[1][0] * A[2][1] - A[1][1] * A[2][0]) / det;
+    inv[2][1] = (A[0][1] * A[2][0] - A[0][0] * A[2][1]) / det;
+    inv[2][2] = (A[0][0] * A[1][1] - A[0][1] * A[1][0]) / det;
+    double Bm[3][3];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) Bm[i][j] = 2. * pi * inv[j][i];
+    // Gamma-centred k mesh, crystal coordinates in [-1/2, 1/2) (MeshGrid.cpp:458-486)
+    std::vector<std::array<double, 3>> kc(Nk), kcart(Nk);
+    std::vector<double> knorm(Nk);
+    for (int i0 = 0; i0 < N[0]; ++i0)
+        for (int i1 = 0; i1 < N[1]; ++i1) {
+            const int ik = i1 + N[1] * i0;
+            const double c[3] = {double(i0) / N[0], double(i1) / N[1], 0.};
+            for (int d = 0; d < 3; ++d) kc[ik][d] = reduce_coord(c[d], -0.5, 0.5, 1.0);
+            for (int i = 0; i < 3; ++i) kcart[ik][i] = Bm[i][0] * kc[ik][0] + Bm[i][1] * kc[ik][1] + Bm[i][2] * kc[ik][2];
+            knorm[ik] = std::sqrt(dot3(kcart[ik].data(), kcart[ik].data()));
+        }
+    std::vector<double> norms;
+    for (int ik = 0; ik < Nk; ++ik) {
+        bool found = false;
+        for (double v : norms) found = found || std::abs(v - knorm[ik]) < threshold;
+        if (!found) norms.push_back(knorm[ik]);
+    }
+    std::sort(norms.begin(), norms.end());
+    std::vector<std::vector<int>> shells(norms.size());
+    for (int ik = 0; ik < Nk; ++ik) {
+        size_t s = 0;
+        while (s < norms.size() && !(std::abs(norms[s] - knorm[ik]) < 1.e-07)) ++s;
+        shells[s].push_back(ik);
+    }
+    if (!(std::abs(norms[0]) < threshold)) throw std::runtime_error("kgradient: no k = 0 shell");
+    shells.erase(shells.begin());
+    // weights (alpha, beta) pairs xx yy zz xy xz yz
+    const int al[6] = {0, 1, 2, 0, 0, 1}, be[6] = {0, 1, 2, 1, 2, 2};
+    const double q[6] = {double(N[0] > 1), double(N[1] > 1), double(N[2] > 1), 0., 0., 0.};
+    std::vector<double> w;
+    for (nshells = 1;; ++nshells) {
+        if (nshells > int(shells.size())) throw std::runtime_error("kgradient: not enough shells for the B1 condition");
+        std::vector<double> Am(size_t(6) * nshells, 0.);
+        for (int j = 0; j < 6; ++j)
+            for (int s = 0; s < nshells; ++s)
+                for (int ik : shells[s]) Am[j * nshells + s] += kcart[ik][al[j]] * kcart[ik][be[j]];
+        const std::vector<double> P = pseudoinverse(Am, 6, nshells);
+        w.assign(nshells, 0.);
+        for (int s = 0; s < nshells; ++s)
+            for (int j = 0; j < 6; ++j) w[s] += P[s * 6 + j] * q[j];
+        double res = 0.;
+        for (int j = 0; j < 6; ++j) {
+            double qg = 0.;
+            for (int s = 0; s < nshells; ++s) qg += Am[j * nshells + s] * w[s];
+            res += (qg - q[j]) * (qg - q[j]);
+        }
+        if (std::sqrt(res) < threshold) break;
+    }
+    shell_weight = w;
+    kstencil.clear();
+    for (int s = 0; s < nshells; ++s)
+        for (int ik : shells[s]) {
+            KStencil st;
+            st.shell = s;
+            st.m0 = int(std::lround(kc[ik][0] * N[0]));
+            st.m1 = int(std::lround(kc[ik][1] * N[1]));
+            for (int d = 0; d < 3; ++d) st.b[d] = kcart[ik][d];
+            kstencil.push_back(st);
+        }
+}
+
+void Simulation::kgradient(std::vector<cplx>& out, const std::vector<cplx>& F, const double u[3]) const
+{
+#pragma omp parallel for schedule(static)
+    for (int ik = 0; ik < Nk; ++ik) {
+        const int i0 = ik / N[1], i1 = ik % N[1];
+        for (const KStencil& st : kstencil) {
+            const int j0 = ((i0 + st.m0) % N[0] + N[0]) % N[0];
+            const int j1 = ((i1 + st.m1) % N[1] + N[1]) % N[1];
+            const int jk = j1 + N[1] * j0;
+            const double bdotu = st.b[0] * u[0] + st.b[1] * u[1] + st.b[2] * u[2];
+            const cplx c = cplx(1., 0.) * shell_weight[st.shell] * bdotu;
+            for (int h = 0; h < M; ++h) out[size_t(ik) * M + h] += c * F[size_t(jk) * M + h];
+        }
+    }
 }
 
 void Simulation::fft_planes(std::vector<cplx>& a, int sign) const
@@ -672,6 +814,30 @@ void Simulation::rhs_E(std::vector<cplx>& out, const double E[3], const std::vec
     fft_planes(inR, -1);
     out.assign(size_t(Nk) * M, 0.);
     std::vector<cplx> HR(size_t(Nk) * M);
+    if (kgrad) {
+        // gradient in k: Output(k) += (E.grad) rho(k) by finite differences;
+        // H(R) as below; the commutator adds to the k-space output
+        kgradient(out, in, E);
+#pragma omp parallel for schedule(static)
+        for (size_t o = 0; o < HR.size(); ++o) HR[o] = 0. + (H0R[o] + E[0] * rR[0][o] + E[1] * rR[1][o] + E[2] * rR[2][o]);
+        if (par.coulomb) {
+            for (int a = 0; a < nb; ++a)
+                for (int b = 0; b < nb; ++b)
+                    HR[a * nb + a] += hartree[a * nb + b] * (inR[b * nb + b] - rho0R[b * nb + b]);
+#pragma omp parallel for schedule(static)
+            for (size_t o = 0; o < HR.size(); ++o) HR[o] -= Wscr[o] * (inR[o] - rho0R[o]);
+        }
+        fft_planes(HR, +1);
+#pragma omp parallel for schedule(static)
+        for (int ik = 0; ik < Nk; ++ik) {
+            const cplx* h = &HR[size_t(ik) * M];
+            const cplx* r = &in[size_t(ik) * M];
+            cplx* o = &out[size_t(ik) * M];
+            zgemm(nb, cplx(0, -1), h, r, 1., o);
+            zgemm(nb, cplx(0, 1), r, h, 1., o);
+        }
+        return;
+    }
 #pragma omp parallel for schedule(static)
     for (int iR = 0; iR < Nk; ++iR) {
         // gradient: Output(R) += i (R.E) rho(R)   (kGradient_definitions.hpp:35-41)

@@ -107,6 +107,7 @@ struct SimParams {
     std::vector<double> r0 = {10.0}; std::string r0_units = "angstrom";
     bool coulomb_center_quirk = true;          // ModelCoulomb.cpp:76-79 behaviour
     bool peierls = false;                      // Peierls gauge (Functional_SourceTerm.hpp:20-44)
+    std::string gradient_space = "R";          // "k": finite-difference gradient (kGradient.cpp:25-210)
     std::vector<LaserParams> lasers;
 };
 
@@ -154,6 +155,16 @@ class Simulation {
     // O(R) *= exp(i sign Av.R) for every plane of an R-space array
     void peierls_phase(std::vector<cplx>& aR, const double Av[3], int sign) const;
     std::vector<cplx> rho_obs;  // the density matrix the observables read (phased in the Peierls gauge)
+    // k-space finite-difference gradient (gradient_space "k"): the shells of
+    // the Gamma-centred k mesh and their weights (kGradient.cpp:44-125), and
+    // for every b of the first nshells shells its grid offset and Cartesian b
+    bool kgrad = false;
+    int nshells = 0;
+    std::vector<double> shell_weight;
+    struct KStencil { int shell, m0, m1; double b[3]; };
+    std::vector<KStencil> kstencil;
+    // Derivative(k) += sum_s w_s sum_{b in s} (b.u) F(k+b)  (kGradient_definitions.hpp:19-31)
+    void kgradient(std::vector<cplx>& out, const std::vector<cplx>& F, const double u[3]) const;
     void rhs(std::vector<cplx>& out, double t, const std::vector<cplx>& in);
     // same right-hand side with the field given instead of evaluated at t
     void rhs_E(std::vector<cplx>& out, const double E[3], const std::vector<cplx>& in);
@@ -169,6 +180,7 @@ class Simulation {
     FFTPlan1D plan[2];
     void fft_planes(std::vector<cplx>& a, int sign) const;   // all nb^2 planes of [k][n][m]
     void setup_coulomb(const TBModel& tb);
+    void setup_kgradient();
 };
 
 double struve0(double x);                          // ModelCoulomb.cpp:7-44

@@ -37,6 +37,7 @@ struct oracle_params {
     int nlasers;
     oracle_laser lasers[10];
     int peierls;
+    int gradient_k;  // 1: gradient_space "k"
 };
 
 static thread_local std::string g_err;
@@ -59,6 +60,7 @@ void* oracle_create(const oracle_params* p)
         s.r0_units = p->r0_units;
         s.coulomb_center_quirk = p->coulomb_center_quirk != 0;
         s.peierls = p->peierls != 0;
+        s.gradient_space = p->gradient_k ? "k" : "R";
         for (int i = 0; i < p->nlasers; ++i) {
             const oracle_laser& l = p->lasers[i];
             LaserParams lp;
@@ -234,5 +236,34 @@ void oracle_fft2d(int n0, int n1, double* data, int sign)
 }
 
 double oracle_struve0(double x) { return struve0(x); }
+
+// k-space gradient stencil: n rows of (shell, m0, m1, b_x, b_y, b_z, w_shell);
+// returns the number of rows (0 when gradient_space is "R").  rows may be null.
+int oracle_kstencil(void* h, double* rows)
+{
+    auto* sim = static_cast<Simulation*>(h);
+    const int n = int(sim->kstencil.size());
+    if (rows)
+        for (int i = 0; i < n; ++i) {
+            const auto& st = sim->kstencil[i];
+            double* r = rows + 7 * i;
+            r[0] = st.shell; r[1] = st.m0; r[2] = st.m1;
+            r[3] = st.b[0]; r[4] = st.b[1]; r[5] = st.b[2];
+            r[6] = sim->shell_weight[st.shell];
+        }
+    return n;
+}
+
+// out = finite-difference (u.grad_k) of in ([k][n][m]), gradient_space "k" only
+int oracle_kgradient(void* h, const double* u, const double* in, double* out)
+{
+    auto* sim = static_cast<Simulation*>(h);
+    if (!sim->kgrad) return -1;
+    const size_t n = size_t(sim->Nk) * sim->M;
+    std::vector<cplx> a(reinterpret_cast<const cplx*>(in), reinterpret_cast<const cplx*>(in) + n), o(n, 0.);
+    sim->kgradient(o, a, u);
+    std::copy(o.begin(), o.end(), reinterpret_cast<cplx*>(out));
+    return 0;
+}
 
 }  // extern "C"

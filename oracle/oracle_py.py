@@ -42,6 +42,7 @@ class _Params(C.Structure):
         ("nlasers", C.c_int),
         ("lasers", _Laser * 10),
         ("peierls", C.c_int),
+        ("gradient_k", C.c_int),
     ]
 
 
@@ -67,6 +68,10 @@ def lib():
         L.oracle_rhs.argtypes = [vp, C.c_double, dp, dp]
         L.oracle_rhs_E.argtypes = [vp, dp, dp, dp]
         L.oracle_rhs_EA.argtypes = [vp, dp, dp, dp, dp]
+        L.oracle_kstencil.argtypes = [vp, dp]
+        L.oracle_kstencil.restype = C.c_int
+        L.oracle_kgradient.argtypes = [vp, dp, dp, dp]
+        L.oracle_kgradient.restype = C.c_int
         L.oracle_vector_potential.argtypes = [vp, C.c_double, dp]
         L.oracle_lattice.argtypes = [vp, dp]
         L.oracle_step.argtypes = [vp, C.c_int]
@@ -109,6 +114,7 @@ def make_params(p: dict) -> _Params:
     s.r0_units = p["r0_units"].encode()
     s.coulomb_center_quirk = int(bool(p.get("coulomb_center_quirk", True)))
     s.peierls = int(bool(p.get("peierls", False)))
+    s.gradient_k = int(p.get("gradient_space", "R") == "k")
     s.nlasers = len(p["lasers"])
     for i, las in enumerate(p["lasers"]):
         L = s.lasers[i]
@@ -191,6 +197,23 @@ class Oracle:
         av = np.ascontiguousarray(Av, dtype=np.float64)
         out = np.empty_like(a)
         lib().oracle_rhs_EA(self.h, _ptr(e), _ptr(av), _ptr(a.view(np.float64)), _ptr(out.view(np.float64)))
+        return out
+
+    def kstencil(self) -> np.ndarray:
+        """Finite-difference gradient stencil: rows (shell, m0, m1, bx, by, bz, w)."""
+        n = lib().oracle_kstencil(self.h, None)
+        out = np.empty((n, 7))
+        if n:
+            lib().oracle_kstencil(self.h, _ptr(out))
+        return out
+
+    def kgradient(self, u, F: np.ndarray) -> np.ndarray:
+        """(u . grad_k) F by the finite-difference stencil (gradient_space "k")."""
+        a = np.ascontiguousarray(F, dtype=np.complex128)
+        uu = np.ascontiguousarray(u, dtype=np.float64)
+        out = np.empty_like(a)
+        if lib().oracle_kgradient(self.h, _ptr(uu), _ptr(a.view(np.float64)), _ptr(out.view(np.float64))) != 0:
+            raise RuntimeError("oracle: gradient_space is not k")
         return out
 
     def vector_potential(self, t: float) -> np.ndarray:

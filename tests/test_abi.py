@@ -40,5 +40,6 @@ def test_config_defaults_and_unit_quirk():
     assert p["r0"] == [10.0]
     assert config.compose({"r0": 40})["r0"] == [40.0]
     assert config.compose({"peierls": True})["peierls"] is True
-    with pytest.raises(NotImplementedError):
-        config.compose({"gradient_space": "k"})
+    assert config.compose({"gradient_space": "k"})["gradient_space"] == "k"
+    with pytest.raises(ValueError):
+        config.compose({"gradient_space": "x"})
