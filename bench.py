@@ -30,7 +30,8 @@ def kernel_bytes(cls: str, npl: int, nk: int, nb: int, n_e: int, solver: str = "
                  coulomb: bool = False) -> float:
     """Algorithmic HBM bytes per launch: one array of npl planes is
     S = npl*nk*16 B.  grad_col reads rho, writes G (2S); grad_row reads rho
-    and G, writes D (3S); hf_row (forward or inverse row transform) 2S;
+    and G, writes D (3S); kgrad (finite-difference stencil, gradient_space
+    "k") reads rho, writes D (2S); hf_row (forward or inverse row transform) 2S;
     hf_col (column convolution with the real W) 2.5S.  comm reads rho_s, H0,
     n_E xi, D (+ the exchange self energy) and the stage's state arrays and
     writes its outputs: RK4 averages (6 + n_E) S over its four stages, AB of
@@ -39,8 +40,8 @@ def kernel_bytes(cls: str, npl: int, nk: int, nb: int, n_e: int, solver: str = "
     s = 16.0 * npl * nk
     sig = 1 if coulomb else 0
     comm = (6 + n_e + sig) if solver == "RK" else (4 + n_e + order + sig)
-    return {"grad_col": 2 * s, "grad_row": 3 * s, "comm": comm * s, "hf_row": 2 * s, "hf_col": 2.5 * s,
-            "monitor": 16.0 * nb * nk}.get(cls, 0.0)
+    return {"grad_col": 2 * s, "grad_row": 3 * s, "kgrad": 2 * s, "comm": comm * s, "hf_row": 2 * s,
+            "hf_col": 2.5 * s, "monitor": 16.0 * nb * nk}.get(cls, 0.0)
 
 
 def cpu_baseline(params: dict, steps: int, integ: str, target_s: float = 15.0):
@@ -92,6 +93,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="ipa14_hex120")
     ap.add_argument("--layout", choices=["full", "hermitian"], default="full")
+    ap.add_argument("--gradient-space", choices=["R", "k"], default="R",
+                    help="R: spectral derivative (the reference default); k: finite-difference stencil")
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64",
                     help="mixed: FP32-MFMA commutator with FP64 state, gradient and updates")
     ap.add_argument("--graph", type=int, default=20,
@@ -125,6 +128,7 @@ def main():
 
     tmp = tempfile.mkdtemp(prefix="edus_bench_")
     params = models.workload(args.workload, tmp)
+    params["gradient_space"] = args.gradient_space
     layout = edus.LAYOUT_FULL if args.layout == "full" else edus.LAYOUT_HERMITIAN
     prec = edus.PREC_FP64 if args.precision == "fp64" else edus.PREC_MIXED
     sim = edus.Simulation(params, layout=layout, precision=prec)
@@ -161,7 +165,7 @@ def main():
     sim.steps(args.steps)
     eng.synchronize()
     elapsed_direct = time.perf_counter() - t1
-    classes = ["grad_col", "grad_row", "comm", "hf_row", "hf_col", "monitor"]
+    classes = ["grad_col", "grad_row", "kgrad", "comm", "hf_row", "hf_col", "monitor"]
     times = {c: eng.timing_get(c) for c in classes}
     eng.timing_enable(False)
 
@@ -196,6 +200,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": args.workload, "nb": nb, "grid": [sim.N0, sim.N1], "dt_au": sim.dt,
                    "integrator": integ, "layout": args.layout, "precision": args.precision,
+                   "gradient_space": args.gradient_space,
                    "graph_steps": args.graph,
                    "model": "seeded random tight-binding (edus.models)", "tuning": eng.get_tuning().as_dict()},
         "roofline": roofline,
