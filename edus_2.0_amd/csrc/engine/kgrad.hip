@@ -86,38 +86,70 @@ std::vector<double> pinv(std::vector<double> A, int m, int n)
     return P;
 }
 
-__global__ void __launch_bounds__(256) kgrad_kernel(const double2* __restrict__ in, double2* __restrict__ out, int N0,
-                                                    int N1, long per_inst, long total,
-                                                    const KStencilPoint* __restrict__ st, int nst, double e0,
-                                                    double e1, double e2, const double* __restrict__ E_tab)
+// One work-group per (row i0, plane): the row index, the instance and with it
+// E and every stencil coefficient w_b (b.E) are uniform over the group; the
+// threads run along the row (contiguous k), so each stencil tap is a shifted
+// coalesced row read served mostly from cache.  Work-groups are dealt to the
+// 8 XCDs round-robin; the (row, plane) order is remapped so that each XCD
+// gets a contiguous range of rows and the neighbour rows a tap reads sit in
+// its own L2 (read once from HBM, not once per XCD).  Taps are taken kTaps at
+// a time: all their loads are issued before the first one is used.
+constexpr int kTaps = 8;
+
+__global__ void __launch_bounds__(128) kgrad_kernel(const double2* __restrict__ in, double2* __restrict__ out, int N0,
+                                                    int N1, int P, const KStencilPoint* __restrict__ st,
+                                                    int nst, double e0, double e1, double e2,
+                                                    const double* __restrict__ E_tab)
 {
-    const long Nk = long(N0) * N1;
-    for (long o = long(blockIdx.x) * blockDim.x + threadIdx.x; o < total; o += long(gridDim.x) * blockDim.x) {
-        const long inst = o / per_inst;
-        const int k = int((o - inst * per_inst) % Nk);
+    long L;
+    {
+        const long nblk = gridDim.x, q = nblk / 8, r = nblk % 8;
+        const long x = blockIdx.x % 8, i = blockIdx.x / 8;
+        L = x * q + min(x, r) + i;
+    }
+    const int i0 = int(L % N0);
+    {
+        const int plane = int(L / N0);  // instance * P + p
+        const int inst = plane / P;
         double E0 = e0, E1 = e1, E2 = e2;
         if (E_tab) {
             E0 = E_tab[inst * 3 + 0];
             E1 = E_tab[inst * 3 + 1];
             E2 = E_tab[inst * 3 + 2];
         }
-        const int i0 = k / N1, i1 = k - i0 * N1;
-        const double2* plane = in + (o - k);
-        double2 acc = make_double2(0.0, 0.0);
-        for (int b = 0; b < nst; ++b) {
-            const KStencilPoint q = st[b];
-            int j0 = i0 + q.m0, j1 = i1 + q.m1;
-            if (j0 < 0) j0 += N0;
-            if (j0 >= N0) j0 -= N0;
-            if (j1 < 0) j1 += N1;
-            if (j1 >= N1) j1 -= N1;
-            const double bdotu = q.b[0] * E0 + q.b[1] * E1 + q.b[2] * E2;
-            const double c = q.w * bdotu;
-            const double2 v = plane[j0 * N1 + j1];
-            acc.x += c * v.x;
-            acc.y += c * v.y;
+        const long Nk = long(N0) * N1;
+        const double2* pin = in + long(plane) * Nk;
+        double2* pout = out + long(plane) * Nk + long(i0) * N1;
+        for (int i1 = threadIdx.x; i1 < N1; i1 += blockDim.x) {
+            double2 acc = make_double2(0.0, 0.0);
+            for (int b0 = 0; b0 < nst; b0 += kTaps) {
+                double2 v[kTaps];
+                double c[kTaps];
+#pragma unroll
+                for (int t = 0; t < kTaps; ++t) {
+                    v[t] = make_double2(0.0, 0.0);
+                    c[t] = 0.0;
+                    if (b0 + t < nst) {
+                        const KStencilPoint& q = st[b0 + t];
+                        int j0 = i0 + q.m0, j1 = i1 + q.m1;
+                        if (j0 < 0) j0 += N0;
+                        if (j0 >= N0) j0 -= N0;
+                        if (j1 < 0) j1 += N1;
+                        if (j1 >= N1) j1 -= N1;
+                        const double bdotu = q.b[0] * E0 + q.b[1] * E1 + q.b[2] * E2;
+                        c[t] = q.w * bdotu;
+                        v[t] = pin[long(j0) * N1 + j1];
+                    }
+                }
+#pragma unroll
+                for (int t = 0; t < kTaps; ++t)
+                    if (b0 + t < nst) {
+                        acc.x += c[t] * v[t].x;
+                        acc.y += c[t] * v[t].y;
+                    }
+            }
+            pout[i1] = acc;
         }
-        out[o] = acc;
     }
 }
 
@@ -222,10 +254,8 @@ std::vector<KStencilPoint> kgradient_stencil(const double A[3][3], int N0, int N
 void launch_kgrad(const double2* in, double2* out, int N0, int N1, int P, int B, const KStencilPoint* st, int nst,
                   const double E[3], const double* E_tab, hipStream_t s)
 {
-    const long per_inst = long(P) * N0 * N1;
-    const long total = per_inst * std::max(B, 1);
-    const int blocks = int(std::min<long>((total + 255) / 256, 65536));
-    hipLaunchKernelGGL(kgrad_kernel, dim3(blocks), dim3(256), 0, s, in, out, N0, N1, per_inst, total, st, nst,
+    const long nblk = long(N0) * P * std::max(B, 1);
+    hipLaunchKernelGGL(kgrad_kernel, dim3(unsigned(nblk)), dim3(128), 0, s, in, out, N0, N1, P, st, nst,
                        E ? E[0] : 0.0, E ? E[1] : 0.0, E ? E[2] : 0.0, E_tab);
 }
 
