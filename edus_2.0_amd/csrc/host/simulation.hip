@@ -165,7 +165,8 @@ class Simulation {
     DESolver desolver;
 
     bool PrintObservables(double t) const { return int(std::round(t / dt)) % print_stride == 0; }
-    int propagate(int n, double* times, double* laser, double* popb, double* popw, double* vel);
+    int propagate(int n, double* times, double* laser, double* popb, double* popw, double* vel,
+                  double* laser_A = nullptr);
 
   private:
     double jacobian() const;
@@ -400,7 +401,8 @@ void Simulation::setup_coulomb(const edus_sim_params& p)
                           "or set coulomb_center_quirk=0");
 }
 
-int Simulation::propagate(int n, double* times, double* laser, double* popb, double* popw, double* vel)
+int Simulation::propagate(int n, double* times, double* laser, double* popb, double* popw, double* vel,
+                          double* laser_A)
 {
     // Simulation::Propagate / do_onestep (Simulation.cpp:352-432): print before stepping
     int printed = 0;
@@ -413,10 +415,14 @@ int Simulation::propagate(int n, double* times, double* laser, double* popb, dou
             setoflaser.field(t, E);
             if (laser)
                 for (int d = 0; d < 3; ++d) laser[3 * printed + d] = E[d];
+            // the vector potential is evaluated at every print in either gauge
+            // (Laser_A.txt, Simulation.cpp:425), which advances its integration state
+            double A[3];
+            setoflaser.vector_potential(t, A);
+            if (laser_A)
+                for (int d = 0; d < 3; ++d) laser_A[3 * printed + d] = A[d];
             const double2* rho_obs = nullptr;
             if (peierls) {  // the printed observables read rho(R) exp(-i A.R) (Simulation.cpp:447-449,593-595)
-                double A[3];
-                setoflaser.vector_potential(t, A);
                 engine->peierls_sandwich(engine->y, nullptr, nullptr, A, -1, engine->G);
                 rho_obs = engine->G;
             }
@@ -461,6 +467,70 @@ static int guarded_sim(F&& fn)
         edus::set_error(e.what());
         return EDUS_ERR_IO;
     }
+}
+
+// DM0.txt: Operator::print_Rdecay("DM", rwann) of the initial state
+// (Operator.hpp:288-311, called at Simulation.cpp:190): for every R of the
+// Gamma-centred FFT grid (grid order) and every (a, b), the distance
+// |rwann_a + R - rwann_b| (bohr) and rho_ab(R) = (1/N) sum_k exp(-i k.R) rho_ab(k)
+// (fftPair.cpp:89-107), printed as the reference's stream does: the distance
+// with the stream's current precision, the complex value through
+// ostream.cpp:3-8 (setw 25, 10 digits each).  The transform is a separable DFT
+// on the host (set-up only).
+static void write_dm0(const Simulation& S, const char* dir)
+{
+    const int nb = S.nb, N0 = S.N0, N1 = S.N1, Nk = N0 * N1, M = nb * nb;
+    const double kPi = 3.14159265358979323846;
+    std::vector<edus::host::cplx> tmp(size_t(Nk) * M), rR(size_t(Nk) * M);
+    // along axis 1, then axis 0, sign -1
+    edus::host::parallel_for(N0, [&](int i0) {
+        for (int j1 = 0; j1 < N1; ++j1)
+            for (int h = 0; h < M; ++h) {
+                edus::host::cplx a = 0.;
+                for (int i1 = 0; i1 < N1; ++i1) {
+                    const double ph = -2. * kPi * double((long(i1) * j1) % N1) / N1;
+                    a += std::polar(1.0, ph) * S.rho0[size_t(i0 * N1 + i1) * M + h];
+                }
+                tmp[size_t(i0 * N1 + j1) * M + h] = a;
+            }
+    });
+    edus::host::parallel_for(N1, [&](int j1) {
+        for (int j0 = 0; j0 < N0; ++j0)
+            for (int h = 0; h < M; ++h) {
+                edus::host::cplx a = 0.;
+                for (int i0 = 0; i0 < N0; ++i0) {
+                    const double ph = -2. * kPi * double((long(i0) * j0) % N0) / N0;
+                    a += std::polar(1.0, ph) * tmp[size_t(i0 * N1 + j1) * M + h];
+                }
+                rR[size_t(j0 * N1 + j1) * M + h] = a / double(Nk);
+            }
+    });
+    std::vector<std::array<double, 3>> c(nb);
+    for (int n = 0; n < nb; ++n) c[n] = S.material.wannier_centre(n);
+    auto fold = [](double x, double lo, double up, double period) {
+        while (x < lo && std::abs(x - lo) > 1.e-08) x += period;
+        while (x > up || std::abs(x - up) < 1.e-08) x -= period;
+        return x;
+    };
+    std::ofstream f(std::string(dir) + "/DM0.txt");
+    if (!f) throw EngineError(EDUS_ERR_IO, std::string("cannot write ") + dir + "/DM0.txt");
+    for (int j0 = 0; j0 < N0; ++j0)
+        for (int j1 = 0; j1 < N1; ++j1) {
+            const double r0 = fold(j0, -N0 / 2., N0 / 2., N0), r1 = fold(j1, -N1 / 2., N1 / 2., N1);
+            double Rc[3];
+            for (int i = 0; i < 3; ++i) Rc[i] = S.material.A[i][0] * r0 + S.material.A[i][1] * r1;
+            for (int a = 0; a < nb; ++a)
+                for (int b = 0; b < nb; ++b) {
+                    double d2 = 0.;
+                    for (int i = 0; i < 3; ++i) {
+                        const double x = c[a][i] + Rc[i] - c[b][i];
+                        d2 += x * x;
+                    }
+                    const edus::host::cplx v = rR[size_t(j0 * N1 + j1) * M + a * nb + b];
+                    f << std::sqrt(d2) << std::setw(25) << std::setprecision(10) << v.real() << std::setw(25)
+                      << std::setprecision(10) << v.imag() << std::endl;
+                }
+        }
 }
 
 extern "C" {
@@ -528,9 +598,11 @@ int edus_sim_write_outputs(edus_sim* sim, const char* dir, int nsteps, int* npri
         const int n = nsteps < 0 ? S.nsteps : nsteps;
         const int nb = S.nb;
         // at most one printed row per step
-        std::vector<double> t(n + 1), las(3 * size_t(n + 1)), pb(size_t(n + 1) * nb), pw(size_t(n + 1) * nb),
-            v(6 * size_t(n + 1));
-        const int k = S.propagate(n, t.data(), las.data(), pb.data(), pw.data(), v.data());
+        std::vector<double> t(n + 1), las(3 * size_t(n + 1)), lasA(3 * size_t(n + 1)), pb(size_t(n + 1) * nb),
+            pw(size_t(n + 1) * nb), v(6 * size_t(n + 1));
+        // DM0.txt (print_Rdecay of the initial state, Simulation.cpp:190) before stepping
+        write_dm0(S, dir);
+        const int k = S.propagate(n, t.data(), las.data(), pb.data(), pw.data(), v.data(), lasA.data());
         if (nprinted) *nprinted = k;
         const std::string d(dir);
         auto open = [&](const char* name) {
@@ -542,12 +614,14 @@ int edus_sim_write_outputs(edus_sim* sim, const char* dir, int nsteps, int* npri
         // Cartesian vector's setw 24 / 15 digits), :459-467 (populations: setw
         // 30, 14 digits, blank after each band) and :614-620 (velocity: setw
         // 20, 8 digits)
-        std::ofstream ft = open("Time.txt"), fl = open("Laser.txt"), fp = open("Population.txt"),
-                      fw = open("Population_wannier.txt"), fv = open("Velocity.txt");
+        std::ofstream ft = open("Time.txt"), fl = open("Laser.txt"), fa = open("Laser_A.txt"),
+                      fp = open("Population.txt"), fw = open("Population_wannier.txt"), fv = open("Velocity.txt");
         for (int r = 0; r < k; ++r) {
             ft << t[r] << std::endl;
             for (int c = 0; c < 3; ++c) fl << std::setw(24) << std::setprecision(15) << las[3 * size_t(r) + c];
             fl << std::endl;
+            for (int c = 0; c < 3; ++c) fa << std::setw(24) << std::setprecision(15) << lasA[3 * size_t(r) + c];
+            fa << std::endl;
             for (int b = 0; b < nb; ++b) fp << std::setw(30) << std::setprecision(14) << pb[size_t(r) * nb + b] << " ";
             fp << std::endl;
             for (int b = 0; b < nb; ++b) fw << std::setw(30) << std::setprecision(14) << pw[size_t(r) * nb + b] << " ";

@@ -968,10 +968,13 @@ void Simulation::propagate(Observables* obs, int nsteps_override)
     int n = nsteps_override >= 0 ? nsteps_override : nsteps;
     for (int it = 0; it < n; ++it) {
         if (obs && print_now(t_current)) {  // Simulation.cpp:420-429
+            // Laser_A.txt: the vector potential is evaluated at every print,
+            // in either gauge, which advances its integration state (:425)
+            double Av[3];
+            vector_potential(t_current, Av);
+            obs->laser_A.push_back({Av[0], Av[1], Av[2]});
             if (par.peierls) {
                 // the printed observables read rho phased by exp(-i A.R) (Simulation.cpp:447-449,593-595)
-                double Av[3];
-                vector_potential(t_current, Av);
                 rho_obs = rho;
                 fft_planes(rho_obs, -1);
                 peierls_phase(rho_obs, Av, -1);

@@ -114,6 +114,7 @@ struct SimParams {
 struct Observables {
     std::vector<double> time;                  // Time.txt
     std::vector<std::array<double, 3>> laser;  // Laser.txt
+    std::vector<std::array<double, 3>> laser_A;  // Laser_A.txt
     std::vector<std::vector<double>> pop_bloch, pop_wannier;
     std::vector<std::array<double, 6>> velocity;
 };
@@ -155,6 +156,7 @@ class Simulation {
     // O(R) *= exp(i sign Av.R) for every plane of an R-space array
     void peierls_phase(std::vector<cplx>& aR, const double Av[3], int sign) const;
     std::vector<cplx> rho_obs;  // the density matrix the observables read (phased in the Peierls gauge)
+    std::vector<double> last_laser_A;  // Laser_A rows of the last propagate through the C API
     // k-space finite-difference gradient (gradient_space "k"): the shells of
     // the Gamma-centred k mesh and their weights (kGradient.cpp:44-125), and
     // for every b of the first nshells shells its grid offset and Cartesian b

@@ -219,7 +219,17 @@ int oracle_propagate(void* h, int nsteps, double* times, double* laser, double* 
         }
         for (int c = 0; c < 6; ++c) vel[6 * i + c] = o.velocity[i][c];
     }
+    s->last_laser_A.clear();
+    for (auto& a : o.laser_A) s->last_laser_A.insert(s->last_laser_A.end(), a.begin(), a.end());
     return n;
+}
+
+// the vector potential printed (Laser_A.txt) by the last oracle_propagate
+int oracle_laser_A(void* h, double* out)
+{
+    auto* s = static_cast<Simulation*>(h);
+    std::copy(s->last_laser_A.begin(), s->last_laser_A.end(), out);
+    return int(s->last_laser_A.size() / 3);
 }
 
 // stand-alone building blocks for the closed-form tests

@@ -68,6 +68,8 @@ def lib():
         L.oracle_rhs.argtypes = [vp, C.c_double, dp, dp]
         L.oracle_rhs_E.argtypes = [vp, dp, dp, dp]
         L.oracle_rhs_EA.argtypes = [vp, dp, dp, dp, dp]
+        L.oracle_laser_A.argtypes = [vp, dp]
+        L.oracle_laser_A.restype = C.c_int
         L.oracle_kstencil.argtypes = [vp, dp]
         L.oracle_kstencil.restype = C.c_int
         L.oracle_kgradient.argtypes = [vp, dp, dp, dp]
@@ -249,8 +251,10 @@ class Oracle:
         times, laser = np.empty(n), np.empty((n, 3))
         pb, pw, vel = np.empty((n, self.nb)), np.empty((n, self.nb)), np.empty((n, 6))
         k = lib().oracle_propagate(self.h, n, _ptr(times), _ptr(laser), _ptr(pb), _ptr(pw), _ptr(vel))
+        la = np.empty((max(k, 1), 3))
+        lib().oracle_laser_A(self.h, _ptr(la))
         return {"time": times[:k], "laser": laser[:k], "pop_bloch": pb[:k], "pop_wannier": pw[:k],
-                "velocity": vel[:k]}
+                "velocity": vel[:k], "laser_A": la[:k]}
 
 
 def eigh(A: np.ndarray):

@@ -281,6 +281,21 @@ def test_hbn_reference_output_files(tmp_path):
         _say(name, "max diff", np.abs(got - gold).max())
         assert np.abs(got - gold).max() <= tol, name
     assert np.loadtxt(os.path.join(str(tmp_path), "Population_wannier.txt")).shape == (102, 2)
+    # DM0.txt: distance |r_a + R - r_b| and rho(R) at t = 0, against the reference's file
+    got = np.loadtxt(os.path.join(str(tmp_path), "DM0.txt"))
+    gold = np.loadtxt(os.path.join(GOLDEN, "hBN", "DM0.txt"))
+    assert got.shape == gold.shape
+    _say("DM0.txt max diff", np.abs(got - gold).max(axis=0))
+    assert np.abs(got[:, 0] - gold[:, 0]).max() <= 1e-8 * np.abs(gold[:, 0]).max()  # 10 printed digits
+    assert np.abs(got[:, 1:] - gold[:, 1:]).max() <= 1e-10
+    # Laser_A.txt: the vector potential evaluated at every print (no reference
+    # fixture; against the oracle, whose integration is pinned by quadrature)
+    o = oracle_py.Oracle(hbn_params())
+    ref = o.propagate()
+    la = np.loadtxt(os.path.join(str(tmp_path), "Laser_A.txt"))
+    _say("Laser_A.txt max diff", np.abs(la - ref["laser_A"]).max(), "max |A|", np.abs(ref["laser_A"]).max())
+    assert la.shape == ref["laser_A"].shape
+    assert np.abs(la - ref["laser_A"]).max() <= 1e-12 * max(1.0, np.abs(ref["laser_A"]).max())
 
 
 def test_ab4_seeded_by_rk4(model_dir):

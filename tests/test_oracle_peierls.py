@@ -74,3 +74,22 @@ def test_vector_potential_shared_state_over_lasers():
         a2 = o2.vector_potential(tt)
         a1 = o1.vector_potential(tt)
         assert np.allclose(a2, 2.0 * a1, rtol=0, atol=1e-15)
+
+
+def test_laser_A_is_recorded_at_every_print():
+    """Laser_A.txt: the vector potential is evaluated at every print step in
+    either gauge (Simulation.cpp:425); in the length gauge those are its only
+    calls, so it is Simpson's rule over the print intervals."""
+    p = hbn_params(grid=[6, 6, 1])
+    o = oracle_py.Oracle(p)
+    res = o.propagate(40)
+    t, la = res["time"], res["laser_A"]
+    assert la.shape == (len(t), 3)
+    ref = np.zeros(3)
+    prev = 0.0
+    for i, ti in enumerate(t):
+        dT = ti - prev
+        if dT >= 1e-7:
+            ref = ref - dT / 6.0 * (o.E(prev) + 4 * o.E(prev + dT / 2) + o.E(ti))
+            prev = ti
+        assert np.abs(la[i] - ref).max() <= 1e-15 + 1e-12 * np.abs(ref).max()
