@@ -363,22 +363,22 @@ std::vector<FFTChoice> fft_candidates(int axis, int n, int other)
             if (o.two_step == c.two_step && o.cfg == c.cfg && o.L == c.L) return;
         out.push_back(c);
     };
-    const int maxl = two_step_max_lines(n);
-    if (maxl > 0) {
-        std::vector<int> Ls{maxl};
-        if (axis == 1) {
-            Ls.push_back(std::max(1, maxl / 2));
-        } else {
-            for (int L : {8 * (maxl / 8), 16, 8})
-                if (L >= 1 && L <= maxl) Ls.push_back(L);
-            for (int L = std::min(maxl, other); L >= 1; --L) {
-                if (other % L == 0) {
-                    Ls.push_back(L);
-                    break;
+    if (two_step_max_lines(n) > 0) {
+        for (int variant : {0, 1, 2, 3}) {
+            const int maxl = two_step_max_lines(n, variant);
+            std::vector<int> Ls{maxl};
+            if (axis == 1) {
+                Ls.push_back(std::max(1, maxl / 2));
+            } else {
+                for (int L : {8 * (maxl / 8), 32, 16, 8})
+                    if (L >= 1 && L <= maxl) Ls.push_back(L);
+                for (int L = std::min(maxl, other); L >= 1; --L) {
+                    if (other % L == 0) {
+                        Ls.push_back(L);
+                        break;
+                    }
                 }
             }
-        }
-        for (int variant : {0, 1}) {
             for (int L : Ls) {
                 FFTChoice c;
                 c.two_step = 1;
@@ -1201,7 +1201,7 @@ int edus_set_tuning(edus_ctx* ctx, const edus_tuning* in)
         edus::FFTChoice next[2];
         for (int a = 0; a < 2; ++a) {
             const int n = a == 0 ? e.N0 : e.N1;
-            const int maxl = edus::two_step_max_lines(n);
+            const int maxl = edus::two_step_max_lines(n, in->fft_cfg[a]);
             edus::FFTChoice c;
             c.two_step = in->fft_family[a];
             c.cfg = in->fft_cfg[a];
@@ -1209,7 +1209,7 @@ int edus_set_tuning(edus_ctx* ctx, const edus_tuning* in)
             if (c.two_step != (maxl > 0 ? 1 : 0)) throw EngineError(EDUS_ERR_ARG, "fft_family is fixed by the axis length");
             int lmax;
             if (c.two_step) {
-                if (c.cfg < 0 || c.cfg > 1) throw EngineError(EDUS_ERR_ARG, "two-step variant must be 0 or 1");
+                if (c.cfg < 0 || c.cfg > 3) throw EngineError(EDUS_ERR_ARG, "two-step variant must be 0..3");
                 lmax = maxl;
             } else {
                 if (c.cfg < 0 || c.cfg >= edus::kNumFFTConfigs || edus::fft_lines(c.cfg, n) * n > edus::kFFTConfigs[c.cfg].values)

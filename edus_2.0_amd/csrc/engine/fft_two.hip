@@ -305,19 +305,38 @@ __global__ void __launch_bounds__(NT) two_step_kernel(AxisFFTArgs a)
     }
 }
 
-template <int RA, int RB, int MODE, bool EARLY>
+template <int RA, int RB, int NT, int MODE, bool EARLY>
 void launch_mode(const AxisFFTArgs& a, int blocks, size_t lds, hipStream_t s)
 {
     using Sh = TwoStepShape<RA, RB>;
-    auto kern = &two_step_kernel<RA, RB, kTwoStepThreads, MODE, EARLY>;
+    auto kern = &two_step_kernel<RA, RB, NT, MODE, EARLY>;
     static bool raised = false;
     if (!raised && lds > 0) {
-        const int maxl = kTwoStepThreads / std::max(RA, RB);
+        const int maxl = NT / std::max(RA, RB);
         (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
                                   int((Sh::n + maxl * Sh::LS) * sizeof(double2)));
         raised = true;
     }
-    hipLaunchKernelGGL(kern, dim3(blocks), dim3(kTwoStepThreads), lds, s, a);
+    hipLaunchKernelGGL(kern, dim3(blocks), dim3(NT), lds, s, a);
+}
+
+// variant: bit 0 the early addend load (derivative mode), bit 1 work-groups
+// of 512 threads (twice the lines, longer contiguous runs along columns)
+template <int RA, int RB, int NT>
+void launch_nt(const AxisFFTArgs& a, int variant, int blocks, size_t lds, hipStream_t s)
+{
+    switch (a.mode) {
+        case 0:
+            if (variant & 1)
+                launch_mode<RA, RB, NT, 0, true>(a, blocks, lds, s);
+            else
+                launch_mode<RA, RB, NT, 0, false>(a, blocks, lds, s);
+            break;
+        case 1: launch_mode<RA, RB, NT, 1, false>(a, blocks, lds, s); break;
+        case 2: launch_mode<RA, RB, NT, 2, false>(a, blocks, lds, s); break;
+        case 3: launch_mode<RA, RB, NT, 3, false>(a, blocks, lds, s); break;
+        default: launch_mode<RA, RB, NT, 4, false>(a, blocks, lds, s); break;
+    }
 }
 
 template <int RA, int RB>
@@ -330,18 +349,10 @@ void launch_two(const AxisFFTArgs& a, int variant, hipStream_t s)
     else
         blocks = a.P * ((a.N1 + a.L - 1) / a.L);
     const size_t lds = RB == 1 ? 0 : size_t(Sh::n + a.L * Sh::LS) * sizeof(double2);
-    switch (a.mode) {
-        case 0:
-            if (variant == 1)
-                launch_mode<RA, RB, 0, true>(a, blocks, lds, s);
-            else
-                launch_mode<RA, RB, 0, false>(a, blocks, lds, s);
-            break;
-        case 1: launch_mode<RA, RB, 1, false>(a, blocks, lds, s); break;
-        case 2: launch_mode<RA, RB, 2, false>(a, blocks, lds, s); break;
-        case 3: launch_mode<RA, RB, 3, false>(a, blocks, lds, s); break;
-        default: launch_mode<RA, RB, 4, false>(a, blocks, lds, s); break;
-    }
+    if (variant & 2)
+        launch_nt<RA, RB, 2 * kTwoStepThreads>(a, variant, blocks, lds, s);
+    else
+        launch_nt<RA, RB, kTwoStepThreads>(a, variant, blocks, lds, s);
 }
 
 // Line lengths with a two-step kernel and their split n = RA * RB.
@@ -354,12 +365,13 @@ void launch_two(const AxisFFTArgs& a, int variant, hipStream_t s)
 
 }  // namespace
 
-int two_step_max_lines(int n)
+int two_step_max_lines(int n, int variant)
 {
+    const int nt = (variant & 2) ? 2 * kTwoStepThreads : kTwoStepThreads;
     switch (n) {
 #define EDUS_X(N, A, B) \
     case N:             \
-        return (B) == 1 ? kTwoStepThreads : kTwoStepThreads / std::max((A), (B));
+        return (B) == 1 ? nt : nt / std::max((A), (B));
         EDUS_TWO_STEP_SIZES(EDUS_X)
 #undef EDUS_X
         default: return 0;

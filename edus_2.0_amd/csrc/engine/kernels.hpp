@@ -97,7 +97,9 @@ void launch_axis_fft(const AxisFFTArgs& a, int cfg, hipStream_t s);
 // Two-step (register DFT + one LDS exchange) axis FFT: lines per work-group
 // it allows for a line length n (0: no kernel for this n), and its launch.
 // variant 1 loads the derivative's addend one phase ahead.
-int two_step_max_lines(int n);
+// lines per work-group of the two-step kernel for a line of n (0: no such
+// kernel); variant bit 1 doubles the work-group (512 threads)
+int two_step_max_lines(int n, int variant = 0);
 void launch_two_step(const AxisFFTArgs& a, int variant, hipStream_t s);
 
 // The kernel family and tile used for one grid axis.  The family follows

@@ -165,7 +165,8 @@ int edus_timing_get(edus_ctx* ctx, const char* kernel_class, double* total_ms, l
  * instance to check that every choice gives the same results):
  *   fft_family[a]  1: two-step register FFT, 0: LDS Stockham; fixed by the
  *                  length of grid axis a (not settable)
- *   fft_cfg[a]     Stockham configuration 0..4, or two-step variant 0..1
+ *   fft_cfg[a]     Stockham configuration 0..4, or two-step variant 0..3 (bit 0:
+ *                  addend loaded early, bit 1: 512-thread work-groups)
  *   fft_lines[a]   lines per work-group
  *   comm_kind      0: FP64 FMAs, 1: FP64 MFMA (EDUS_PREC_FP64);
  *                  2: FP32 MFMA (EDUS_PREC_MIXED, not settable)
