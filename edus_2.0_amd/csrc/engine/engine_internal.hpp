@@ -133,6 +133,9 @@ class Engine {
     void observables(double* pop_bloch, double* pop_wannier, double* vel, int inst = 0,
                      const double2* rho_override = nullptr);
     void collect_timing();
+    // k -> R of every plane of a FULL-layout field, sign -1 and 1/Nk
+    // (Operator::go_to_R, Operator.hpp:566-606, fftPair.cpp:89-107); uses G
+    void transform_to_R(const double2* in, double2* out);
 
   private:
     std::vector<void*> owned;

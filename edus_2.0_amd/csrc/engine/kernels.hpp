@@ -132,4 +132,21 @@ std::vector<KStencilPoint> kgradient_stencil(const double A[3][3], int N0, int N
 void launch_kgrad(const double2* in, double2* out, int N0, int N1, int P, int B, const KStencilPoint* st, int nst,
                   const double E[3], const double* E_tab, hipStream_t s);
 
+
+// Lindhard response of the screened-potential application: launch shape for
+// nb bands, nf filled, an N0 x N1 grid (element pairs per lane, q per
+// work-group, LDS sizes), and the launch.  U: [Nk][nb][nb] eigenvectors in
+// columns, eps: [Nk][nb]; work: lindhard_work_bytes of device scratch (the
+// band-split copy of U); X: planes [nb * nb][Nk].
+struct LindhardPlan {
+    int nb, nv, nc, N0, N1, Nk;
+    int Pg, G, W, tri, TQ;
+    int nqb_row, nR, nvp, buf, blocks;
+    size_t lds;
+};
+LindhardPlan lindhard_plan(int nb, int nf, int N0, int N1);
+size_t lindhard_work_bytes(const LindhardPlan& p);
+void launch_lindhard(const LindhardPlan& p, const double2* U, const double* eps, double2* work, double2* X,
+                     hipStream_t s);
+
 }  // namespace edus

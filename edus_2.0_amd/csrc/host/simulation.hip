@@ -533,6 +533,97 @@ static void write_dm0(const Simulation& S, const char* dir)
         }
 }
 
+// ScreenedPotential::Calculate (src/Screened_potential/Screened_potential.cpp:139-199)
+// on a set-up simulation: the application derives from Simulation and uses
+// its eigenvectors, band energies, grid and Wannier centres.  All results are
+// R-space fields [R][a][b] over the FFT grid (grid order, Gamma-centred R):
+//   X       ResponseFunction (:17-58): the Lindhard sum on the device, then k -> R
+//   V       BareCoulomb (:79-104): V_ab(R) = 1/|r_a + R - r_b|, 0 below 1e-5 bohr
+//   Eps     Epsilon (:122-132): 1 - V_ab(R) X_ab(R), element by element
+//   InvEps  (:149-157): 1/Eps where |Eps| >= 1e-3, else 0
+//   W       (:168-174): InvEps_ab(R) V_ab(R)
+struct ScreenedFields {
+    std::vector<edus::host::cplx> X, V, Eps, InvEps, W;
+};
+
+static ScreenedFields screened_potential(Simulation& S)
+{
+    using edus::host::cplx;
+    edus::Engine& e = *S.engine;
+    const int nb = S.nb, N0 = S.N0, N1 = S.N1, Nk = S.Nk, M = nb * nb;
+    const edus::LindhardPlan plan = edus::lindhard_plan(nb, S.filled, N0, N1);
+    ScreenedFields out;
+    // ResponseFunction on the device, in the engine's per-step scratch arrays
+    // (FULL layout: nb^2 planes each): U into G, the energies into D, the
+    // band-split copy of U into f, X(k) into aux; then k -> R into D.
+    if (e.layout != edus::kFull || e.B != 1)
+        throw EngineError(EDUS_ERR_UNSUPPORTED, "the screened potential needs a simulation in the FULL layout");
+    if (edus::lindhard_work_bytes(plan) > size_t(e.npl) * Nk * sizeof(double2))
+        throw EngineError(EDUS_ERR_STATE, "screened potential: scratch arrays too small");
+    EDUS_HIP_CHECK(hipMemcpyAsync(e.G, S.U.data(), size_t(Nk) * M * sizeof(double2), hipMemcpyHostToDevice, e.s));
+    EDUS_HIP_CHECK(hipMemcpyAsync(e.D, S.eps.data(), size_t(Nk) * nb * sizeof(double), hipMemcpyHostToDevice, e.s));
+    edus::launch_lindhard(plan, e.G, reinterpret_cast<const double*>(e.D), e.f, e.aux, e.s);
+    e.transform_to_R(e.aux, e.D);
+    out.X.resize(size_t(Nk) * M);
+    e.download(e.D, reinterpret_cast<double*>(out.X.data()));  // synchronises the stream
+
+    std::vector<std::array<double, 3>> c(nb);
+    for (int n = 0; n < nb; ++n) c[n] = S.material.wannier_centre(n);
+    out.V.assign(size_t(Nk) * M, 0.);
+    out.Eps.resize(out.V.size());
+    out.InvEps.resize(out.V.size());
+    out.W.resize(out.V.size());
+    for (int j0 = 0; j0 < N0; ++j0)
+        for (int j1 = 0; j1 < N1; ++j1) {
+            const double r0 = edus::host::fold(j0, -N0 / 2., N0 / 2., N0), r1 = edus::host::fold(j1, -N1 / 2., N1 / 2., N1);
+            double Rc[3];
+            for (int i = 0; i < 3; ++i) Rc[i] = S.material.A[i][0] * r0 + S.material.A[i][1] * r1;
+            for (int a = 0; a < nb; ++a)
+                for (int b = 0; b < nb; ++b) {
+                    const size_t o = size_t(j0 * N1 + j1) * M + a * nb + b;
+                    double d2 = 0.;
+                    for (int i = 0; i < 3; ++i) {
+                        const double x = c[a][i] + Rc[i] - c[b][i];
+                        d2 += x * x;
+                    }
+                    const double d = std::sqrt(d2);
+                    out.V[o] = d < 1.e-05 ? 0. : 1. / d;
+                    out.Eps[o] = -out.V[o] * out.X[o] + 1.;
+                    out.InvEps[o] = std::abs(out.Eps[o]) < 1.e-03 ? cplx(0.) : 1. / out.Eps[o];
+                    out.W[o] = out.InvEps[o] * out.V[o];
+                }
+        }
+    return out;
+}
+
+// Operator::print_Rdecay(name, rwann) (Operator.hpp:288-311): file name + "0.txt"
+// (one process), per R and (a, b) the distance |r_a + R - r_b| and the value,
+// both through the stream's defaults (6 significant digits, "(re,im)").
+static void write_rdecay(const Simulation& S, const std::vector<edus::host::cplx>& f, const std::string& path)
+{
+    const int nb = S.nb, N0 = S.N0, N1 = S.N1, M = nb * nb;
+    std::ofstream os(path);
+    if (!os) throw EngineError(EDUS_ERR_IO, "cannot write " + path);
+    std::vector<std::array<double, 3>> c(nb);
+    for (int n = 0; n < nb; ++n) c[n] = S.material.wannier_centre(n);
+    for (int j0 = 0; j0 < N0; ++j0)
+        for (int j1 = 0; j1 < N1; ++j1) {
+            const double r0 = edus::host::fold(j0, -N0 / 2., N0 / 2., N0), r1 = edus::host::fold(j1, -N1 / 2., N1 / 2., N1);
+            double Rc[3];
+            for (int i = 0; i < 3; ++i) Rc[i] = S.material.A[i][0] * r0 + S.material.A[i][1] * r1;
+            for (int a = 0; a < nb; ++a)
+                for (int b = 0; b < nb; ++b) {
+                    double d2 = 0.;
+                    for (int i = 0; i < 3; ++i) {
+                        const double x = c[a][i] + Rc[i] - c[b][i];
+                        d2 += x * x;
+                    }
+                    os << std::sqrt(d2);
+                    os << f[size_t(j0 * N1 + j1) * M + a * nb + b] << std::endl;
+                }
+        }
+}
+
 extern "C" {
 
 edus_sim* edus_sim_create(const edus_sim_params* p)
@@ -629,6 +720,39 @@ int edus_sim_write_outputs(edus_sim* sim, const char* dir, int nsteps, int* npri
             for (int c = 0; c < 6; ++c) fv << std::setw(20) << std::setprecision(8) << v[6 * size_t(r) + c];
             fv << std::endl;
         }
+    });
+}
+
+int edus_sim_screened(edus_sim* sim, double* X, double* V, double* Eps, double* InvEps, double* W)
+{
+    if (!sim) { edus::set_error("null simulation"); return EDUS_ERR_ARG; }
+    return guarded_sim([&] {
+        const ScreenedFields f = screened_potential(*sim->s);
+        auto put = [](const std::vector<edus::host::cplx>& v, double* dst) {
+            if (dst) std::memcpy(dst, v.data(), v.size() * sizeof(edus::host::cplx));
+        };
+        put(f.X, X);
+        put(f.V, V);
+        put(f.Eps, Eps);
+        put(f.InvEps, InvEps);
+        put(f.W, W);
+    });
+}
+
+int edus_sim_write_screened(edus_sim* sim, const char* dir)
+{
+    if (!sim || !dir) { edus::set_error("null argument"); return EDUS_ERR_ARG; }
+    return guarded_sim([&] {
+        const Simulation& S = *sim->s;
+        const ScreenedFields f = screened_potential(*sim->s);
+        const std::string d(dir);
+        // the order of ScreenedPotential::Calculate; the reference's
+        // inverse-permittivity print goes to "InverseEpsilon0.txt"
+        write_rdecay(S, f.X, d + "/X0.txt");
+        write_rdecay(S, f.V, d + "/V0.txt");
+        write_rdecay(S, f.Eps, d + "/Epsilon0.txt");
+        write_rdecay(S, f.InvEps, d + "/InverseEpsilon0.txt");
+        write_rdecay(S, f.W, d + "/W0.txt");
     });
 }
 

@@ -93,6 +93,7 @@ EXPORTS = [
     "edus_mlp_destroy", "edus_mlp_num_params", "edus_mlp_set_params", "edus_mlp_get_params", "edus_mlp_forward",
     "edus_mlp_gradients", "edus_mlp_train", "edus_sim_create", "edus_sim_destroy", "edus_sim_info",
     "edus_sim_engine", "edus_sim_field", "edus_sim_time", "edus_sim_propagate", "edus_sim_steps", "edus_sim_write_outputs", "edus_sim_get",
+    "edus_lindhard", "edus_lindhard_device", "edus_sim_screened", "edus_sim_write_screened",
 ]
 
 
@@ -165,6 +166,14 @@ def lib():
         L.edus_sim_write_outputs.argtypes = [vp, C.c_char_p, C.c_int, ip]
         L.edus_sim_get.restype = C.c_long
         L.edus_sim_get.argtypes = [vp, C.c_char_p, dp]
+        L.edus_lindhard.restype = C.c_int
+        L.edus_lindhard.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, dp, dp, dp]
+        L.edus_lindhard_device.restype = C.c_int
+        L.edus_lindhard_device.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, vp, vp, vp, vp, vp]
+        L.edus_sim_screened.restype = C.c_int
+        L.edus_sim_screened.argtypes = [vp, dp, dp, dp, dp, dp]
+        L.edus_sim_write_screened.restype = C.c_int
+        L.edus_sim_write_screened.argtypes = [vp, C.c_char_p]
         _LIB = L
     return _LIB
 
@@ -340,6 +349,30 @@ def kgrad_stencil(A, N0: int, N1: int) -> np.ndarray:
     out = np.empty((n, 7))
     lib().edus_kgrad_stencil(C.byref(a), int(N0), int(N1), _dp(out), n)
     return out
+
+
+def lindhard(U, eps, N0: int, N1: int, filledbands: int) -> np.ndarray:
+    """edus_lindhard: the Lindhard response X(q) [q][a][b] of the screened-potential
+    application (Screened_potential.cpp:17-58) for eigenvectors U [k][a][n]
+    (bands in columns, ascending) and energies eps [k][n], computed on the device."""
+    U = np.ascontiguousarray(U, dtype=np.complex128)
+    e = np.ascontiguousarray(eps, dtype=np.float64)
+    nk, nb, nb2 = U.shape
+    if nb != nb2 or nk != int(N0) * int(N1) or e.shape != (nk, nb):
+        raise ValueError("U must be [N0*N1][nb][nb] and eps [N0*N1][nb]")
+    X = np.empty_like(U)
+    _check(lib().edus_lindhard(nb, int(N0), int(N1), int(filledbands), _dp(U.view(np.float64)), _dp(e),
+                               _dp(X.view(np.float64))))
+    return X
+
+
+def lindhard_device(nb: int, N0: int, N1: int, filledbands: int, U_ptr: int, eps_ptr: int, work_ptr: int,
+                    X_ptr: int, stream: int = 0):
+    """edus_lindhard_device on device pointers (U [k][a][n], eps [k][n], work of
+    Nk*nb*nb complex, X out as planes [nb*nb][Nk]) in the given HIP stream."""
+    _check(lib().edus_lindhard_device(int(nb), int(N0), int(N1), int(filledbands), C.c_void_p(U_ptr),
+                                      C.c_void_p(eps_ptr), C.c_void_p(work_ptr), C.c_void_p(X_ptr),
+                                      C.c_void_p(stream)))
 
 
 def _fp(a):
@@ -552,6 +585,18 @@ class Simulation:
         _check(lib().edus_sim_write_outputs(self.h, directory.encode(), -1 if nsteps is None else int(nsteps),
                                             C.byref(k)))
         return k.value
+
+    def screened_potential(self) -> dict:
+        """edus_sim_screened (ScreenedPotential::Calculate): the R-space fields
+        X, V, Eps, InvEps, W, each [R][nb][nb] over the FFT grid."""
+        names = ("X", "V", "Eps", "InvEps", "W")
+        out = {k: np.empty((self.Nk, self.nb, self.nb), dtype=np.complex128) for k in names}
+        _check(lib().edus_sim_screened(self.h, *(_dp(out[k].view(np.float64)) for k in names)))
+        return out
+
+    def write_screened(self, directory: str):
+        """edus_sim_write_screened: X0.txt, V0.txt, Epsilon0.txt, InverseEpsilon0.txt, W0.txt."""
+        _check(lib().edus_sim_write_screened(self.h, directory.encode()))
 
     def propagate(self, nsteps: int | None = None) -> dict:
         n = self.nsteps if nsteps is None else int(nsteps)

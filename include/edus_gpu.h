@@ -342,14 +342,46 @@ int edus_sim_propagate(edus_sim* sim, int nsteps, double* times, double* laser, 
 /* Advance nsteps without observables (benchmark path). */
 int edus_sim_steps(edus_sim* sim, int nsteps);
 /* Propagate nsteps (< 0: the whole run) and write the reference's text
- * outputs into directory dir: Time.txt, Laser.txt, Population.txt,
- * Population_wannier.txt, Velocity.txt, one row per printed step in the
- * reference's formats (Simulation.cpp:420-429,459-467,614-620).  Laser_A.txt
- * and output.h5 are not written. */
+ * outputs into directory dir: DM0.txt (the initial state in R,
+ * Simulation.cpp:190 with Operator.hpp:288-311) and Time.txt, Laser.txt,
+ * Laser_A.txt, Population.txt, Population_wannier.txt, Velocity.txt, one row
+ * per printed step in the reference's formats (Simulation.cpp:420-429,
+ * 459-467,614-620).  output.h5 is not written. */
 int edus_sim_write_outputs(edus_sim* sim, const char* dir, int nsteps, int* nprinted);
 /* Set-up arrays for inspection: "H0k", "rk0", "rk1", "rk2", "U", "Vel0",
- * "Vel1", "Vel2", "W", "rho0R", "hartree" ([nb][nb]), "eps" (real [Nk][nb]). */
+ * "Vel1", "Vel2", "W", "rho0" (initial state in k), "hartree" ([nb][nb]),
+ * "eps" (real [Nk][nb]). */
 long edus_sim_get(edus_sim* sim, const char* name, double* out);
+
+/* ---- screened-potential application (src/Screened_potential/) ---- */
+/* ScreenedPotential::ResponseFunction (Screened_potential.cpp:17-58) on the
+ * device: the Lindhard response in the point-like approximation,
+ *   X_ab(q) = 1/Nk sum_k sum_{v < nf <= c} U_k(a,v) U_k+q(b,c) conj(U_k+q(a,c))
+ *             conj(U_k(b,v)) / (e_v(k) - e_c(k+q)),
+ * for the eigenvectors U ([Nk][nb][nb], U(a,n) at a*nb+n, bands ascending)
+ * and band energies eps ([Nk][nb]) on an N0 x N1 grid (k order i1 + N1 i0);
+ * 2 <= nb <= 16.  Host arrays; X out [Nk][nb][nb] (q, a, b), interleaved
+ * complex.  Results agree with the reference's loop to rounding (another
+ * summation order). */
+int edus_lindhard(int nb, int N0, int N1, int filledbands, const double* U, const double* eps, double* X);
+/* The same on device arrays in the stream `stream` (a hipStream_t, 0: the
+ * null stream); work: device scratch of Nk*nb*nb complex doubles (the
+ * band-split copy of U); X out as planes [nb*nb][Nk] (plane a*nb+b).
+ * Allocates nothing. */
+int edus_lindhard_device(int nb, int N0, int N1, int filledbands, const double* U, const double* eps, double* work,
+                         double* X, void* stream);
+/* ScreenedPotential::Calculate (Screened_potential.cpp:139-199) on a set-up
+ * simulation (the application derives from Simulation): X(R) (the response
+ * transformed k -> R, Operator::go_to_R), the bare Coulomb V(R)
+ * = 1/|r_a + R - r_b| (0 below 1e-5 bohr), Eps(R) = 1 - V X, its elementwise
+ * inverse (0 where |Eps| < 1e-3) and W(R) = Eps^-1 V; each [Nk][nb][nb] over
+ * the FFT R grid in grid order; null outputs are skipped.  The simulation must
+ * use the FULL layout.  HDF5 output (output.h5) is not written. */
+int edus_sim_screened(edus_sim* sim, double* X, double* V, double* Eps, double* InvEps, double* W);
+/* The same, written as the reference's Operator::print_Rdecay files
+ * (Operator.hpp:288-311) X0.txt, V0.txt, Epsilon0.txt, InverseEpsilon0.txt and
+ * W0.txt in directory dir. */
+int edus_sim_write_screened(edus_sim* sim, const char* dir);
 
 #ifdef __cplusplus
 }

@@ -351,6 +351,13 @@ Simulation::Simulation(const SimParams& p) : par(p)
             for (int d = 0; d < 3; ++d) Rgc[d][ik] = reduce_coord(c[d], -N[d] / 2., N[d] / 2., N[d]);
         }
 
+    // Wannier centres (Model.hpp:72-78); tb.r is in bohr here
+    rwann.assign(nb, {0., 0., 0.});
+    for (int iR = 0; iR < tb.nR; ++iR)
+        if (std::abs(tb.R[iR][0]) + std::abs(tb.R[iR][1]) + std::abs(tb.R[iR][2]) < 1e-6)
+            for (int n = 0; n < nb; ++n)
+                for (int d = 0; d < 3; ++d) rwann[n][d] = tb.r[d][size_t(iR) * M + n * nb + n].real();
+
     if (p.gradient_space == "k") setup_kgradient();
     else if (p.gradient_space != "R") throw std::runtime_error("gradient_space must be R or k");
 
@@ -993,6 +1000,70 @@ void Simulation::propagate(Observables* obs, int nsteps_override)
             obs->velocity.push_back(v);
         }
         step();
+    }
+}
+
+
+std::vector<cplx> lindhard(int nb, int N0, int N1, int filled, const cplx* U, const double* eps)
+{
+    const int Nk = N0 * N1, M = nb * nb;
+    std::vector<cplx> Udag(size_t(Nk) * M);
+    for (int k = 0; k < Nk; ++k)
+        for (int a = 0; a < nb; ++a)
+            for (int n = 0; n < nb; ++n) Udag[size_t(k) * M + n * nb + a] = std::conj(U[size_t(k) * M + a * nb + n]);
+    std::vector<cplx> X(size_t(Nk) * M, 0.);
+#pragma omp parallel for schedule(dynamic)
+    for (int iq = 0; iq < Nk; ++iq) {
+        const int q0 = iq / N1, q1 = iq % N1;
+        for (int ik = 0; ik < Nk; ++ik) {
+            // MeshGrid::find(k + q) on the periodic mesh
+            const int k0 = ik / N1, k1 = ik % N1;
+            const int ikp = ((k0 + q0) % N0) * N1 + (k1 + q1) % N1;
+            const cplx* Uk = U + size_t(ik) * M;
+            const cplx* Ukp = U + size_t(ikp) * M;
+            const cplx* Udk = Udag.data() + size_t(ik) * M;
+            const cplx* Udkp = Udag.data() + size_t(ikp) * M;
+            for (int a = 0; a < nb; ++a)
+                for (int b = 0; b < nb; ++b)
+                    for (int c = filled; c < nb; ++c)
+                        for (int v = 0; v < filled; ++v)
+                            X[size_t(iq) * M + a * nb + b] +=
+                                (Uk[a * nb + v] * Ukp[b * nb + c] * Udkp[c * nb + a] * Udk[v * nb + b]) /
+                                (double(Nk) * (eps[size_t(ik) * nb + v] - eps[size_t(ikp) * nb + c]));
+        }
+    }
+    return X;
+}
+
+void Simulation::screened_potential(std::vector<cplx>& XR, std::vector<cplx>& VR, std::vector<cplx>& EpsR,
+                                    std::vector<cplx>& InvEpsR, std::vector<cplx>& WR) const
+{
+    // ResponseFunction, then X_.go_to_R() (sign -1, 1/N)
+    XR = lindhard(nb, N[0], N[1], par.filledbands, U.data(), eps.data());
+    fft_planes(XR, -1);
+    VR.assign(size_t(Nk) * M, 0.);
+    EpsR.assign(size_t(Nk) * M, 0.);
+    InvEpsR.assign(size_t(Nk) * M, 0.);
+    WR.assign(size_t(Nk) * M, 0.);
+    for (int iR = 0; iR < Nk; ++iR) {
+        double Rc[3];
+        for (int i = 0; i < 3; ++i) Rc[i] = A[i][0] * Rgc[0][iR] + A[i][1] * Rgc[1][iR] + A[i][2] * Rgc[2][iR];
+        for (int a = 0; a < nb; ++a)
+            for (int b = 0; b < nb; ++b) {
+                const size_t o = size_t(iR) * M + a * nb + b;
+                // BareCoulomb (:79-104)
+                double d2 = 0.;
+                for (int i = 0; i < 3; ++i) {
+                    const double x = rwann[a][i] + Rc[i] - rwann[b][i];
+                    d2 += x * x;
+                }
+                const double d = std::sqrt(d2);
+                VR[o] = d < 1.e-05 ? cplx(0.) : cplx(1. / d);
+                // Epsilon (:122-132), inverse (:149-157), W (:168-174)
+                EpsR[o] = -VR[o] * XR[o] + 1.;
+                InvEpsR[o] = std::abs(EpsR[o]) < 1.e-03 ? cplx(0.) : 1. / EpsR[o];
+                WR[o] = InvEpsR[o] * VR[o];
+            }
     }
 }
 

@@ -175,6 +175,14 @@ class Simulation {
     void populations(std::vector<double>& bloch, std::vector<double>& wannier) const;
     void velocity(double v[6]) const;
     void rho_R(std::vector<cplx>& out) const;       // FFT k->R of rho (1/N)
+    // Wannier centres (Model.hpp:72-78): the diagonal of r(R = 0), bohr
+    std::vector<std::array<double, 3>> rwann;
+    // ScreenedPotential::Calculate (Screened_potential.cpp:139-199) on this
+    // simulation's eigenvectors and energies: X(R) (ResponseFunction then
+    // go_to_R), the bare Coulomb V(R), Eps(R) = 1 - V X, its elementwise
+    // inverse and W(R) = InvEps V, each [R][n][m] in grid order
+    void screened_potential(std::vector<cplx>& XR, std::vector<cplx>& VR, std::vector<cplx>& EpsR,
+                            std::vector<cplx>& InvEpsR, std::vector<cplx>& WR) const;
     bool print_now(double t) const;
     void fft_planes_public(std::vector<cplx>& a, int sign) const { fft_planes(a, sign); }
 
@@ -186,5 +194,11 @@ class Simulation {
 };
 
 double struve0(double x);                          // ModelCoulomb.cpp:7-44
+
+// ScreenedPotential::ResponseFunction (Screened_potential.cpp:17-58): the
+// Lindhard response X(q) [q][a][b] for eigenvectors U [k][a][n] (bands in
+// columns, ascending) and energies eps [k][n] on an N0 x N1 grid, with the
+// reference's loop order (q, k, a, b, c, v) and its per-term division.
+std::vector<cplx> lindhard(int nb, int N0, int N1, int filled, const cplx* U, const double* eps);
 
 }  // namespace oracle

@@ -194,6 +194,38 @@ void oracle_observables(void* h, double* popb, double* popw, double* vel6)
     s->velocity(vel6);
 }
 
+// Lindhard response for given eigenvectors and energies (free function)
+int oracle_lindhard(int nb, int N0, int N1, int filled, const double* U, const double* eps, double* X)
+{
+    try {
+        const std::vector<cplx> x = lindhard(nb, N0, N1, filled, reinterpret_cast<const cplx*>(U), eps);
+        std::memcpy(X, x.data(), x.size() * sizeof(cplx));
+        return 0;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return -1;
+    }
+}
+
+// Wannier centres [nb][3] (bohr)
+void oracle_rwann(void* h, double* out)
+{
+    auto* s = static_cast<Simulation*>(h);
+    for (int n = 0; n < s->nb; ++n)
+        for (int d = 0; d < 3; ++d) out[3 * n + d] = s->rwann[n][d];
+}
+
+// ScreenedPotential::Calculate on this simulation; null outputs skipped
+void oracle_screened(void* h, double* X, double* V, double* Eps, double* InvEps, double* W)
+{
+    auto* s = static_cast<Simulation*>(h);
+    std::vector<cplx> f[5];
+    s->screened_potential(f[0], f[1], f[2], f[3], f[4]);
+    double* out[5] = {X, V, Eps, InvEps, W};
+    for (int i = 0; i < 5; ++i)
+        if (out[i]) std::memcpy(out[i], f[i].data(), f[i].size() * sizeof(cplx));
+}
+
 void oracle_rhoR(void* h, double* out)
 {
     auto* s = static_cast<Simulation*>(h);

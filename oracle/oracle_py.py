@@ -80,6 +80,10 @@ def lib():
         L.oracle_E.argtypes = [vp, C.c_double, dp]
         L.oracle_observables.argtypes = [vp, dp, dp, dp]
         L.oracle_rhoR.argtypes = [vp, dp]
+        L.oracle_lindhard.restype = C.c_int
+        L.oracle_lindhard.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, dp, dp, dp]
+        L.oracle_screened.argtypes = [vp, dp, dp, dp, dp, dp]
+        L.oracle_rwann.argtypes = [vp, dp]
         L.oracle_propagate.restype = C.c_int
         L.oracle_propagate.argtypes = [vp, C.c_int, dp, dp, dp, dp, dp]
         L.oracle_eigh.argtypes = [C.c_int, dp, dp, dp]
@@ -246,6 +250,18 @@ class Oracle:
         lib().oracle_rhoR(self.h, _ptr(out.view(np.float64)))
         return out
 
+    def screened_potential(self) -> dict:
+        """ScreenedPotential::Calculate: R-space fields X, V, Eps, InvEps, W [R][n][m]."""
+        out = {k: np.empty((self.Nk, self.nb, self.nb), dtype=np.complex128) for k in ("X", "V", "Eps", "InvEps", "W")}
+        lib().oracle_screened(self.h, *(_ptr(out[k].view(np.float64)) for k in ("X", "V", "Eps", "InvEps", "W")))
+        return out
+
+    def wannier_centres(self) -> np.ndarray:
+        """Diagonal of r(R = 0) per orbital (bohr), [nb][3]."""
+        out = np.empty((self.nb, 3))
+        lib().oracle_rwann(self.h, _ptr(out))
+        return out
+
     def propagate(self, nsteps: int | None = None) -> dict:
         n = self.nsteps if nsteps is None else int(nsteps)
         times, laser = np.empty(n), np.empty((n, 3))
@@ -274,3 +290,16 @@ def fft2d(x: np.ndarray, sign: int) -> np.ndarray:
 
 def struve0(x: float) -> float:
     return lib().oracle_struve0(float(x))
+
+
+def lindhard(U: np.ndarray, eps: np.ndarray, N0: int, N1: int, filled: int) -> np.ndarray:
+    """Lindhard response X(q) [q][a][b] (Screened_potential.cpp:17-58) for
+    eigenvectors U [k][a][n] and energies eps [k][n] on an N0 x N1 grid."""
+    U = np.ascontiguousarray(U, dtype=np.complex128)
+    e = np.ascontiguousarray(eps, dtype=np.float64)
+    nk, nb, _ = U.shape
+    assert nk == N0 * N1 and e.shape == (nk, nb)
+    X = np.empty_like(U)
+    if lib().oracle_lindhard(nb, N0, N1, int(filled), _ptr(U.view(np.float64)), _ptr(e), _ptr(X.view(np.float64))) != 0:
+        raise RuntimeError(lib().oracle_last_error().decode())
+    return X

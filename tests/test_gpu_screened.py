@@ -1,0 +1,159 @@
+"""GPU parity of the screened-potential application (src/Screened_potential/
+Screened_potential.cpp) against the oracle, pinned on the CPU by
+test_oracle_screened.py, and against closed forms at full size.
+
+Bar (floating point, another summation order than the reference's):
+max|X_gpu - X_ref| <= 1e-11 max|X_ref| for the Lindhard response; the R-space
+fields within 1e-11 of the oracle's, relative to their maxima.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import hbn_params
+import screened_models as sm
+
+import edus
+from edus import models
+
+oracle_py = pytest.importorskip("oracle_py")
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-11
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+# (nb, filled, N0, N1): upper-triangle and full pair lists, several q per
+# wave (small nb), two and three waves per q (nb >= 12), odd grids, the
+# extremes of filled
+CASES = [(2, 1, 6, 5), (3, 1, 4, 7), (3, 2, 5, 3), (4, 2, 5, 6), (5, 3, 6, 6), (6, 3, 8, 6), (7, 3, 5, 5),
+         (8, 4, 6, 7), (9, 2, 4, 4), (10, 5, 5, 4), (11, 6, 4, 5), (12, 6, 3, 5), (14, 7, 6, 6), (16, 8, 4, 5),
+         (16, 1, 3, 4), (16, 15, 3, 4)]
+
+
+@pytest.mark.parametrize("nb,nf,N0,N1", CASES)
+def test_lindhard_matches_oracle(nb, nf, N0, N1):
+    _say("case nb", nb, "filled", nf, "grid", N0, N1)
+    U, w = sm.random_bands(nb, N0, N1, seed=100 + nb + nf)
+    ref = oracle_py.lindhard(U, w, N0, N1, nf)
+    got = edus.lindhard(U, w, N0, N1, nf)
+    err, scale = np.abs(got - ref).max(), np.abs(ref).max()
+    _say("  max|dX|", err, "max|X|", scale)
+    assert scale > 0 and err <= TOL * scale
+
+
+def test_lindhard_no_valence_or_conduction_is_zero():
+    U, w = sm.random_bands(4, 3, 3, seed=1)
+    assert np.abs(edus.lindhard(U, w, 3, 3, 0)).max() == 0.0
+    assert np.abs(edus.lindhard(U, w, 3, 3, 4)).max() == 0.0
+
+
+@pytest.mark.parametrize("nblocks,N0,N1", [(1, 30, 30), (4, 64, 64), (7, 96, 90), (7, 120, 120)])
+def test_lindhard_closed_form_full_size(nblocks, N0, N1):
+    """Gauge-phased blocks at full grid sizes, on the launch shapes of the
+    large cases (four q per lane; 14 bands: two waves per q; 120 x 120 with
+    14 bands is the benchmark shape): the hand-derived response."""
+    _say("blocks", nblocks, "grid", N0, N1)
+    U, eps, nf = sm.gauge_block_model(nblocks, N0, N1)
+    got = edus.lindhard(U, eps, N0, N1, nf)
+    ref = sm.closed_form(nblocks, N0, N1)
+    err, scale = np.abs(got - ref).max(), np.abs(ref).max()
+    _say("  max|dX|", err, "max|X|", scale)
+    assert err <= TOL * scale
+
+
+def test_lindhard_hermitian_and_gauge_invariant_full_size():
+    nb, nf, N0, N1 = 14, 7, 48, 48
+    U, w = sm.random_bands(nb, N0, N1, seed=5)
+    X = edus.lindhard(U, w, N0, N1, nf)
+    scale = np.abs(X).max()
+    herm = np.abs(X - X.conj().transpose(0, 2, 1)).max()
+    rng = np.random.default_rng(9)
+    Xg = edus.lindhard(U * np.exp(1j * rng.uniform(0, 2 * np.pi, size=(N0 * N1, 1, nb))), w, N0, N1, nf)
+    gauge = np.abs(Xg - X).max()
+    _say("hermiticity", herm, "gauge", gauge, "scale", scale)
+    assert herm <= 1e-13 * scale and gauge <= TOL * scale
+
+
+def test_lindhard_device_entry_matches_host_entry():
+    import hip_rt
+
+    nb, nf, N0, N1 = 8, 4, 12, 10
+    nk = N0 * N1
+    U, w = sm.random_bands(nb, N0, N1, seed=3)
+    host = edus.lindhard(U, w, N0, N1, nf)
+    dU = hip_rt.DeviceBuffer.from_array(U)
+    de = hip_rt.DeviceBuffer.from_array(np.ascontiguousarray(w))
+    work = hip_rt.DeviceBuffer(U.nbytes)
+    dX = hip_rt.DeviceBuffer(U.nbytes)
+    edus.lindhard_device(nb, N0, N1, nf, dU.address, de.address, work.address, dX.address, 0)
+    hip_rt.synchronize()
+    planes = dX.to_array(np.complex128, (nb * nb, nk))
+    got = planes.T.reshape(nk, nb, nb)
+    assert np.array_equal(got, host)
+
+
+def _compare_fields(got, ref):
+    for k in ("X", "V", "Eps", "InvEps", "W"):
+        err, scale = np.abs(got[k] - ref[k]).max(), max(np.abs(ref[k]).max(), 1e-300)
+        _say("  ", k, "max abs err", err, "max", scale)
+        assert err <= TOL * scale, k
+
+
+def test_hbn_screened_potential_against_oracle():
+    p = hbn_params()
+    o = oracle_py.Oracle(p)
+    sim = edus.Simulation(p)
+    _compare_fields(sim.screened_potential(), o.screened_potential())
+
+
+def test_hsex_model_screened_potential_against_oracle(model_dir):
+    p = models.workload("hsex8_hex90", os.path.join(model_dir, "hsex8_hex90_9x11"), grid=[9, 11, 1])
+    p["coulomb"] = False
+    o = oracle_py.Oracle(p)
+    sim = edus.Simulation(p)
+    _compare_fields(sim.screened_potential(), o.screened_potential())
+
+
+def test_screened_potential_needs_full_layout():
+    sim = edus.Simulation(hbn_params(), layout=edus.LAYOUT_HERMITIAN)
+    with pytest.raises(edus.EdusError):
+        sim.screened_potential()
+
+
+def _parse_rdecay(path):
+    """Rows 'distance(re,im)' of Operator::print_Rdecay."""
+    d, v = [], []
+    with open(path) as f:
+        for line in f:
+            head, rest = line.strip().split("(")
+            re_, im_ = rest.rstrip(")").split(",")
+            d.append(float(head))
+            v.append(complex(float(re_), float(im_)))
+    return np.array(d), np.array(v)
+
+
+def test_hbn_screened_files(tmp_path):
+    p = hbn_params()
+    sim = edus.Simulation(p)
+    f = sim.screened_potential()
+    sim.write_screened(str(tmp_path))
+    dm0 = None
+    for name, key in (("X0.txt", "X"), ("V0.txt", "V"), ("Epsilon0.txt", "Eps"), ("InverseEpsilon0.txt", "InvEps"),
+                      ("W0.txt", "W")):
+        d, v = _parse_rdecay(os.path.join(str(tmp_path), name))
+        ref = f[key].reshape(-1)
+        assert v.shape == ref.shape
+        # six significant digits (the stream's default precision)
+        assert np.all(np.abs(v - ref) <= 1e-5 * np.maximum(np.abs(ref.real), np.abs(ref.imag)) + 1e-300), name
+        if dm0 is None:
+            dm0 = d
+        assert np.array_equal(d, dm0)
+    assert np.abs(f["V"].reshape(-1).real[dm0 > 1e-5] * dm0[dm0 > 1e-5] - 1.0).max() < 1e-5
