@@ -17,11 +17,12 @@
 //   * a split pass stores the valence columns of every U_k as Uv[k][v][a] and
 //     the conduction columns as Uc[k][c][a] (one array each), so the columns a
 //     work-group needs are contiguous;
-//   * one work-group = one wave of 64 lanes; each lane owns one element pair
-//     (a, b) (the upper triangle when that packs the lanes better, the lower
-//     triangle then follows from X_ba(q) = conj X_ab(q)), for G q-groups per
-//     wave when the pairs fit G times in 64 lanes, or one part of a q when
-//     they need W waves;
+//   * one work-group = W waves; each lane owns one element pair (a, b) (the
+//     upper triangle when that packs the lanes better, the lower triangle
+//     then follows from X_ba(q) = conj X_ab(q)), for G q-groups per wave when
+//     the pairs fit G times in 64 lanes (W = 1), or spread over the W waves of
+//     the work-group when they need more than 64 lanes (G = 1); the waves
+//     share the staged columns and reciprocals;
 //   * the work-group's GT = G * TQ q lie on one grid row at consecutive q1,
 //     so for every k their k + q are consecutive on one row (two runs where
 //     the row wraps): per k the wave copies Uv(k) and the Uc(k + q) runs, with
@@ -50,20 +51,23 @@ namespace edus {
 
 namespace {
 
-constexpr int kLindStageR = 4;  // reciprocals formed per lane per k
+constexpr int kLindStageR = 4;  // reciprocals formed per thread per k
+constexpr int kLindGatherU = 8;  // conduction-column elements gathered per thread per k
+constexpr int kLindGatherE = 3;  // conduction-energy words gathered per thread per k
 
 struct LindhardArgs {
     const double2* Uv;  // [Nk][nv][nb]: U_k(a, v) at (k nv + v) nb + a
-    const double2* Uc;  // [Nk][nc][nb]: U_k(a, nv + c) at (k nc + c) nb + a
+    const double2* Uc;  // planes [nc * nb][Nk]: U_k(a, nv + c) at (c nb + a) Nk + k
     const double* eps;  // [Nk][nb]
     double2* X;         // planes [nb * nb][Nk]
     int nb, nv, nc, N0, N1, Nk;
     int Pg;             // element pairs per q
     int G;              // q groups per wave (Pg <= 64), else 1
-    int W;              // waves per q (Pg > 64), else 1
+    int W;              // waves per work-group (Pg > 64), else 1
     int tri;            // pairs are the upper triangle
     int nqb_row;        // q-blocks per grid row
-    int nR;             // reciprocals per k (whole wave)
+    int KS;             // k-range parts (work-groups per q-block); > 1: partial sums added into X
+    int nR;             // reciprocals per k (whole work-group)
     int nvp;            // nv rounded up to even (reciprocal row length)
     int buf;            // one LDS buffer, in double2
     double inv_nk;
@@ -85,44 +89,50 @@ __device__ inline void pair_of(int i, int nb, int tri, int& a, int& b)
     b = a + i;
 }
 
-// U[k][a][n] -> Uv[k][v][a], Uc[k][c][a]
+// U[k][a][n] -> Uv[k][v][a] and the conduction planes Uc[(c nb + a)][k]
 __global__ void lindhard_split_kernel(const double2* __restrict__ U, double2* __restrict__ Uv,
-                                      double2* __restrict__ Uc, int nb, int nv, long total)
+                                      double2* __restrict__ Uc, int nb, int nv, int Nk)
 {
     const long i = long(blockIdx.x) * blockDim.x + threadIdx.x;
+    const long total = long(Nk) * nb * nb;
     if (i >= total) return;
-    const int M = nb * nb, nc = nb - nv;
+    const int M = nb * nb;
     const long k = i / M;
     const int r = int(i - k * M), al = r / nb, n = r % nb;
     const double2 v = U[i];
     if (n < nv)
         Uv[(k * nv + n) * nb + al] = v;
     else
-        Uc[(k * nc + (n - nv)) * nb + al] = v;
+        Uc[long((n - nv) * nb + al) * Nk + k] = v;
 }
 
-// copy count 16-byte elements src[0..count) to LDS dst (wave-uniform base)
-__device__ inline void copy16(const double2* src, double2* dst, int count, int lane)
+// copy count 16-byte elements src[0..count) to LDS dst: wave w of nw takes
+// the 64-element chunks w, w + nw, ... (wave-uniform LDS base per chunk)
+__device__ inline void copy16(const double2* src, double2* dst, int count, int lane, int w, int nw)
 {
-    for (int i = 0; i < count; i += 64)
+    for (int i = 64 * w; i < count; i += 64 * nw)
         if (i + lane < count) __builtin_amdgcn_global_load_lds(src + i + lane, dst + i, 16, 0, 0);
 }
 
-// copy count 4-byte words to LDS
-__device__ inline void copy4(const unsigned* src, unsigned* dst, int count, int lane)
+// copy count 4-byte words to LDS, chunks spread over the waves as above
+__device__ inline void copy4(const unsigned* src, unsigned* dst, int count, int lane, int w, int nw)
 {
-    for (int i = 0; i < count; i += 64)
+    for (int i = 64 * w; i < count; i += 64 * nw)
         if (i + lane < count) __builtin_amdgcn_global_load_lds(src + i + lane, dst + i, 4, 0, 0);
 }
 
 template <int NV, int TQ>
-__global__ void __launch_bounds__(64) lindhard_kernel(LindhardArgs a)
+__global__ void __launch_bounds__(256) lindhard_kernel(LindhardArgs a)
 {
     extern __shared__ double2 lds[];
+    constexpr int NVP = (NV + 1) & ~1;
     const int nb = a.nb, nc = a.nc, N1 = a.N1, GT = a.G * TQ;
-    // buffer layout (double2 units): Uv(k) [NV][nb], Uc runs [GT][nc][nb],
-    // energies (doubles) [1 + GT][nb]; the reciprocals [GT][nc][nvp] after both buffers
-    const int oUc = NV * nb, oE = oUc + GT * nc * nb;
+    // buffer layout (double2 units): Uv(k) [NV][nb]; the conduction columns
+    // of the GT slots, slot fastest: [nc][nb][GT]; then (doubles) the valence
+    // energies of k [NVP] and the conduction energies of the slots [GT][nc].
+    // The reciprocals [nc][GT][NVP] follow both buffers.  With the slot
+    // fastest, the TQ q of a lane sit at fixed offsets from one address.
+    const int oUc = NV * nb, oE = oUc + nc * nb * GT;
     double* sR = reinterpret_cast<double*>(lds + 2 * a.buf);
 
     // XCD-aware order: consecutive logical blocks on one XCD
@@ -132,14 +142,15 @@ __global__ void __launch_bounds__(64) lindhard_kernel(LindhardArgs a)
         const long x = blockIdx.x % 8, i = blockIdx.x / 8;
         L = x * q + min(x, r) + i;
     }
-    const int qbi = int(L / a.W), wpart = int(L % a.W);
+    const int qbi = int(L / a.KS), ks = int(L % a.KS);
+    const int kb = int(long(a.Nk) * ks / a.KS), ke = int(long(a.Nk) * (ks + 1) / a.KS);
     const int q0 = qbi / a.nqb_row, q1f = (qbi % a.nqb_row) * GT;  // row and first q1 of the block
     const int nvalid = min(GT, N1 - q1f);
-    const int lane = threadIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nw = a.W, nthr = 64 * a.W;
     int s, pi;
     if (a.W > 1) {
         s = 0;
-        pi = lane + 64 * wpart;
+        pi = tid;
     } else {
         s = lane / a.Pg;
         pi = lane % a.Pg;
@@ -148,17 +159,42 @@ __global__ void __launch_bounds__(64) lindhard_kernel(LindhardArgs a)
     int al = 0, be = 0;
     if (active) pair_of(pi, nb, a.tri, al, be);
 
-    // reciprocal entries of this lane: energy offsets and destination
+    // per-thread work lists, fixed for the whole k loop:
+    //  reciprocal entries: valence index, conduction-energy index, destination
     int rv[kLindStageR], rc[kLindStageR], rd[kLindStageR];
 #pragma unroll
     for (int t = 0; t < kLindStageR; ++t) {
-        const int e = lane + 64 * t;
+        const int e = tid + nthr * t;
         rv[t] = rc[t] = rd[t] = -1;
         if (e < a.nR) {
-            const int slot = e / (nc * NV), c = (e / NV) % nc, v = e % NV;
+            const int c = e / (GT * NV), slot = (e / NV) % GT, v = e % NV;
             rv[t] = v;
-            rc[t] = (1 + slot) * nb + NV + c;
-            rd[t] = (slot * nc + c) * a.nvp + v;
+            rc[t] = slot * nc + c;
+            rd[t] = (c * GT + slot) * NVP + v;
+        }
+    }
+    //  gathered conduction-column elements: plane (c nb + a) and slot
+    const int nU = nc * nb * GT;
+    int gp[kLindGatherU], gs[kLindGatherU];
+#pragma unroll
+    for (int t = 0; t < kLindGatherU; ++t) {
+        const int e = tid + nthr * t;
+        gp[t] = gs[t] = -1;
+        if (e < nU && e % GT < nvalid) {
+            gp[t] = e / GT;
+            gs[t] = e % GT;
+        }
+    }
+    //  gathered conduction-energy words: slot and word offset in the eps row
+    const int nEw = 2 * GT * nc;
+    int es[kLindGatherE], eo[kLindGatherE];
+#pragma unroll
+    for (int t = 0; t < kLindGatherE; ++t) {
+        const int f = tid + nthr * t;
+        es[t] = eo[t] = -1;
+        if (f < nEw && f / (2 * nc) < nvalid) {
+            es[t] = f / (2 * nc);
+            eo[t] = 2 * (NV + (f / 2) % nc) + (f & 1);
         }
     }
 
@@ -170,54 +206,88 @@ __global__ void __launch_bounds__(64) lindhard_kernel(LindhardArgs a)
         if (r0 >= a.N0) r0 -= a.N0;
         int s1 = k1 + q1f;
         if (s1 >= N1) s1 -= N1;
-        const int n1 = min(nvalid, N1 - s1), n2 = nvalid - n1;
-        const long row = long(r0) * N1;
-        copy16(a.Uv + long(k) * NV * nb, buf, NV * nb, lane);
-        copy16(a.Uc + (row + s1) * nc * nb, buf + oUc, n1 * nc * nb, lane);
-        if (n2 > 0) copy16(a.Uc + row * nc * nb, buf + oUc + n1 * nc * nb, n2 * nc * nb, lane);
-        unsigned* eb = reinterpret_cast<unsigned*>(buf + oE);
+        const int row = r0 * N1;
+        copy16(a.Uv + long(k) * NV * nb, buf, NV * nb, lane, wave, nw);
         const unsigned* ep = reinterpret_cast<const unsigned*>(a.eps);
-        copy4(ep + 2 * long(k) * nb, eb, 2 * nb, lane);
-        copy4(ep + 2 * (row + s1) * nb, eb + 2 * nb, 2 * n1 * nb, lane);
-        if (n2 > 0) copy4(ep + 2 * row * nb, eb + 2 * nb * (1 + n1), 2 * n2 * nb, lane);
+        unsigned* eb = reinterpret_cast<unsigned*>(buf + oE);
+        copy4(ep + 2 * long(k) * nb, eb, 2 * NV, lane, wave, nw);
+        // element e = tid + nthr t lands at LDS slot oUc + e: wave-uniform base
+        // oUc + nthr t + 64 wave, lane offset 16 bytes
+#pragma unroll
+        for (int t = 0; t < kLindGatherU; ++t) {
+            if (nthr * t >= nU) break;
+            if (gp[t] >= 0) {
+                int kk = s1 + gs[t];
+                if (kk >= N1) kk -= N1;
+                __builtin_amdgcn_global_load_lds(a.Uc + long(gp[t]) * a.Nk + row + kk,
+                                                 buf + oUc + nthr * t + 64 * wave, 16, 0, 0);
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < kLindGatherE; ++t) {
+            if (nthr * t >= nEw) break;
+            if (es[t] >= 0) {
+                int kk = s1 + es[t];
+                if (kk >= N1) kk -= N1;
+                __builtin_amdgcn_global_load_lds(ep + 2 * long(row + kk) * nb + eo[t],
+                                                 eb + 2 * NVP + nthr * t + 64 * wave, 4, 0, 0);
+            }
+        }
     };
 
     double2 acc[TQ];
 #pragma unroll
     for (int j = 0; j < TQ; ++j) acc[j] = make_double2(0.0, 0.0);
 
-    issue(0, 0);
-    for (int k = 0; k < a.Nk; ++k) {
-        const double2* buf = lds + (k & 1) * a.buf;
+    issue(kb, 0);
+    for (int k = kb; k < ke; ++k) {
+        const double2* buf = lds + ((k - kb) & 1) * a.buf;
         __builtin_amdgcn_s_waitcnt(0);  // this wave's copies of step k are in LDS
         __syncthreads();                // and every lane is done with step k - 1
-        if (k + 1 < a.Nk) issue(k + 1, (k + 1) & 1);
-        const double* eb = reinterpret_cast<const double*>(buf + oE);
+        if (k + 1 < ke) issue(k + 1, (k + 1 - kb) & 1);
+        const double* ev = reinterpret_cast<const double*>(buf + oE);
+        const double* ec = ev + NVP;
 #pragma unroll
         for (int t = 0; t < kLindStageR; ++t)
-            if (rd[t] >= 0) sR[rd[t]] = 1.0 / (eb[rv[t]] - eb[rc[t]]);
+            if (rd[t] >= 0) sR[rd[t]] = 1.0 / (ev[rv[t]] - ec[rc[t]]);
         __syncthreads();
         if (active) {
             double2 A[NV];
 #pragma unroll
             for (int v = 0; v < NV; ++v) A[v] = cmul(buf[v * nb + al], cconj(buf[v * nb + be]));
+            // c outer, the TQ q inner: TQ independent chains per iteration;
+            // the TQ slots of a lane are adjacent in the Uc tile and the
+            // reciprocals, so their reads share one address
+            const double2* ua = buf + oUc + al * GT + s * TQ;
+            const double2* ub = buf + oUc + be * GT + s * TQ;
+            const double* rr = sR + s * TQ * NVP;
+            for (int c = 0; c < nc; ++c) {
+                double2 B[TQ], S[TQ];
 #pragma unroll
-            for (int j = 0; j < TQ; ++j) {
-                const int slot = s * TQ + j;
-                const double2* ucs = buf + oUc + slot * nc * nb;
-                const double* rs = sR + slot * nc * a.nvp;
-                for (int c = 0; c < nc; ++c) {
-                    const double2* uc = ucs + c * nb;
-                    const double2 B = cmul(cconj(uc[al]), uc[be]);
-                    const double* r = rs + c * a.nvp;
-                    double2 S = make_double2(0.0, 0.0);
+                for (int j = 0; j < TQ; ++j) B[j] = cmul(cconj(ua[j]), ub[j]);
 #pragma unroll
-                    for (int v = 0; v < NV; ++v) {
-                        S.x += r[v] * A[v].x;
-                        S.y += r[v] * A[v].y;
+                for (int j = 0; j < TQ; ++j) {
+                    const double2* r2 = reinterpret_cast<const double2*>(rr + j * NVP);  // 16-byte aligned
+                    S[j] = make_double2(0.0, 0.0);
+#pragma unroll
+                    for (int v = 0; v < NV; v += 2) {
+                        const double2 r = r2[v / 2];
+                        S[j].x = fma(r.x, A[v].x, S[j].x);
+                        S[j].y = fma(r.x, A[v].y, S[j].y);
+                        if (v + 1 < NV) {
+                            S[j].x = fma(r.y, A[v + 1].x, S[j].x);
+                            S[j].y = fma(r.y, A[v + 1].y, S[j].y);
+                        }
                     }
-                    acc[j] = cadd(acc[j], cmul(S, B));
                 }
+#pragma unroll
+                for (int j = 0; j < TQ; ++j) {
+                    acc[j].x = fma(S[j].x, B[j].x, fma(-S[j].y, B[j].y, acc[j].x));
+                    acc[j].y = fma(S[j].x, B[j].y, fma(S[j].y, B[j].x, acc[j].y));
+                }
+                ua += nb * GT;
+                ub += nb * GT;
+                rr += GT * NVP;
             }
         }
     }
@@ -229,15 +299,43 @@ __global__ void __launch_bounds__(64) lindhard_kernel(LindhardArgs a)
         if (q1 >= N1) continue;
         const int q = q0 * N1 + q1;
         const double2 x = cscale(acc[j], a.inv_nk);
-        a.X[long(al * nb + be) * a.Nk + q] = x;
-        if (a.tri && al != be) a.X[long(be * nb + al) * a.Nk + q] = cconj(x);
+        double2* x1 = a.X + long(al * nb + be) * a.Nk + q;
+        double2* x2 = a.X + long(be * nb + al) * a.Nk + q;
+        if (a.KS == 1) {
+            *x1 = x;
+            if (a.tri && al != be) *x2 = cconj(x);
+        } else {
+            // two parts onto a zeroed X: 0 + p1 + p2 is the same sum in
+            // either order, so the result does not depend on timing
+            atomicAdd(&x1->x, x.x);
+            atomicAdd(&x1->y, x.y);
+            if (a.tri && al != be) {
+                atomicAdd(&x2->x, x.x);
+                atomicAdd(&x2->y, -x.y);
+            }
+        }
     }
 }
 
+// Work-groups of one launch run in rounds of (resident work-groups per CU) x
+// CUs; when the q-blocks leave the last round mostly empty, each q-block's k
+// range is split in two (partial sums added into a zeroed X) if that fills
+// the rounds better.
 template <int NV, int TQ>
-void launch_nv(const LindhardArgs& a, int blocks, size_t lds, hipStream_t s)
+void launch_nv(const LindhardArgs& a0, int blocks, size_t lds, hipStream_t s)
 {
-    hipLaunchKernelGGL((lindhard_kernel<NV, TQ>), dim3(blocks), dim3(64), lds, s, a);
+    LindhardArgs a = a0;
+    auto kern = &lindhard_kernel<NV, TQ>;
+    int per_cu = 0, dev = 0, ncu = 0;
+    EDUS_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(kern),
+                                                                64 * a.W, lds));
+    EDUS_HIP_CHECK(hipGetDevice(&dev));
+    EDUS_HIP_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    const long cap = long(std::max(per_cu, 1)) * std::max(ncu, 1);
+    auto fill = [&](long n) { return double(n) / double(((n + cap - 1) / cap) * cap); };
+    a.KS = (a.Nk >= 64 && fill(2L * blocks) > fill(blocks) + 0.05) ? 2 : 1;
+    if (a.KS > 1) EDUS_HIP_CHECK(hipMemsetAsync(a.X, 0, size_t(a.nb) * a.nb * a.Nk * sizeof(double2), s));
+    hipLaunchKernelGGL(kern, dim3(blocks * a.KS), dim3(64 * a.W), lds, s, a);
 }
 
 template <int TQ>
@@ -280,15 +378,18 @@ LindhardPlan lindhard_plan(int nb, int nf, int N0, int N1)
     auto row_blocks = [&](int tq) { return (N1 + p.G * tq - 1) / (p.G * tq); };
     const double fill4 = double(N1) / double(row_blocks(4) * p.G * 4);
     p.TQ = (long(N0) * row_blocks(4) * p.W >= 1024 && fill4 >= 0.9) ? 4 : 1;
+    if (p.W > 4) throw EngineError(EDUS_ERR_UNSUPPORTED, "lindhard: more than 256 element pairs");
     const int GT = p.G * p.TQ;
     p.nqb_row = row_blocks(p.TQ);
     p.nR = GT * p.nc * p.nv;
     p.nvp = (p.nv + 1) & ~1;
-    p.buf = p.nv * nb + GT * p.nc * nb + ((1 + GT) * nb + 1) / 2;
-    p.blocks = N0 * p.nqb_row * p.W;
+    p.buf = p.nv * nb + p.nc * nb * GT + (p.nvp + GT * p.nc + 1) / 2;
+    p.blocks = N0 * p.nqb_row;
     p.lds = size_t(2 * p.buf) * sizeof(double2) + size_t(GT) * p.nc * p.nvp * sizeof(double);
-    if (p.nv > 0 && p.nc > 0 && p.nR > 64 * kLindStageR)
-        throw EngineError(EDUS_ERR_UNSUPPORTED, "lindhard: more reciprocals per step than the lanes hold");
+    const int nthr = 64 * p.W;
+    if (p.nv > 0 && p.nc > 0 &&
+        (p.nR > nthr * kLindStageR || p.nc * nb * GT > nthr * kLindGatherU || 2 * GT * p.nc > nthr * kLindGatherE))
+        throw EngineError(EDUS_ERR_UNSUPPORTED, "lindhard: a step needs more per-thread staging than the kernel holds");
     return p;
 }
 
@@ -306,7 +407,7 @@ void launch_lindhard(const LindhardPlan& p, const double2* U, const double* eps,
     double2* Uc = work + size_t(p.Nk) * p.nv * p.nb;
     const long total = long(p.Nk) * p.nb * p.nb;
     hipLaunchKernelGGL(lindhard_split_kernel, dim3(unsigned((total + 255) / 256)), dim3(256), 0, s, U, Uv, Uc, p.nb,
-                       p.nv, total);
+                       p.nv, p.Nk);
     LindhardArgs a{};
     a.Uv = Uv;
     a.Uc = Uc;
