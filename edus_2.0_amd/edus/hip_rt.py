@@ -1,5 +1,6 @@
-"""Minimal ctypes access to the HIP runtime the engine library links against
-(device buffers and events for tests of the device-pointer entry points)."""
+"""Minimal ctypes access to the HIP runtime the engine library links against:
+device buffers and events for the device-pointer entry points (tests and the
+benchmark)."""
 import ctypes as C
 
 import numpy as np
@@ -10,9 +11,9 @@ _RT = None
 def rt():
     global _RT
     if _RT is None:
-        import edus
+        from .engine import lib
 
-        edus.lib()  # loads libamdhip64 through the engine library first
+        lib()  # loads libamdhip64 through the engine library first
         L = C.CDLL("libamdhip64.so.7")
         L.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
         L.hipFree.argtypes = [C.c_void_p]

@@ -83,7 +83,7 @@ def test_lindhard_hermitian_and_gauge_invariant_full_size():
 
 
 def test_lindhard_device_entry_matches_host_entry():
-    import hip_rt
+    from edus import hip_rt
 
     nb, nf, N0, N1 = 8, 4, 12, 10
     nk = N0 * N1
