@@ -173,7 +173,7 @@ __global__ void __launch_bounds__(256) lindhard_kernel(LindhardArgs a)
             rd[t] = (c * GT + slot) * NVP + v;
         }
     }
-    //  gathered conduction-column elements: plane (c nb + a) and slot
+    //  gathered conduction-column elements: offset of plane (c nb + a) and slot
     const int nU = nc * nb * GT;
     int gp[kLindGatherU], gs[kLindGatherU];
 #pragma unroll
@@ -181,7 +181,7 @@ __global__ void __launch_bounds__(256) lindhard_kernel(LindhardArgs a)
         const int e = tid + nthr * t;
         gp[t] = gs[t] = -1;
         if (e < nU && e % GT < nvalid) {
-            gp[t] = e / GT;
+            gp[t] = (e / GT) * a.Nk;  // < nb^2 Nk, fits an int for any grid the engine takes
             gs[t] = e % GT;
         }
     }
@@ -199,9 +199,8 @@ __global__ void __launch_bounds__(256) lindhard_kernel(LindhardArgs a)
     }
 
     // issue the direct copies of step k into buffer b
-    auto issue = [&](int k, int b) {
+    auto issue = [&](int k, int k0, int k1, int b) {
         double2* buf = lds + b * a.buf;
-        const int k0 = k / N1, k1 = k - k0 * N1;
         int r0 = k0 + q0;
         if (r0 >= a.N0) r0 -= a.N0;
         int s1 = k1 + q1f;
@@ -219,7 +218,7 @@ __global__ void __launch_bounds__(256) lindhard_kernel(LindhardArgs a)
             if (gp[t] >= 0) {
                 int kk = s1 + gs[t];
                 if (kk >= N1) kk -= N1;
-                __builtin_amdgcn_global_load_lds(a.Uc + long(gp[t]) * a.Nk + row + kk,
+                __builtin_amdgcn_global_load_lds(a.Uc + (gp[t] + row + kk),
                                                  buf + oUc + nthr * t + 64 * wave, 16, 0, 0);
             }
         }
@@ -239,12 +238,24 @@ __global__ void __launch_bounds__(256) lindhard_kernel(LindhardArgs a)
 #pragma unroll
     for (int j = 0; j < TQ; ++j) acc[j] = make_double2(0.0, 0.0);
 
-    issue(kb, 0);
+    // grid coordinates of the next step to issue, advanced without divisions
+    int n0 = kb / N1, n1 = kb - (kb / N1) * N1;
+    auto advance = [&]() {
+        if (++n1 == N1) {
+            n1 = 0;
+            ++n0;
+        }
+    };
+    issue(kb, n0, n1, 0);
+    advance();
     for (int k = kb; k < ke; ++k) {
         const double2* buf = lds + ((k - kb) & 1) * a.buf;
         __builtin_amdgcn_s_waitcnt(0);  // this wave's copies of step k are in LDS
         __syncthreads();                // and every lane is done with step k - 1
-        if (k + 1 < ke) issue(k + 1, (k + 1 - kb) & 1);
+        if (k + 1 < ke) {
+            issue(k + 1, n0, n1, (k + 1 - kb) & 1);
+            advance();
+        }
         const double* ev = reinterpret_cast<const double*>(buf + oE);
         const double* ec = ev + NVP;
 #pragma unroll
@@ -361,6 +372,8 @@ LindhardPlan lindhard_plan(int nb, int nf, int N0, int N1)
     if (nb < 2 || nb > kMaxNb) throw EngineError(EDUS_ERR_ARG, "lindhard: nb must be in [2, 16]");
     if (nf < 0 || nf > nb) throw EngineError(EDUS_ERR_ARG, "lindhard: filledbands out of range");
     if (N0 < 1 || N1 < 1) throw EngineError(EDUS_ERR_ARG, "lindhard: grid sizes must be positive");
+    if (long(nb) * nb * N0 * N1 >= (1L << 31))
+        throw EngineError(EDUS_ERR_UNSUPPORTED, "lindhard: nb^2 * Nk must stay below 2^31");
     LindhardPlan p{};
     p.nb = nb;
     p.nv = nf;
@@ -373,12 +386,22 @@ LindhardPlan lindhard_plan(int nb, int nf, int N0, int N1)
     p.Pg = p.tri ? P : M;
     p.G = p.Pg <= 64 ? 64 / p.Pg : 1;
     p.W = p.Pg <= 64 ? 1 : (p.Pg + 63) / 64;
-    // four q per lane when that keeps >= 4 work-groups per CU and wastes
-    // under 10% of the slots at the row ends, else one
-    auto row_blocks = [&](int tq) { return (N1 + p.G * tq - 1) / (p.G * tq); };
-    const double fill4 = double(N1) / double(row_blocks(4) * p.G * 4);
-    p.TQ = (long(N0) * row_blocks(4) * p.W >= 1024 && fill4 >= 0.9) ? 4 : 1;
+    // four q per lane when that keeps >= 1024 waves, wastes under 10% of the
+    // slots at the row ends and the per-thread staging lists hold a step;
+    // else one (eight measured slower at nb = 14: 220 VGPRs, 2 waves per SIMD)
     if (p.W > 4) throw EngineError(EDUS_ERR_UNSUPPORTED, "lindhard: more than 256 element pairs");
+    auto row_blocks = [&](int tq) { return (N1 + p.G * tq - 1) / (p.G * tq); };
+    const int nthr = 64 * p.W;
+    auto fits = [&](int tq) {
+        const int gt = p.G * tq;
+        return gt * p.nc * p.nv <= nthr * kLindStageR && p.nc * nb * gt <= nthr * kLindGatherU &&
+               2 * gt * p.nc <= nthr * kLindGatherE;
+    };
+    auto good = [&](int tq) {
+        const double fill = double(N1) / double(row_blocks(tq) * p.G * tq);
+        return long(N0) * row_blocks(tq) * p.W >= 1024 && fill >= 0.9 && fits(tq);
+    };
+    p.TQ = good(4) ? 4 : 1;
     const int GT = p.G * p.TQ;
     p.nqb_row = row_blocks(p.TQ);
     p.nR = GT * p.nc * p.nv;
@@ -386,9 +409,7 @@ LindhardPlan lindhard_plan(int nb, int nf, int N0, int N1)
     p.buf = p.nv * nb + p.nc * nb * GT + (p.nvp + GT * p.nc + 1) / 2;
     p.blocks = N0 * p.nqb_row;
     p.lds = size_t(2 * p.buf) * sizeof(double2) + size_t(GT) * p.nc * p.nvp * sizeof(double);
-    const int nthr = 64 * p.W;
-    if (p.nv > 0 && p.nc > 0 &&
-        (p.nR > nthr * kLindStageR || p.nc * nb * GT > nthr * kLindGatherU || 2 * GT * p.nc > nthr * kLindGatherE))
+    if (p.nv > 0 && p.nc > 0 && !fits(p.TQ))
         throw EngineError(EDUS_ERR_UNSUPPORTED, "lindhard: a step needs more per-thread staging than the kernel holds");
     return p;
 }
