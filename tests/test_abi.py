@@ -43,3 +43,21 @@ def test_config_defaults_and_unit_quirk():
     assert config.compose({"gradient_space": "k"})["gradient_space"] == "k"
     with pytest.raises(ValueError):
         config.compose({"gradient_space": "x"})
+
+
+def test_lindhard_rejects_bad_shapes_before_any_device_call():
+    """edus_lindhard validates its arguments on the host first (nb in [2, 16],
+    0 <= filledbands <= nb, matching array shapes); no GPU is touched."""
+    import numpy as np
+
+    U = np.zeros((6, 2, 2), dtype=np.complex128)
+    e = np.zeros((6, 2))
+    with pytest.raises(ValueError):
+        edus.lindhard(U, e, 4, 2, 1)  # 6 k-points for a 4 x 2 grid
+    with pytest.raises(edus.EdusError):
+        edus.lindhard(np.zeros((6, 1, 1), dtype=np.complex128), np.zeros((6, 1)), 3, 2, 0)  # nb = 1
+    with pytest.raises(edus.EdusError):
+        edus.lindhard(U, e, 3, 2, 3)  # filledbands > nb
+    big = np.zeros((2, 17, 17), dtype=np.complex128)
+    with pytest.raises(edus.EdusError):
+        edus.lindhard(big, np.zeros((2, 17)), 2, 1, 8)  # nb = 17
