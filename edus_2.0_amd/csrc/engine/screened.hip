@@ -24,14 +24,16 @@
 //     the work-group when they need more than 64 lanes (G = 1); the waves
 //     share the staged columns and reciprocals;
 //   * the work-group's GT = G * TQ q lie on one grid row at consecutive q1,
-//     so for every k their k + q are consecutive on one row (two runs where
-//     the row wraps): per k the wave copies Uv(k) and the Uc(k + q) runs, with
-//     the band energies, straight from memory into LDS (global_load_lds, no
-//     registers), double-buffered: step k + 1 is in flight while step k is
-//     computed;
-//   * the reciprocals r_vc of each (q, k) are formed once per wave and read by
-//     all lanes of that q from LDS (one address for all: broadcast); A_v(k) is
-//     built once per k in registers and used for all TQ q of a lane;
+//     so for every k their k + q are consecutive on one row: per k the
+//     work-group copies Uv(k) and the conduction columns of the k + q
+//     straight from memory into LDS (global_load_lds, no registers),
+//     double-buffered: step k + 1 is in flight while step k is computed;
+//   * the reciprocals r_vc of each (q, k) are formed once per work-group,
+//     from energies loaded into registers one step ahead, into one of two
+//     tables, and read by all lanes of that q (one address for all:
+//     broadcast); one barrier per k publishes both the columns and the
+//     table; A_v(k) is built once per k in registers and used for all TQ q
+//     of a lane;
 //   * work-groups of neighbouring q go to the same XCD: their Uc(k + q)
 //     windows overlap and stay in that XCD's L2.
 // The summation order differs from the reference's (k outermost over pairs
@@ -53,7 +55,6 @@ namespace {
 
 constexpr int kLindStageR = 4;  // reciprocals formed per thread per k
 constexpr int kLindGatherU = 8;  // conduction-column elements gathered per thread per k
-constexpr int kLindGatherE = 3;  // conduction-energy words gathered per thread per k
 
 struct LindhardArgs {
     const double2* Uv;  // [Nk][nv][nb]: U_k(a, v) at (k nv + v) nb + a
@@ -114,13 +115,6 @@ __device__ inline void copy16(const double2* src, double2* dst, int count, int l
         if (i + lane < count) __builtin_amdgcn_global_load_lds(src + i + lane, dst + i, 16, 0, 0);
 }
 
-// copy count 4-byte words to LDS, chunks spread over the waves as above
-__device__ inline void copy4(const unsigned* src, unsigned* dst, int count, int lane, int w, int nw)
-{
-    for (int i = 64 * w; i < count; i += 64 * nw)
-        if (i + lane < count) __builtin_amdgcn_global_load_lds(src + i + lane, dst + i, 4, 0, 0);
-}
-
 template <int NV, int TQ>
 __global__ void __launch_bounds__(256) lindhard_kernel(LindhardArgs a)
 {
@@ -128,12 +122,13 @@ __global__ void __launch_bounds__(256) lindhard_kernel(LindhardArgs a)
     constexpr int NVP = (NV + 1) & ~1;
     const int nb = a.nb, nc = a.nc, N1 = a.N1, GT = a.G * TQ;
     // buffer layout (double2 units): Uv(k) [NV][nb]; the conduction columns
-    // of the GT slots, slot fastest: [nc][nb][GT]; then (doubles) the valence
-    // energies of k [NVP] and the conduction energies of the slots [GT][nc].
-    // The reciprocals [nc][GT][NVP] follow both buffers.  With the slot
-    // fastest, the TQ q of a lane sit at fixed offsets from one address.
-    const int oUc = NV * nb, oE = oUc + nc * nb * GT;
-    double* sR = reinterpret_cast<double*>(lds + 2 * a.buf);
+    // of the GT slots, slot fastest: [nc][nb][GT].  With the slot fastest,
+    // the TQ q of a lane sit at fixed offsets from one address.  After both
+    // buffers: two reciprocal tables [nc][GT][NVP] (doubles), one per step
+    // parity.
+    const int oUc = NV * nb;
+    const int nRt = nc * GT * NVP;
+    double* sR0 = reinterpret_cast<double*>(lds + 2 * a.buf);
 
     // XCD-aware order: consecutive logical blocks on one XCD
     long L;
@@ -160,16 +155,18 @@ __global__ void __launch_bounds__(256) lindhard_kernel(LindhardArgs a)
     if (active) pair_of(pi, nb, a.tri, al, be);
 
     // per-thread work lists, fixed for the whole k loop:
-    //  reciprocal entries: valence index, conduction-energy index, destination
-    int rv[kLindStageR], rc[kLindStageR], rd[kLindStageR];
+    //  reciprocal entries: valence band, slot, conduction band (as a column
+    //  of the energy row) and destination in the table
+    int rv[kLindStageR], rs[kLindStageR], rc[kLindStageR], rd[kLindStageR];
 #pragma unroll
     for (int t = 0; t < kLindStageR; ++t) {
         const int e = tid + nthr * t;
-        rv[t] = rc[t] = rd[t] = -1;
-        if (e < a.nR) {
+        rv[t] = rs[t] = rc[t] = rd[t] = -1;
+        if (e < a.nR && (e / NV) % GT < nvalid) {
             const int c = e / (GT * NV), slot = (e / NV) % GT, v = e % NV;
             rv[t] = v;
-            rc[t] = slot * nc + c;
+            rs[t] = slot;
+            rc[t] = NV + c;
             rd[t] = (c * GT + slot) * NVP + v;
         }
     }
@@ -185,31 +182,21 @@ __global__ void __launch_bounds__(256) lindhard_kernel(LindhardArgs a)
             gs[t] = e % GT;
         }
     }
-    //  gathered conduction-energy words: slot and word offset in the eps row
-    const int nEw = 2 * GT * nc;
-    int es[kLindGatherE], eo[kLindGatherE];
-#pragma unroll
-    for (int t = 0; t < kLindGatherE; ++t) {
-        const int f = tid + nthr * t;
-        es[t] = eo[t] = -1;
-        if (f < nEw && f / (2 * nc) < nvalid) {
-            es[t] = f / (2 * nc);
-            eo[t] = 2 * (NV + (f / 2) % nc) + (f & 1);
-        }
-    }
 
+    // first grid point of row r0 + shift s1 of step (k0, k1)
+    auto shift = [&](int k0, int k1, int& row, int& s1) {
+        int r0 = k0 + q0;
+        if (r0 >= a.N0) r0 -= a.N0;
+        s1 = k1 + q1f;
+        if (s1 >= N1) s1 -= N1;
+        row = r0 * N1;
+    };
     // issue the direct copies of step k into buffer b
     auto issue = [&](int k, int k0, int k1, int b) {
         double2* buf = lds + b * a.buf;
-        int r0 = k0 + q0;
-        if (r0 >= a.N0) r0 -= a.N0;
-        int s1 = k1 + q1f;
-        if (s1 >= N1) s1 -= N1;
-        const int row = r0 * N1;
+        int row, s1;
+        shift(k0, k1, row, s1);
         copy16(a.Uv + long(k) * NV * nb, buf, NV * nb, lane, wave, nw);
-        const unsigned* ep = reinterpret_cast<const unsigned*>(a.eps);
-        unsigned* eb = reinterpret_cast<unsigned*>(buf + oE);
-        copy4(ep + 2 * long(k) * nb, eb, 2 * NV, lane, wave, nw);
         // element e = tid + nthr t lands at LDS slot oUc + e: wave-uniform base
         // oUc + nthr t + 64 wave, lane offset 16 bytes
 #pragma unroll
@@ -222,16 +209,26 @@ __global__ void __launch_bounds__(256) lindhard_kernel(LindhardArgs a)
                                                  buf + oUc + nthr * t + 64 * wave, 16, 0, 0);
             }
         }
+    };
+    // the energies of this thread's reciprocal entries for step k, into registers
+    double ev[kLindStageR], ec[kLindStageR];
+    auto load_energies = [&](int k, int k0, int k1) {
+        int row, s1;
+        shift(k0, k1, row, s1);
 #pragma unroll
-        for (int t = 0; t < kLindGatherE; ++t) {
-            if (nthr * t >= nEw) break;
-            if (es[t] >= 0) {
-                int kk = s1 + es[t];
+        for (int t = 0; t < kLindStageR; ++t) {
+            if (rd[t] >= 0) {
+                int kk = s1 + rs[t];
                 if (kk >= N1) kk -= N1;
-                __builtin_amdgcn_global_load_lds(ep + 2 * long(row + kk) * nb + eo[t],
-                                                 eb + 2 * NVP + nthr * t + 64 * wave, 4, 0, 0);
+                ev[t] = a.eps[long(k) * nb + rv[t]];
+                ec[t] = a.eps[long(row + kk) * nb + rc[t]];
             }
         }
+    };
+    auto store_reciprocals = [&](double* sR) {
+#pragma unroll
+        for (int t = 0; t < kLindStageR; ++t)
+            if (rd[t] >= 0) sR[rd[t]] = 1.0 / (ev[t] - ec[t]);
     };
 
     double2 acc[TQ];
@@ -247,21 +244,21 @@ __global__ void __launch_bounds__(256) lindhard_kernel(LindhardArgs a)
         }
     };
     issue(kb, n0, n1, 0);
+    load_energies(kb, n0, n1);
+    store_reciprocals(sR0);
     advance();
     for (int k = kb; k < ke; ++k) {
-        const double2* buf = lds + ((k - kb) & 1) * a.buf;
+        const int par = (k - kb) & 1;
+        const double2* buf = lds + par * a.buf;
+        const double* sR = sR0 + par * nRt;
         __builtin_amdgcn_s_waitcnt(0);  // this wave's copies of step k are in LDS
         __syncthreads();                // and every lane is done with step k - 1
-        if (k + 1 < ke) {
-            issue(k + 1, n0, n1, (k + 1 - kb) & 1);
+        const bool next = k + 1 < ke;
+        if (next) {
+            issue(k + 1, n0, n1, par ^ 1);
+            load_energies(k + 1, n0, n1);
             advance();
         }
-        const double* ev = reinterpret_cast<const double*>(buf + oE);
-        const double* ec = ev + NVP;
-#pragma unroll
-        for (int t = 0; t < kLindStageR; ++t)
-            if (rd[t] >= 0) sR[rd[t]] = 1.0 / (ev[rv[t]] - ec[rc[t]]);
-        __syncthreads();
         if (active) {
             double2 A[NV];
 #pragma unroll
@@ -301,6 +298,9 @@ __global__ void __launch_bounds__(256) lindhard_kernel(LindhardArgs a)
                 rr += GT * NVP;
             }
         }
+        // reciprocals of step k + 1 into the other table: its readers finished
+        // step k - 1 before this step's barrier; the next barrier publishes it
+        if (next) store_reciprocals(sR0 + (par ^ 1) * nRt);
     }
 
     if (!active) return;
@@ -394,8 +394,7 @@ LindhardPlan lindhard_plan(int nb, int nf, int N0, int N1)
     const int nthr = 64 * p.W;
     auto fits = [&](int tq) {
         const int gt = p.G * tq;
-        return gt * p.nc * p.nv <= nthr * kLindStageR && p.nc * nb * gt <= nthr * kLindGatherU &&
-               2 * gt * p.nc <= nthr * kLindGatherE;
+        return gt * p.nc * p.nv <= nthr * kLindStageR && p.nc * nb * gt <= nthr * kLindGatherU;
     };
     auto good = [&](int tq) {
         const double fill = double(N1) / double(row_blocks(tq) * p.G * tq);
@@ -406,9 +405,9 @@ LindhardPlan lindhard_plan(int nb, int nf, int N0, int N1)
     p.nqb_row = row_blocks(p.TQ);
     p.nR = GT * p.nc * p.nv;
     p.nvp = (p.nv + 1) & ~1;
-    p.buf = p.nv * nb + p.nc * nb * GT + (p.nvp + GT * p.nc + 1) / 2;
+    p.buf = p.nv * nb + p.nc * nb * GT;
     p.blocks = N0 * p.nqb_row;
-    p.lds = size_t(2 * p.buf) * sizeof(double2) + size_t(GT) * p.nc * p.nvp * sizeof(double);
+    p.lds = size_t(2 * p.buf) * sizeof(double2) + size_t(2 * GT) * p.nc * p.nvp * sizeof(double);
     if (p.nv > 0 && p.nc > 0 && !fits(p.TQ))
         throw EngineError(EDUS_ERR_UNSUPPORTED, "lindhard: a step needs more per-thread staging than the kernel holds");
     return p;
