@@ -223,9 +223,12 @@ def run_batch(args):
     """Heterogeneous batch: 64 pulse runs over nb in {2, 6, 14} and grids
     48^2, 96^2, 120^2 (one seeded model per class), peak fields log-spaced
     over 1e-3 .. 3e-2 a.u., CEP uniform in [0, 2 pi), photon energy uniform in
-    [0.8, 2.0] eV.  The same steps run once member by member (direct launches,
-    each context alone) and once as one batch (stacked groups, one hipGraph
-    per graph_steps steps); throughput = sum over members of Nk * steps / s."""
+    [0.8, 2.0] eV.  The same number of steps runs four ways: sequentially
+    (one member after another, each with graph replay of graph_steps steps),
+    as 64 independent contexts launched back to back (each on its own stream,
+    so they overlap) with graph replay and with direct launches, and as one
+    batch (stacked groups, one hipGraph per graph_steps steps); throughput =
+    sum over members of Nk * steps / s."""
     import numpy as np
 
     import edus
@@ -250,26 +253,53 @@ def run_batch(args):
         nks.append(g * g)
     dt, K, W = 0.05, args.steps, args.warmup
     t_start = 200.0  # inside the pulses
-    E = np.stack([stage_table(pl, t_start, dt, 2 * W + 2 * K) for pl in pulses], axis=2)  # [s][j][i][3]
-    # member by member
+    E = np.stack([stage_table(pl, t_start, dt, 4 * W + 4 * K) for pl in pulses], axis=2)  # [s][j][i][3]
+    for m in members:
+        m.graph(args.graph)
+    # sequential: one member at a time
     for i, m in enumerate(members):
         m.rk4_steps(dt, E[:W, :, i, :])
         m.synchronize()
     t0 = time.perf_counter()
     for i, m in enumerate(members):
         m.rk4_steps(dt, E[W:W + K, :, i, :])
-    for m in members:
         m.synchronize()
     t_seq = time.perf_counter() - t0
-    _progress("member by member", t_seq, "s")
+    _progress("sequential", t_seq, "s")
+    # independent contexts launched back to back (they overlap on their streams)
+    for i, m in enumerate(members):
+        m.rk4_steps(dt, E[W + K:2 * W + K, :, i, :])
+    for m in members:
+        m.synchronize()
+    t0 = time.perf_counter()
+    for i, m in enumerate(members):
+        m.rk4_steps(dt, E[2 * W + K:2 * W + 2 * K, :, i, :])
+    for m in members:
+        m.synchronize()
+    t_conc = time.perf_counter() - t0
+    _progress("concurrent contexts, graph replay", t_conc, "s")
+    # the same with direct launches
+    for m in members:
+        m.graph(0)
+    for i, m in enumerate(members):
+        m.rk4_steps(dt, E[2 * W + 2 * K:3 * W + 2 * K, :, i, :])
+    for m in members:
+        m.synchronize()
+    t0 = time.perf_counter()
+    for i, m in enumerate(members):
+        m.rk4_steps(dt, E[3 * W + 2 * K:3 * W + 3 * K, :, i, :])
+    for m in members:
+        m.synchronize()
+    t_conc_d = time.perf_counter() - t0
+    _progress("concurrent contexts, direct launches", t_conc_d, "s")
     # as one batch, from where the members stand
     batch = edus.Batch(members)
     ng, _ = batch.groups()
     batch.graph(args.graph)
-    batch.rk4_steps(dt, E[W + K:2 * W + K])
+    batch.rk4_steps(dt, E[3 * W + 3 * K:4 * W + 3 * K])
     batch.synchronize()
     t1 = time.perf_counter()
-    batch.rk4_steps(dt, E[2 * W + K:2 * W + 2 * K])
+    batch.rk4_steps(dt, E[4 * W + 3 * K:4 * W + 4 * K])
     batch.synchronize()
     t_batch = time.perf_counter() - t1
     tr, herm = batch.check()
@@ -282,7 +312,10 @@ def run_batch(args):
         "config": {"workload": "hetero_batch64", "members": n, "groups": ng, "graph_steps": args.graph,
                    "classes": [f"{w} {g}x{g}" for w, g in classes]},
         "sequential": {"value": work / t_seq, "ms_per_step": t_seq / K * 1e3},
+        "concurrent_contexts_graph": {"value": work / t_conc, "ms_per_step": t_conc / K * 1e3},
+        "concurrent_contexts_direct": {"value": work / t_conc_d, "ms_per_step": t_conc_d / K * 1e3},
         "batch_over_sequential": t_seq / t_batch,
+        "batch_over_best_concurrent": min(t_conc, t_conc_d) / t_batch,
         "monitors": {"trace_dev": tr, "herm_dev": herm},
     }
     print(json.dumps(out), flush=True)
