@@ -14,9 +14,10 @@
 // The sum is elementwise in (a, b): no contraction that matrix cores could
 // take, and O(Nk^2) pairs (q, k) with O(nb^2 nv nc) FP64 work each, so the
 // kernel is bound by the FP64 vector pipes, not by HBM.  Mapping:
-//   * a split pass stores the valence columns of every U_k as Uv[k][v][a] and
-//     the conduction columns as Uc[k][c][a] (one array each), so the columns a
-//     work-group needs are contiguous;
+//   * a split pass stores the valence columns of every U_k as Uv[k][v][a]
+//     and the conduction columns as planes Uc[c nb + a][k], so that a step's
+//     valence block is one contiguous run and the conduction elements of the
+//     consecutive k + q of a work-group are adjacent;
 //   * one work-group = W waves; each lane owns one element pair (a, b) (the
 //     upper triangle when that packs the lanes better, the lower triangle
 //     then follows from X_ba(q) = conj X_ab(q)), for G q-groups per wave when
@@ -35,7 +36,10 @@
 //     table; A_v(k) is built once per k in registers and used for all TQ q
 //     of a lane;
 //   * work-groups of neighbouring q go to the same XCD: their Uc(k + q)
-//     windows overlap and stay in that XCD's L2.
+//     windows overlap and stay in that XCD's L2;
+//   * when the q-blocks would leave the last round of resident work-groups
+//     mostly empty, each block's k range is split in two and the partial sums
+//     are added into a zeroed X (0 + p1 + p2 is the same in either order).
 // The summation order differs from the reference's (k outermost over pairs
 // of bands, r multiplied instead of divided), so results agree to rounding,
 // not bit for bit.
