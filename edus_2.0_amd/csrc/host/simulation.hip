@@ -19,6 +19,7 @@
 
 #include "../../../include/edus_gpu.h"
 #include "../engine/engine_internal.hpp"
+#include "h5out.hpp"
 #include "setup.hpp"
 
 namespace edus {
@@ -168,6 +169,14 @@ class Simulation {
     int propagate(int n, double* times, double* laser, double* popb, double* popw, double* vel,
                   double* laser_A = nullptr);
 
+    // output.h5 (Simulation.cpp:215-229,385-420; GlobalFunctions.cpp:19-66)
+    edus_sim_params par;               // the input keys, as given (the /json group)
+    std::unique_ptr<H5Out> h5;         // open while edus_sim_write_outputs runs with write_h5
+    int h5_counter = 0;                // get_it_sparse (Simulation.cpp:685-691): one node per sparse print
+    bool PrintSparse(double t) const;  // PrintObservables(t, true) (Simulation.cpp:240-257)
+    void h5_open(const std::string& path);
+    void h5_record(double t);
+
   private:
     double jacobian() const;
     void setup_coulomb(const edus_sim_params& p);
@@ -180,7 +189,7 @@ double Simulation::jacobian() const
     return std::abs(A[0][0] * A[1][1] - A[1][0] * A[0][1]);
 }
 
-Simulation::Simulation(const edus_sim_params& p)
+Simulation::Simulation(const edus_sim_params& p) : par(p)
 {
     // times in a.u. (Simulation.cpp:18-26); step count (:355)
     const double ti = Convert(p.initialtime, unit(p.initialtime_units), AuTime);
@@ -313,6 +322,129 @@ Simulation::Simulation(const edus_sim_params& p)
     desolver.set_ResolutionTime(dt);
 }
 
+bool Simulation::PrintSparse(double t) const
+{
+    // inside any pulse (t0, t0 + duration] the pulse resolution, else the
+    // general one; a zero resolution (the reference divides by it) falls back
+    // to the other
+    int pr = par.printresolution;
+    for (const Laser& L : setoflaser.lasers) {
+        if (t > L.t0 + 1.e-07 && t < L.t0 + L.duration + 1.e-07) {
+            pr = par.printresolution_pulse;
+            break;
+        }
+        pr = par.printresolution;
+    }
+    if (pr <= 0) pr = par.printresolution > 0 ? par.printresolution : par.printresolution_pulse;
+    if (pr <= 0) throw EngineError(EDUS_ERR_ARG, "printresolution must be positive");
+    return int(std::round(t / dt)) % pr == 0;
+}
+
+void Simulation::h5_open(const std::string& path)
+{
+    h5 = std::make_unique<H5Out>(path);
+    h5_counter = 0;
+    H5Out& f = *h5;
+    // dump_json_in_h5 (GlobalFunctions.cpp:19-66): the input's own values
+    f.group("/json");
+    f.write_bytes("/json/tb_file", par.tb_file);
+    const int coul = par.coulomb != 0;
+    f.write_ints("/json/coulomb", &coul, 1);
+    f.write_ints("/json/grid", par.grid, 3);
+    f.write_ints("/json/filledbands", &par.filledbands, 1);
+    f.write_doubles("/json/dt", &par.dt, 1);
+    f.write_ints("/json/printresolution", &par.printresolution, 1);
+    f.write_doubles("/json/initialtime", &par.initialtime, 1);
+    f.write_doubles("/json/finaltime", &par.finaltime, 1);
+    f.write_ints("/json/num_bands", &nb, 1);
+    f.write_ints("/json/num_kpoints", &Nk, 1);
+    f.write_bytes("/json/solver", par.solver);
+    const int one = 1;  // the k-pool size of the reference's MPI build, one process here
+    f.write_ints("/json/comm_size", &one, 1);
+    for (int i = 0; i < par.nlasers; ++i) {
+        const edus_laser_params& L = par.lasers[i];
+        const std::string g = "/json/laser" + std::to_string(i);
+        f.group(g);
+        f.write_doubles(g + "/intensity", &L.intensity, 1);
+        f.write_doubles(g + "/frequency", &L.frequency, 1);
+        f.write_doubles(g + "/wavelength", &L.wavelength, 1);
+        const int cycles = int(L.cycles);  // json get<int>() truncates
+        f.write_ints(g + "/cycles", &cycles, 1);
+        // the reference stores the cycle count under "polarization" too (:58)
+        f.write_ints(g + "/polarization", &cycles, 1);
+    }
+    // k points in the reciprocal basis, [Nk][3] (Simulation.cpp:208-214), and the
+    // two bases as 3 x 3 matrices, column j = vector j, row-major
+    std::vector<double> kp(size_t(Nk) * 3, 0.0);
+    for (int ik = 0; ik < Nk; ++ik) {
+        kp[3 * size_t(ik)] = double(ik / N1) / N0;
+        kp[3 * size_t(ik) + 1] = double(ik % N1) / N1;
+    }
+    f.write_doubles("/json/kpoints", kp.data(), kp.size());
+    double Am[9], Bm[9];
+    const double(*A)[3] = material.A;
+    const double det = A[0][0] * (A[1][1] * A[2][2] - A[1][2] * A[2][1]) - A[0][1] * (A[1][0] * A[2][2] - A[1][2] * A[2][0]) +
+                       A[0][2] * (A[1][0] * A[2][1] - A[1][1] * A[2][0]);
+    const double inv[3][3] = {{(A[1][1] * A[2][2] - A[1][2] * A[2][1]) / det, (A[0][2] * A[2][1] - A[0][1] * A[2][2]) / det,
+                               (A[0][1] * A[1][2] - A[0][2] * A[1][1]) / det},
+                              {(A[1][2] * A[2][0] - A[1][0] * A[2][2]) / det, (A[0][0] * A[2][2] - A[0][2] * A[2][0]) / det,
+                               (A[0][2] * A[1][0] - A[0][0] * A[1][2]) / det},
+                              {(A[1][0] * A[2][1] - A[1][1] * A[2][0]) / det, (A[0][1] * A[2][0] - A[0][0] * A[2][1]) / det,
+                               (A[0][0] * A[1][1] - A[0][1] * A[1][0]) / det}};
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            Am[3 * i + j] = A[i][j];
+            Bm[3 * i + j] = 2. * kPi * inv[j][i];  // 2 pi (A^-1)^T (Model.hpp:46)
+        }
+    f.write_doubles("/json/A", Am, 9);
+    f.write_doubles("/json/B", Bm, 9);
+    // the groups of Simulation.cpp:225-229
+    for (const char* g : {"/fullH", "/DensityMatrix_k", "/DensityMatrix_k_bloch", "/SelfEnergy", "/time_au"}) f.group(g);
+}
+
+void Simulation::h5_record(double t)
+{
+    H5Out& f = *h5;
+    const std::string node = std::to_string(h5_counter++);
+    const size_t M = size_t(nb) * nb;
+    if (par.toprint_dmk_wannier || par.toprint_dmk_bloch) {
+        std::vector<cplx> rho(size_t(Nk) * M);
+        engine->download(engine->y, reinterpret_cast<double*>(rho.data()));
+        if (par.toprint_dmk_wannier) {
+            const std::string g = "/DensityMatrix_k/" + node;
+            f.group(g);
+            f.group(g + "/0");
+            f.write_doubles(g + "/0/local", reinterpret_cast<const double*>(rho.data()), 2 * rho.size());
+        }
+        if (par.toprint_dmk_bloch) {
+            // go_to_bloch (Operator.hpp:528-564): O_b = U^dagger O_w U per k
+            std::vector<cplx> rb(rho.size());
+            parallel_for(Nk, [&](int ik) {
+                const cplx* u = &U[size_t(ik) * M];
+                const cplx* r = &rho[size_t(ik) * M];
+                std::vector<cplx> tmp(M);
+                for (int a = 0; a < nb; ++a)
+                    for (int m = 0; m < nb; ++m) {
+                        cplx sum = 0.;
+                        for (int b = 0; b < nb; ++b) sum += r[a * nb + b] * u[b * nb + m];
+                        tmp[a * nb + m] = sum;
+                    }
+                for (int n = 0; n < nb; ++n)
+                    for (int m = 0; m < nb; ++m) {
+                        cplx sum = 0.;
+                        for (int a = 0; a < nb; ++a) sum += std::conj(u[a * nb + n]) * tmp[a * nb + m];
+                        rb[size_t(ik) * M + n * nb + m] = sum;
+                    }
+            });
+            const std::string g = "/DensityMatrix_k_bloch/" + node;
+            f.group(g);
+            f.group(g + "/0");
+            f.write_doubles(g + "/0/local", reinterpret_cast<const double*>(rb.data()), 2 * rb.size());
+        }
+    }
+    f.write_doubles("/time_au/" + node, &t, 1);
+}
+
 void Simulation::setup_coulomb(const edus_sim_params& p)
 {
     // ModelCoulomb.cpp:89-134 and Coulomb.cpp:19-59
@@ -409,6 +541,7 @@ int Simulation::propagate(int n, double* times, double* laser, double* popb, dou
     int it = 0;
     while (it < n) {
         const double t = desolver.get_CurrentTime();
+        if (h5 && PrintSparse(t)) h5_record(t);  // the HDF5 block comes first in do_onestep
         if (PrintObservables(t)) {
             if (times) times[printed] = t;
             double E[3];
@@ -433,7 +566,7 @@ int Simulation::propagate(int n, double* times, double* laser, double* popb, dou
         // advance to the next printed step in one batch
         int batch = 1;
         double tt = t + dt;
-        while (it + batch < n && !PrintObservables(tt)) {
+        while (it + batch < n && !PrintObservables(tt) && !(h5 && PrintSparse(tt))) {
             ++batch;
             tt += dt;
         }
@@ -693,6 +826,11 @@ int edus_sim_write_outputs(edus_sim* sim, const char* dir, int nsteps, int* npri
             pw(size_t(n + 1) * nb), v(6 * size_t(n + 1));
         // DM0.txt (print_Rdecay of the initial state, Simulation.cpp:190) before stepping
         write_dm0(S, dir);
+        if (S.par.write_h5) S.h5_open(std::string(dir) + "/output.h5");
+        struct CloseH5 {
+            Simulation& s;
+            ~CloseH5() { s.h5.reset(); }
+        } close_h5{S};
         const int k = S.propagate(n, t.data(), las.data(), pb.data(), pw.data(), v.data(), lasA.data());
         if (nprinted) *nprinted = k;
         const std::string d(dir);

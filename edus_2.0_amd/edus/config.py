@@ -38,9 +38,11 @@ DEFAULTS = {
     "finaltime": 0.0, "finaltime_units": "autime",
     "solver": "RK", "order": 4,
     "peierls": False,
+    "toprint": {"DMk_wannier": "false", "DMk_bloch": "false", "fullH": "false", "SelfEnergy": "false"},
     "lasers": [],
     # engine extensions (not reference keys)
     "coulomb_center_quirk": True,   # keep ModelCoulomb.cpp:76-79 behaviour
+    "write_h5": False,              # write output.h5 next to the text outputs
 }
 
 
@@ -62,6 +64,12 @@ def compose(user: dict, base_dir: str | None = None) -> dict:
         out["tb_file"] = os.path.join(base_dir, out["tb_file"])
     if out["gradient_space"] not in ("R", "k"):
         raise ValueError("gradient_space must be 'R' or 'k'")
+    tp = dict(DEFAULTS["toprint"])
+    tp.update(out.get("toprint") or {})
+    out["toprint"] = tp
+    for key in ("fullH", "SelfEnergy"):
+        if str(tp[key]) == "true":
+            raise ValueError(f"toprint {key} is not written by this engine (only DMk_wannier and DMk_bloch)")
     if len(out["lasers"]) > 10:
         raise ValueError("at most 10 lasers (input_schema.json maxItems)")
     return out

@@ -79,6 +79,7 @@ class _SimParams(C.Structure):
         ("lasers", _Laser * 10),
         ("precision", C.c_int),
         ("layout", C.c_int), ("ab_seed", C.c_int), ("peierls", C.c_int), ("gradient_space", C.c_int),
+        ("write_h5", C.c_int), ("toprint_dmk_wannier", C.c_int), ("toprint_dmk_bloch", C.c_int),
     ]
 
 
@@ -526,6 +527,10 @@ def _sim_params(p: dict, precision: int, layout: int) -> _SimParams:
     s.ab_seed = int(bool(p.get("ab_seed", False)))
     s.peierls = int(bool(p.get("peierls", False)))
     s.gradient_space = 1 if p.get("gradient_space", "R") == "k" else 0
+    s.write_h5 = int(bool(p.get("write_h5", False)))
+    toprint = p.get("toprint", {})
+    s.toprint_dmk_wannier = int(str(toprint.get("DMk_wannier", "false")) == "true")
+    s.toprint_dmk_bloch = int(str(toprint.get("DMk_bloch", "false")) == "true")
     return s
 
 

@@ -324,6 +324,10 @@ typedef struct {
     int ab_seed;              /* 1: start Adams-Bashforth with RK4 steps (edus_ab_seed), 0: zero history */
     int peierls;              /* the reference's "peierls" key */
     int gradient_space;       /* the reference's "gradient_space" key: 0 "R", 1 "k" */
+    int write_h5;             /* 1: edus_sim_write_outputs also writes output.h5 (the HDF5 1.10 C
+                                 library is loaded at run time: $EDUS_HDF5_LIB or libhdf5.so.103) */
+    int toprint_dmk_wannier;  /* the reference's "toprint": {"DMk_wannier": "true"} */
+    int toprint_dmk_bloch;    /* the reference's "toprint": {"DMk_bloch": "true"} */
 } edus_sim_params;
 
 edus_sim* edus_sim_create(const edus_sim_params* p);
@@ -346,7 +350,11 @@ int edus_sim_steps(edus_sim* sim, int nsteps);
  * Simulation.cpp:190 with Operator.hpp:288-311) and Time.txt, Laser.txt,
  * Laser_A.txt, Population.txt, Population_wannier.txt, Velocity.txt, one row
  * per printed step in the reference's formats (Simulation.cpp:420-429,
- * 459-467,614-620).  output.h5 is not written. */
+ * 459-467,614-620).  With write_h5, also output.h5 as the reference's HDF5
+ * build writes it (Simulation.cpp:215-229,385-420, GlobalFunctions.cpp:19-66):
+ * the /json group, and at every sparse print step /time_au/<i> and, if asked
+ * for by toprint, /DensityMatrix_k/<i>/0/local and
+ * /DensityMatrix_k_bloch/<i>/0/local (interleaved complex [Nk][nb][nb]). */
 int edus_sim_write_outputs(edus_sim* sim, const char* dir, int nsteps, int* nprinted);
 /* Set-up arrays for inspection: "H0k", "rk0", "rk1", "rk2", "U", "Vel0",
  * "Vel1", "Vel2", "W", "rho0" (initial state in k), "hartree" ([nb][nb]),

@@ -61,3 +61,14 @@ def test_lindhard_rejects_bad_shapes_before_any_device_call():
     big = np.zeros((2, 17, 17), dtype=np.complex128)
     with pytest.raises(edus.EdusError):
         edus.lindhard(big, np.zeros((2, 17)), 2, 1, 8)  # nb = 17
+
+
+def test_toprint_keys():
+    """The reference's "toprint" flags are strings; the density matrices are
+    written, fullH and SelfEnergy are refused."""
+    p = config.compose({"toprint": {"DMk_bloch": "true"}})
+    assert p["toprint"] == {"DMk_wannier": "false", "DMk_bloch": "true", "fullH": "false", "SelfEnergy": "false"}
+    assert p["write_h5"] is False
+    for key in ("fullH", "SelfEnergy"):
+        with pytest.raises(ValueError):
+            config.compose({"toprint": {key: "true"}})
