@@ -605,6 +605,14 @@ void Engine::transform_to_R(const double2* in, double2* out)
     axis_fft(0, 1, G, nullptr, out, 0.0, 1.0 / double(Nk), nullptr, "to_R_col");
 }
 
+void Engine::transform_to_k(const double2* in, double2* out)
+{
+    if (layout != kFull || B != 1) throw EngineError(EDUS_ERR_UNSUPPORTED, "transform_to_k needs one FULL-layout context");
+    if (N0 < 2 || N1 < 2) throw EngineError(EDUS_ERR_UNSUPPORTED, "transform_to_k needs a 2-D grid");
+    axis_fft(1, 2, in, nullptr, G, 0.0, 1.0, nullptr, "to_k_row");
+    axis_fft(0, 2, G, nullptr, out, 0.0, 1.0, nullptr, "to_k_col");
+}
+
 void Engine::evaluate(const double2* in, const double E[3], const StageUpdate& up)
 {
     double cg[2];

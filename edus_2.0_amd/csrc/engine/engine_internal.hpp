@@ -136,6 +136,8 @@ class Engine {
     // k -> R of every plane of a FULL-layout field, sign -1 and 1/Nk
     // (Operator::go_to_R, Operator.hpp:566-606, fftPair.cpp:89-107); uses G
     void transform_to_R(const double2* in, double2* out);
+    // R -> k of every plane, sign +1 and no scaling (Operator::go_to_k); uses G
+    void transform_to_k(const double2* in, double2* out);
 
   private:
     std::vector<void*> owned;

@@ -8,6 +8,7 @@
 #include <dlfcn.h>
 
 #include <cstdint>
+#include <cstdio>
 #include <cstdlib>
 #include <stdexcept>
 #include <vector>
@@ -25,6 +26,7 @@ using hsize_t = unsigned long long;
 constexpr hid_t kDefault = 0;   // H5P_DEFAULT
 constexpr hid_t kAll = 0;       // H5S_ALL
 constexpr unsigned kTrunc = 2;  // H5F_ACC_TRUNC
+constexpr unsigned kRdwr = 1;   // H5F_ACC_RDWR
 
 }  // namespace
 
@@ -35,6 +37,7 @@ struct H5Out::Impl {
     void* lib = nullptr;
     herr_t (*open)() = nullptr;
     hid_t (*fcreate)(const char*, unsigned, hid_t, hid_t) = nullptr;
+    hid_t (*fopen)(const char*, unsigned, hid_t) = nullptr;
     herr_t (*fclose)(hid_t) = nullptr;
     hid_t (*gcreate)(hid_t, const char*, hid_t, hid_t, hid_t) = nullptr;
     herr_t (*gclose)(hid_t) = nullptr;
@@ -63,6 +66,7 @@ struct H5Out::Impl {
         if (!lib) throw std::runtime_error("output.h5: no HDF5 1.10 library found (set EDUS_HDF5_LIB)");
         sym(open, "H5open");
         sym(fcreate, "H5Fcreate");
+        sym(fopen, "H5Fopen");
         sym(fclose, "H5Fclose");
         sym(gcreate, "H5Gcreate2");
         sym(gclose, "H5Gclose");
@@ -94,11 +98,18 @@ struct H5Out::Impl {
     }
 };
 
-H5Out::H5Out(const std::string& path) : impl_(new Impl)
+H5Out::H5Out(const std::string& path, bool append) : impl_(new Impl)
 {
     impl_->load();
-    impl_->file = impl_->fcreate(path.c_str(), kTrunc, kDefault, kDefault);
-    if (impl_->file < 0) throw std::runtime_error("cannot create " + path);
+    bool exists = false;
+    if (append) {
+        std::FILE* fp = std::fopen(path.c_str(), "rb");
+        exists = fp != nullptr;
+        if (fp) std::fclose(fp);
+    }
+    impl_->file = exists ? impl_->fopen(path.c_str(), kRdwr, kDefault)
+                         : impl_->fcreate(path.c_str(), kTrunc, kDefault, kDefault);
+    if (impl_->file < 0) throw std::runtime_error("cannot open " + path);
 }
 
 H5Out::~H5Out()

@@ -12,9 +12,10 @@ namespace host {
 
 class H5Out {
   public:
-    // Creates (truncates) the file.  Throws when the HDF5 library cannot be
-    // loaded or the file cannot be created.
-    explicit H5Out(const std::string& path);
+    // Creates (truncates) the file, or with append opens an existing one for
+    // writing (creating it when absent).  Throws when the HDF5 library cannot
+    // be loaded or the file cannot be opened.
+    explicit H5Out(const std::string& path, bool append = false);
     ~H5Out();
     H5Out(const H5Out&) = delete;
     H5Out& operator=(const H5Out&) = delete;

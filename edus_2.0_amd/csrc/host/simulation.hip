@@ -677,6 +677,7 @@ static void write_dm0(const Simulation& S, const char* dir)
 //   W       (:168-174): InvEps_ab(R) V_ab(R)
 struct ScreenedFields {
     std::vector<edus::host::cplx> X, V, Eps, InvEps, W;
+    std::vector<edus::host::cplx> Xk;  // the response in k, before go_to_R
 };
 
 static ScreenedFields screened_potential(Simulation& S)
@@ -696,9 +697,11 @@ static ScreenedFields screened_potential(Simulation& S)
     EDUS_HIP_CHECK(hipMemcpyAsync(e.G, S.U.data(), size_t(Nk) * M * sizeof(double2), hipMemcpyHostToDevice, e.s));
     EDUS_HIP_CHECK(hipMemcpyAsync(e.D, S.eps.data(), size_t(Nk) * nb * sizeof(double), hipMemcpyHostToDevice, e.s));
     edus::launch_lindhard(plan, e.G, reinterpret_cast<const double*>(e.D), e.f, e.aux, e.s);
+    out.Xk.resize(size_t(Nk) * M);
+    e.download(e.aux, reinterpret_cast<double*>(out.Xk.data()));  // synchronises the stream
     e.transform_to_R(e.aux, e.D);
     out.X.resize(size_t(Nk) * M);
-    e.download(e.D, reinterpret_cast<double*>(out.X.data()));  // synchronises the stream
+    e.download(e.D, reinterpret_cast<double*>(out.X.data()));
 
     std::vector<std::array<double, 3>> c(nb);
     for (int n = 0; n < nb; ++n) c[n] = S.material.wannier_centre(n);
@@ -727,6 +730,27 @@ static ScreenedFields screened_potential(Simulation& S)
                 }
         }
     return out;
+}
+
+// Operator::go_to_k (sign +1, no scaling) of an R-space field [R][a][b], on
+// the device through the engine's FULL-layout scratch arrays
+static std::vector<edus::host::cplx> field_to_k(Simulation& S, const std::vector<edus::host::cplx>& fR)
+{
+    edus::Engine& e = *S.engine;
+    std::vector<edus::host::cplx> fk(fR.size());
+    e.upload(reinterpret_cast<const double*>(fR.data()), e.aux);
+    e.transform_to_k(e.aux, e.D);
+    e.download(e.D, reinterpret_cast<double*>(fk.data()));
+    return fk;
+}
+
+// BlockMatrix::write_h5(name, node, label) (BlockMatrix_definitions.hpp:345-380),
+// one process: group node/label/0 holding "local", the interleaved doubles
+static void write_blockmatrix(edus::host::H5Out& f, const std::string& path, const std::vector<edus::host::cplx>& a)
+{
+    f.group(path);
+    f.group(path + "/0");
+    f.write_doubles(path + "/0/local", reinterpret_cast<const double*>(a.data()), 2 * a.size());
 }
 
 // Operator::print_Rdecay(name, rwann) (Operator.hpp:288-311): file name + "0.txt"
@@ -891,6 +915,20 @@ int edus_sim_write_screened(edus_sim* sim, const char* dir)
         write_rdecay(S, f.Eps, d + "/Epsilon0.txt");
         write_rdecay(S, f.InvEps, d + "/InverseEpsilon0.txt");
         write_rdecay(S, f.W, d + "/W0.txt");
+        if (S.par.write_h5) {
+            // the HDF5 build's nodes under "-1" in output.h5, added to an
+            // existing file (Screened_potential.cpp:60-70,104-109,137-142,192-196):
+            // Xk, Uk, Vk, Epsilonk, Wk.  The inverse permittivity is written
+            // under "Epsilonk" again there (:174), which HDF5 refuses (the
+            // node exists), so the file keeps Epsilonk = eps(k).
+            edus::host::H5Out h5(d + "/output.h5", true);
+            h5.group("/-1");
+            write_blockmatrix(h5, "/-1/Xk", f.Xk);
+            write_blockmatrix(h5, "/-1/Uk", sim->s->U);
+            write_blockmatrix(h5, "/-1/Vk", field_to_k(*sim->s, f.V));
+            write_blockmatrix(h5, "/-1/Epsilonk", field_to_k(*sim->s, f.Eps));
+            write_blockmatrix(h5, "/-1/Wk", field_to_k(*sim->s, f.W));
+        }
     });
 }
 

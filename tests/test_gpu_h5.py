@@ -100,3 +100,41 @@ def test_hbn_output_h5(tmp_path):
             if node % 5 == 0:
                 _say("  node", node, "step", i, "max|d rho|", err, "bloch", errb)
             assert err <= 1e-10 and errb <= 1e-10
+
+
+def test_screened_potential_h5_nodes(tmp_path):
+    """The screened-potential application's nodes under "-1"
+    (Screened_potential.cpp:60-70,104-109,137-142,192-196): Xk is the response
+    in k, Uk the eigenvectors, Vk, Epsilonk and Wk the R-space fields taken
+    to k with sign +1 and no scaling (numpy's N * ifft2); added to an
+    existing output.h5."""
+    try:
+        h5read.lib()
+    except OSError:
+        pytest.skip("no HDF5 library on this machine")
+    p = hbn_params(write_h5=True, grid=[12, 10, 1])
+    sim = edus.Simulation(p)
+    sim.write_outputs(str(tmp_path), 3)  # creates output.h5 with the propagation's nodes
+    sim.write_screened(str(tmp_path))
+    f_ = sim.screened_potential()
+    o = oracle_py.Oracle(p)
+    nk, nb = o.Nk, o.nb
+    Xk_ref = oracle_py.lindhard(sim.get("U"), sim.get("eps"), 12, 10, 1)
+
+    def to_k(a):
+        g = a.reshape(12, 10, nb, nb)
+        return (np.fft.ifft2(g, axes=(0, 1)) * (12 * 10)).reshape(nk, nb, nb)
+
+    with h5read.File(os.path.join(str(tmp_path), "output.h5")) as f:
+        assert f.exists("/json/grid") and f.exists("/time_au/0")  # the earlier content is kept
+
+        def node(name):
+            return f.read(f"/-1/{name}/0/local").view(np.complex128).reshape(nk, nb, nb)
+
+        assert np.abs(node("Xk") - Xk_ref).max() <= 1e-11 * np.abs(Xk_ref).max()
+        assert np.array_equal(node("Uk"), sim.get("U"))
+        for name, key in (("Vk", "V"), ("Epsilonk", "Eps"), ("Wk", "W")):
+            ref = to_k(f_[key])
+            err = np.abs(node(name) - ref).max()
+            _say(name, "max abs err", err, "max", np.abs(ref).max())
+            assert err <= 1e-12 * max(1.0, np.abs(ref).max()), name
