@@ -384,11 +384,13 @@ int edus_lindhard_device(int nb, int N0, int N1, int filledbands, const double* 
  * = 1/|r_a + R - r_b| (0 below 1e-5 bohr), Eps(R) = 1 - V X, its elementwise
  * inverse (0 where |Eps| < 1e-3) and W(R) = Eps^-1 V; each [Nk][nb][nb] over
  * the FFT R grid in grid order; null outputs are skipped.  The simulation must
- * use the FULL layout.  HDF5 output (output.h5) is not written. */
+ * use the FULL layout. */
 int edus_sim_screened(edus_sim* sim, double* X, double* V, double* Eps, double* InvEps, double* W);
 /* The same, written as the reference's Operator::print_Rdecay files
  * (Operator.hpp:288-311) X0.txt, V0.txt, Epsilon0.txt, InverseEpsilon0.txt and
- * W0.txt in directory dir. */
+ * W0.txt in directory dir; with write_h5 also the nodes -1/Xk, -1/Uk, -1/Vk,
+ * -1/Epsilonk and -1/Wk (each <node>/0/local, interleaved complex) added to
+ * dir/output.h5 (Screened_potential.cpp:60-70,104-109,137-142,192-196). */
 int edus_sim_write_screened(edus_sim* sim, const char* dir);
 
 #ifdef __cplusplus
