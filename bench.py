@@ -27,7 +27,7 @@ HBM_PEAK_GBS = 8000.0
 
 
 def kernel_bytes(cls: str, npl: int, nk: int, nb: int, n_e: int, solver: str = "RK", order: int = 4,
-                 coulomb: bool = False) -> float:
+                 coulomb: bool = False, peierls: bool = False) -> float:
     """Algorithmic HBM bytes per launch: one array of npl planes is
     S = npl*nk*16 B.  grad_col reads rho, writes G (2S); grad_row reads rho
     and G, writes D (3S); kgrad (finite-difference stencil, gradient_space
@@ -36,12 +36,18 @@ def kernel_bytes(cls: str, npl: int, nk: int, nb: int, n_e: int, solver: str = "
     n_E xi, D (+ the exchange self energy) and the stage's state arrays and
     writes its outputs: RK4 averages (6 + n_E) S over its four stages, AB of
     order p reads y and p-1 history arrays and writes y and the new history
-    entry, (4 + n_E + p) S.  monitor reads the nb diagonal planes."""
+    entry, (4 + n_E + p) S.  monitor reads the nb diagonal planes.  Peierls
+    gauge: peierls_col reads H0 and the n_E xi and writes the half-phased
+    planes ((2 + n_E) S), peierls_row reads those and writes S_A[H] (2S), and
+    comm reads S_A[H] instead of H0, xi and D (one array in place of 2 + n_E)."""
     s = 16.0 * npl * nk
     sig = 1 if coulomb else 0
     comm = (6 + n_e + sig) if solver == "RK" else (4 + n_e + order + sig)
+    if peierls:
+        comm -= 1 + n_e
     return {"grad_col": 2 * s, "grad_row": 3 * s, "kgrad": 2 * s, "comm": comm * s, "hf_row": 2 * s,
-            "hf_col": 2.5 * s, "monitor": 16.0 * nb * nk}.get(cls, 0.0)
+            "hf_col": 2.5 * s, "monitor": 16.0 * nb * nk, "peierls_col": (2 + n_e) * s,
+            "peierls_row": 2 * s}.get(cls, 0.0)
 
 
 def cpu_baseline(params: dict, steps: int, integ: str, target_s: float = 15.0):
@@ -93,6 +99,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="ipa14_hex120")
     ap.add_argument("--layout", choices=["full", "hermitian"], default="full")
+    ap.add_argument("--peierls", action="store_true",
+                    help="Peierls gauge (the reference's peierls: true); steps use direct launches")
     ap.add_argument("--gradient-space", choices=["R", "k"], default="R",
                     help="R: spectral derivative (the reference default); k: finite-difference stencil")
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64",
@@ -132,6 +140,7 @@ def main():
     tmp = tempfile.mkdtemp(prefix="edus_bench_")
     params = models.workload(args.workload, tmp)
     params["gradient_space"] = args.gradient_space
+    params["peierls"] = bool(args.peierls)
     layout = edus.LAYOUT_FULL if args.layout == "full" else edus.LAYOUT_HERMITIAN
     prec = edus.PREC_FP64 if args.precision == "fp64" else edus.PREC_MIXED
     sim = edus.Simulation(params, layout=layout, precision=prec)
@@ -168,7 +177,7 @@ def main():
     sim.steps(args.steps)
     eng.synchronize()
     elapsed_direct = time.perf_counter() - t1
-    classes = ["grad_col", "grad_row", "kgrad", "comm", "hf_row", "hf_col", "monitor"]
+    classes = ["grad_col", "grad_row", "kgrad", "comm", "hf_row", "hf_col", "monitor", "peierls_col", "peierls_row"]
     times = {c: eng.timing_get(c) for c in classes}
     eng.timing_enable(False)
 
@@ -179,7 +188,7 @@ def main():
         pol += np.abs(np.asarray(las["polarization"], dtype=float))
     n_e = int(np.count_nonzero(pol))
     solver, order, coul = params["solver"], int(params["order"]), bool(params["coulomb"])
-    kb = {c: kernel_bytes(c, npl, nk, nb, n_e, solver, order, coul) for c in classes}
+    kb = {c: kernel_bytes(c, npl, nk, nb, n_e, solver, order, coul, bool(args.peierls)) for c in classes}
     dom = max(classes, key=lambda c: times[c][0])
     ms, cnt = times[dom]
     avg_s = ms / 1e3 / max(cnt, 1)
@@ -203,8 +212,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": args.workload, "nb": nb, "grid": [sim.N0, sim.N1], "dt_au": sim.dt,
                    "integrator": integ, "layout": args.layout, "precision": args.precision,
-                   "gradient_space": args.gradient_space,
-                   "graph_steps": args.graph,
+                   "gradient_space": args.gradient_space, "peierls": bool(args.peierls),
+                   "graph_steps": 0 if args.peierls else args.graph,
                    "model": "seeded random tight-binding (edus.models)", "tuning": eng.get_tuning().as_dict()},
         "roofline": roofline,
         "monitors": {"trace_dev": trace_dev, "herm_dev": herm_dev},

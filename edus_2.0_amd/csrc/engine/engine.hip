@@ -173,6 +173,8 @@ Engine::Engine(const edus_desc& d, const EngineOptions& opt)
     if (peierls) {
         alloc(reinterpret_cast<void**>(&Hp), pb);
         alloc(reinterpret_cast<void**>(&Tq), pb);
+        alloc(reinterpret_cast<void**>(&ptab[0]), size_t(N0) * sizeof(double2));
+        alloc(reinterpret_cast<void**>(&ptab[1]), size_t(N1) * sizeof(double2));
     }
     if (opt.velocity_det > 0.0) build_velocity(opt.velocity_det);
 
@@ -866,6 +868,9 @@ void Engine::peierls_sandwich(const double2* in, const double2* const add[3], co
         if (n < 2) continue;
         AxisFFTArgs a = axis_args(axis, fft_choice[axis], 4, src, nullptr, (two && axis == 0) ? Tq : out, th[axis],
                                   1.0 / n, nullptr);
+        // the phase of each frequency once per launch, not once per element
+        launch_phase_table(ptab[axis], n, th[axis], 1.0 / n, s);
+        a.ptab = ptab[axis];
         if (src == in)
             for (int d = 0; d < 3; ++d) {
                 a.in_add[d] = add ? add[d] : nullptr;

@@ -256,9 +256,15 @@ __global__ void __launch_bounds__(NT) axis_fft_kernel(AxisFFTArgs a)
         if (a.mode == 4) {  // scale exp(i coef R)
             for (int idx = threadIdx.x; idx < nlines * n; idx += NT) {
                 const int j = idx % n;
-                double sn, cs;
-                sincos(a.coef * double(gamma_centred(j, n)), &sn, &cs);
-                x[idx] = cmul(x[idx], make_double2(a.scale * cs, a.scale * sn));
+                double2 ph;
+                if (a.ptab) {
+                    ph = a.ptab[j];
+                } else {
+                    double sn, cs;
+                    sincos(a.coef * double(gamma_centred(j, n)), &sn, &cs);
+                    ph = make_double2(a.scale * cs, a.scale * sn);
+                }
+                x[idx] = cmul(x[idx], ph);
             }
         } else if (a.mode == 0) {
             for (int idx = threadIdx.x; idx < nlines * n; idx += NT) {

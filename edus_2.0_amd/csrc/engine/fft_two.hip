@@ -52,11 +52,12 @@ __device__ inline double2 load_in(const AxisFFTArgs& a, long o)
     return v;
 }
 
-// mode 4 multiplier scale * exp(i coef R)
-__device__ inline double2 phase(const AxisFFTArgs& a, int R)
+// mode 4 multiplier scale * exp(i coef R(j)) of frequency index j (line length n)
+__device__ inline double2 phase(const AxisFFTArgs& a, int j, int n)
 {
+    if (a.ptab) return a.ptab[j];
     double sn, cs;
-    sincos(a.coef * double(R), &sn, &cs);
+    sincos(a.coef * double(gamma_centred(j, n)), &sn, &cs);
     return make_double2(a.scale * cs, a.scale * sn);
 }
 
@@ -154,7 +155,7 @@ __global__ void __launch_bounds__(NT) two_step_kernel(AxisFFTArgs a)
             } else if constexpr (mode == 4) {
                 sfor<RA>([&](auto k_) {
                     constexpr int k = decltype(k_)::value;
-                    v[k] = cmul(v[k], phase(a, gamma_centred(k, RA)));
+                    v[k] = cmul(v[k], phase(a, k, RA));
                 });
             }
             if constexpr (sandwich) reg::rdft<RA, 1>(v);
@@ -263,7 +264,7 @@ __global__ void __launch_bounds__(NT) two_step_kernel(AxisFFTArgs a)
                     } else if constexpr (mode == 4) {
                         sfor<RB>([&](auto k_) {
                             constexpr int k = decltype(k_)::value;
-                            u[k] = cmul(u[k], phase(a, gamma_centred(ka + RA * k, n)));
+                            u[k] = cmul(u[k], phase(a, ka + RA * k, n));
                         });
                     } else {
                         sfor<RB>([&](auto k_) {
@@ -364,6 +365,20 @@ void launch_two(const AxisFFTArgs& a, int variant, hipStream_t s)
     X(144, 12, 12) X(160, 16, 10) X(180, 12, 15) X(192, 16, 12) X(240, 16, 15) X(256, 16, 16)
 
 }  // namespace
+
+__global__ void phase_table_kernel(double2* __restrict__ tab, int n, double coef, double scale)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    double sn, cs;
+    sincos(coef * double(gamma_centred(j, n)), &sn, &cs);
+    tab[j] = make_double2(scale * cs, scale * sn);
+}
+
+void launch_phase_table(double2* tab, int n, double coef, double scale, hipStream_t s)
+{
+    hipLaunchKernelGGL(phase_table_kernel, dim3((n + 255) / 256), dim3(256), 0, s, tab, n, coef, scale);
+}
 
 int two_step_max_lines(int n, int variant)
 {

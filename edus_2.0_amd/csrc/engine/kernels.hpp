@@ -34,6 +34,9 @@ struct AxisFFTArgs {
     // mode 4 input: in + sum_d in_add_c[d] * in_add[d] (null terms skipped)
     const double2* in_add[3];
     double in_add_c[3];
+    // mode 4 multiplier per frequency index j < n: scale * exp(i coef R(j)),
+    // R(j) Gamma-centred (launch_phase_table); null: formed per element
+    const double2* ptab;
 };
 
 // Per-stage update rule of the fused commutator kernel.  With f the
@@ -112,6 +115,8 @@ struct FFTChoice {
     int L = 1;         // lines per work-group
 };
 void launch_fft(const AxisFFTArgs& a, const FFTChoice& c, hipStream_t s);
+// tab[j] = scale * exp(i coef R(j)), j < n, R(j) the Gamma-centred index
+void launch_phase_table(double2* tab, int n, double coef, double scale, hipStream_t s);
 // Engines for [H, rho] in the commutator kernel: FP64 FMAs, FP64 MFMA, or
 // FP32 MFMA with FP32 accumulation (mixed precision; everything else FP64).
 enum CommKind { kCommScalar = 0, kCommMfma64 = 1, kCommMfma32 = 2 };
