@@ -872,9 +872,10 @@ void Engine::peierls_sandwich(const double2* in, const double2* const add[3], co
         launch_phase_table(ptab[axis], n, th[axis], 1.0 / n, s);
         a.ptab = ptab[axis];
         if (src == in)
-            for (int d = 0; d < 3; ++d) {
-                a.in_add[d] = add ? add[d] : nullptr;
-                a.in_add_c[d] = addc ? addc[d] : 0.0;
+            for (int d = 0; d < 3; ++d) {  // a zero field component reads nothing
+                const bool use = add && add[d] && addc && addc[d] != 0.0;
+                a.in_add[d] = use ? add[d] : nullptr;
+                a.in_add_c[d] = use ? addc[d] : 0.0;
             }
         TimedLaunch t;
         tbegin(axis == 0 ? "peierls_col" : "peierls_row", t);

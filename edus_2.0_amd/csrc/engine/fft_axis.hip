@@ -226,17 +226,31 @@ __global__ void __launch_bounds__(NT) axis_fft_kernel(AxisFFTArgs a)
             const int c = a.axis == 1 ? idx / n : idx % L;
             const int j = a.axis == 1 ? idx - c * n : idx / L;
             if (idx < span && c < nlines) {
-                const long o = base + c * cstride + long(j) * rowstride;
-                double2 x = a.in[o];
-                if (a.mode == 4) {  // input: a sum of fields
-                    for (int d = 0; d < 3; ++d)
-                        if (a.in_add[d]) {
-                            const double2 w = a.in_add[d][o];
-                            x.x += a.in_add_c[d] * w.x;
-                            x.y += a.in_add_c[d] * w.y;
-                        }
+                v[e] = a.in[base + c * cstride + long(j) * rowstride];
+            }
+        }
+        if (a.mode == 4) {  // input: a sum of fields, each field's loads issued together
+            for (int d = 0; d < 3; ++d) {
+                const double2* p = a.in_add[d];
+                if (!p) continue;
+                const double cd = a.in_add_c[d];
+                double2 w[PER];
+#pragma unroll
+                for (int e = 0; e < PER; ++e) {
+                    const int idx = threadIdx.x + e * NT;
+                    const int c = a.axis == 1 ? idx / n : idx % L;
+                    const int j = a.axis == 1 ? idx - c * n : idx / L;
+                    if (idx < span && c < nlines) w[e] = p[base + c * cstride + long(j) * rowstride];
                 }
-                v[e] = x;
+#pragma unroll
+                for (int e = 0; e < PER; ++e) {
+                    const int idx = threadIdx.x + e * NT;
+                    const int c = a.axis == 1 ? idx / n : idx % L;
+                    if (idx < span && c < nlines) {
+                        v[e].x += cd * w[e].x;
+                        v[e].y += cd * w[e].y;
+                    }
+                }
             }
         }
         for (int i = threadIdx.x; i < n; i += NT) twl[i] = a.tw[i];

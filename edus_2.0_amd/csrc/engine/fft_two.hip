@@ -35,21 +35,32 @@ using reg::sfor;
 
 constexpr int kTwoStepThreads = 256;
 
-// the kernel's input value at global offset o (mode 4: a sum of fields)
-template <int MODE>
-__device__ inline double2 load_in(const AxisFFTArgs& a, long o)
+// v[j] = the kernel's input at global offset off(j), j < R (mode 4: a sum of
+// fields; each field's R loads are issued together before any is used)
+template <int MODE, int R, class Off>
+__device__ inline void load_line(const AxisFFTArgs& a, double2 (&v)[R], Off off)
 {
-    double2 v = a.in[o];
+    sfor<R>([&](auto j_) {
+        constexpr int j = decltype(j_)::value;
+        v[j] = a.in[off(j)];
+    });
     if constexpr (MODE == 4) {
-#pragma unroll
-        for (int d = 0; d < 3; ++d)
-            if (a.in_add[d]) {
-                const double2 w = a.in_add[d][o];
-                v.x += a.in_add_c[d] * w.x;
-                v.y += a.in_add_c[d] * w.y;
-            }
+        for (int d = 0; d < 3; ++d) {
+            const double2* p = a.in_add[d];
+            if (!p) continue;
+            const double c = a.in_add_c[d];
+            double2 w[R];
+            sfor<R>([&](auto j_) {
+                constexpr int j = decltype(j_)::value;
+                w[j] = p[off(j)];
+            });
+            sfor<R>([&](auto j_) {
+                constexpr int j = decltype(j_)::value;
+                v[j].x += c * w[j].x;
+                v[j].y += c * w[j].y;
+            });
+        }
     }
-    return v;
 }
 
 // mode 4 multiplier scale * exp(i coef R(j)) of frequency index j (line length n)
@@ -126,10 +137,7 @@ __global__ void __launch_bounds__(NT) two_step_kernel(AxisFFTArgs a)
         // the whole line in one thread: no exchange
         if (!doA) return;
         double2 v[RA];
-        sfor<RA>([&](auto j_) {
-            constexpr int j = decltype(j_)::value;
-            v[j] = load_in<mode>(a, gA + long(j) * rowstride);
-        });
+        load_line<mode>(a, v, [&](int j) { return gA + long(j) * rowstride; });
         double mul[RA];
         if constexpr (mode == 3) {
             sfor<RA>([&](auto j_) {
@@ -177,10 +185,7 @@ __global__ void __launch_bounds__(NT) two_step_kernel(AxisFFTArgs a)
             // ---- load (A layout), step A forward
             double2 v[RA];
             if (doA) {
-                sfor<RA>([&](auto j_) {
-                    constexpr int j = decltype(j_)::value;
-                    v[j] = load_in<mode>(a, gA + long(j * RB) * rowstride + long(jb) * rowstride);
-                });
+                load_line<mode>(a, v, [&](int j) { return gA + long(j * RB + jb) * rowstride; });
             }
             for (int i = tid; i < n; i += NT) twl[i] = a.tw[i];
             __syncthreads();
