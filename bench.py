@@ -34,19 +34,21 @@ def kernel_bytes(cls: str, npl: int, nk: int, nb: int, n_e: int, solver: str = "
     "k") reads rho, writes D (2S); hf_row (forward or inverse row transform) 2S;
     hf_col (column convolution with the real W) 2.5S.  comm reads rho_s, H0,
     n_E xi, D (+ the exchange self energy) and the stage's state arrays and
-    writes its outputs: RK4 averages (6 + n_E) S over its four stages, AB of
+    writes its outputs: RK4's stages move (2, 2, 4, 2) S beyond the 3 + n_E
+    operand arrays, (5.5 + n_E) S on average (the second stage leaves the
+    accumulator alone; the third recovers its term from rho - y), AB of
     order p reads y and p-1 history arrays and writes y and the new history
-    entry, (4 + n_E + p) S.  monitor reads the nb diagonal planes.  Peierls
-    gauge: peierls_col reads H0 and the n_E xi and writes the half-phased
+    entry, (4 + n_E + p) S.  Peierls gauge: peierls_col reads H0 and the
+    n_E xi and writes the half-phased
     planes ((2 + n_E) S), peierls_row reads those and writes S_A[H] (2S), and
     comm reads S_A[H] instead of H0, xi and D (one array in place of 2 + n_E)."""
     s = 16.0 * npl * nk
     sig = 1 if coulomb else 0
-    comm = (6 + n_e + sig) if solver == "RK" else (4 + n_e + order + sig)
+    comm = (5.5 + n_e + sig) if solver == "RK" else (4 + n_e + order + sig)
     if peierls:
         comm -= 1 + n_e
     return {"grad_col": 2 * s, "grad_row": 3 * s, "kgrad": 2 * s, "comm": comm * s, "hf_row": 2 * s,
-            "hf_col": 2.5 * s, "monitor": 16.0 * nb * nk, "peierls_col": (2 + n_e) * s,
+            "hf_col": 2.5 * s, "peierls_col": (2 + n_e) * s,
             "peierls_row": 2 * s}.get(cls, 0.0)
 
 
@@ -177,7 +179,7 @@ def main():
     sim.steps(args.steps)
     eng.synchronize()
     elapsed_direct = time.perf_counter() - t1
-    classes = ["grad_col", "grad_row", "kgrad", "comm", "hf_row", "hf_col", "monitor", "peierls_col", "peierls_row"]
+    classes = ["grad_col", "grad_row", "kgrad", "comm", "hf_row", "hf_col", "peierls_col", "peierls_row"]
     times = {c: eng.timing_get(c) for c in classes}
     eng.timing_enable(False)
 

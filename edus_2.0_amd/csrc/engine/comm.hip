@@ -184,6 +184,20 @@ __device__ inline void mfma_commutator32(const double2* sH, const double2* sR, i
 
 // AB: the stage is an Adams-Bashforth step (its history is prefetched, so the
 // register budget of the other stages does not carry four history arrays).
+// x1 += g1 (rho - x2) for a thread's elements, rho read from the tile
+template <int PER>
+__device__ inline void fold_g1(double g1, const double2* sR, const int (&slot_of)[PER], const bool (&ok)[PER],
+                               const double2 (&x2)[PER], double2 (&x1)[PER])
+{
+#pragma unroll
+    for (int j = 0; j < PER; ++j)
+        if (ok[j]) {
+            const double2 r = sR[slot_of[j]];
+            x1[j].x += g1 * (r.x - x2[j].x);
+            x1[j].y += g1 * (r.y - x2[j].y);
+        }
+}
+
 template <int NB, int LAYOUT, int KIND, int TILE, bool AB>
 __global__ void __launch_bounds__(kCommThreads) comm_kernel(CommArgs a)
 {
@@ -395,6 +409,7 @@ __global__ void __launch_bounds__(kCommThreads) comm_kernel(CommArgs a)
             else
                 mfma_commutator32<NB, LAYOUT, PS>(sH, sR, kk, lane, cre[w], cim[w]);
         }
+        if (!ab && u.g1 != 0.0) fold_g1(u.g1, sR, slot_of, ok, ex[0], x1v);  // rho before the products replace it
         __syncthreads();
         const int i = lane & 15, q = lane >> 4;
 #pragma unroll
@@ -414,6 +429,8 @@ __global__ void __launch_bounds__(kCommThreads) comm_kernel(CommArgs a)
         __syncthreads();
     }
 
+    if constexpr (KIND == kCommScalar && !AB)  // rho stays in the tile
+        if (u.g1 != 0.0) fold_g1(u.g1, sR, slot_of, ok, ex[0], x1v);
 #pragma unroll
     for (int j = 0; j < PER; ++j) {
         if (!ok[j]) continue;
@@ -458,19 +475,26 @@ __global__ void __launch_bounds__(kCommThreads) comm_kernel(CommArgs a)
                 }
             }
             u.y[og] = yv;
+            x1v[j] = yv;  // kept for the monitor
         } else {
-            if (u.out1) u.out1[og] = make_double2(x1v[j].x + u.b1 * f.x, x1v[j].y + u.b1 * f.y);
+            if (u.out1) {
+                const double2 o1 = make_double2(x1v[j].x + u.b1 * f.x, x1v[j].y + u.b1 * f.y);
+                u.out1[og] = o1;
+                x1v[j] = o1;  // kept for the monitor
+            }
             if (u.out2) u.out2[og] = make_double2(ex[0][j].x + u.b2 * f.x, ex[0][j].y + u.b2 * f.y);
         }
     }
 
     if (u.mon) {
-        // The block's new state is in global memory (visible to the block after
-        // the barrier, mostly still in cache): check trace and Hermiticity of
-        // its TK k-points without another pass over HBM.
+        // Trace and Hermiticity of the state this stage wrote (AB: y, RK: out1)
+        // over the tile's TK k-points, from the values the threads still hold,
+        // put back into the rho tile (no longer read by anyone after the barrier).
         __syncthreads();
-        double* sred = reinterpret_cast<double*>(lds);  // the tile is no longer needed
-        const double2* ys = u.mon_src;
+#pragma unroll
+        for (int j = 0; j < PER; ++j)
+            if (ok[j]) sR[slot_of[j]] = x1v[j];
+        __syncthreads();
         double mt = 0., mh = 0.;
         // trace and diagonal imaginary part: one thread per k-point of the tile
         for (int kk = threadIdx.x; kk < TK; kk += NT) {
@@ -479,12 +503,12 @@ __global__ void __launch_bounds__(kCommThreads) comm_kernel(CommArgs a)
             double2 tr = make_double2(-u.nf, 0.);
             for (int n = 0; n < NB; ++n) {
                 bool c;
-                const double2 v = ys[ioff + long(slot<NB, LAYOUT>(n, n, c)) * Nk + k];
+                const double2 v = sR[slot<NB, LAYOUT>(n, n, c) * PS + kk];
                 tr = cadd(tr, v);
-                mh = fmax(mh, 2.0 * fabs(v.y));
+                mh = mon_max(mh, 2.0 * fabs(v.y));
             }
             const double t = sqrt(tr.x * tr.x + tr.y * tr.y);
-            mt = (t != t) ? __builtin_huge_val() : fmax(mt, t);  // NaN trips the monitor
+            mt = mon_max(mt, t);
         }
         // |rho_nm - conj(rho_mn)| over all pairs n < m, spread over the block
         if (LAYOUT == kFull) {
@@ -500,24 +524,32 @@ __global__ void __launch_bounds__(kCommThreads) comm_kernel(CommArgs a)
                     ++n;
                 }
                 const int m = n + 1 + r;
-                const double2 x = ys[ioff + long(n * NB + m) * Nk + k];
-                const double2 z = ys[ioff + long(m * NB + n) * Nk + k];
+                const double2 x = sR[(n * NB + m) * PS + kk];
+                const double2 z = sR[(m * NB + n) * PS + kk];
                 const double d = sqrt((x.x - z.x) * (x.x - z.x) + (x.y + z.y) * (x.y + z.y));
-                mh = (d != d) ? __builtin_huge_val() : fmax(mh, d);
+                mh = mon_max(mh, d);
             }
         }
-        sred[threadIdx.x] = mt;
-        sred[NT + threadIdx.x] = mh;
+        // block maximum: across the wave by shuffles (no NaN is left: mon_max
+        // made it +inf), then across the waves in LDS
+#pragma unroll
+        for (int w = 32; w > 0; w >>= 1) {
+            mt = fmax(mt, __shfl_xor(mt, w));
+            mh = fmax(mh, __shfl_xor(mh, w));
+        }
+        constexpr int NW = NT / 64;
+        double* sred = reinterpret_cast<double*>(sH);  // the H tile is no longer read
+        if ((threadIdx.x & 63) == 0) {
+            sred[threadIdx.x >> 6] = mt;
+            sred[NW + (threadIdx.x >> 6)] = mh;
+        }
         __syncthreads();
-        for (int st = NT / 2; st > 0; st >>= 1) {
-            if (threadIdx.x < st) {
-                sred[threadIdx.x] = fmax(sred[threadIdx.x], sred[threadIdx.x + st]);
-                sred[NT + threadIdx.x] = fmax(sred[NT + threadIdx.x], sred[NT + threadIdx.x + st]);
-            }
-            __syncthreads();
-        }
         if (threadIdx.x == 0) {
-            mon_slot(u.mon, blockIdx.x, sred[0], sred[NT]);
+            for (int w = 1; w < NW; ++w) {
+                mt = fmax(mt, sred[w]);
+                mh = fmax(mh, sred[NW + w]);
+            }
+            mon_slot(u.mon, blockIdx.x, mt, mh);
         }
     }
 }

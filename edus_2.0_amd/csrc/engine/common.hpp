@@ -64,6 +64,10 @@ __host__ __device__ inline int gamma_centred(int j, int n) { return (2 * j >= n)
 constexpr int kMonSlots = 256, kMonStride = 16, kMonHalf = kMonSlots * kMonStride;
 constexpr int kMonWords = 2 * kMonHalf;
 
+// running maximum of a monitor: a NaN value becomes +inf, so it trips the
+// monitor instead of being dropped by fmax
+__device__ inline double mon_max(double m, double v) { return (v != v) ? __builtin_huge_val() : fmax(m, v); }
+
 __device__ inline void mon_slot(unsigned long long* mon, unsigned block, double trace_dev, double herm_dev)
 {
     const int o = int(block % kMonSlots) * kMonStride;

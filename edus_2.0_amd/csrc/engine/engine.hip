@@ -494,7 +494,7 @@ void Engine::autotune()
         };
         // the operand sets of the hot uses, so that the candidates see the
         // step's real HBM traffic, not a cache-resident subset (the arrays
-        // hold no state yet; every coefficient is 0).  RK4 middle stage:
+        // hold no state yet; every coefficient is 0).  RK4 third stage:
         // stage input, H0, one xi, D and two stage arrays read, two written.
         CommArgs c{};
         c.Nk = Nk;
@@ -508,8 +508,9 @@ void Engine::autotune()
                 break;
             }
         c.D = D;
-        c.up.x1 = acc;
+        c.up.x1 = acc;  // the RK4 step's heaviest stage (the third)
         c.up.out1 = acc;
+        c.up.g1 = 2. / 3.;
         c.up.x2 = y;
         c.up.out2 = f;
         // Adams-Bashforth step of order 4: state (read and written in place),
@@ -641,41 +642,41 @@ void Engine::evaluate(const double2* in, const double E[3], const StageUpdate& u
     tend(t);
 }
 
-void Engine::monitor(const double2* rho)
+
+// The four stage updates of one classical RK4 step (DESolver.hpp:155-187):
+//   k1 = f(t, y);        aux = y + dt/2 k1;  acc = y + dt/6 k1     (:166-171)
+//   k2 = f(t+dt/2, aux); aux = y + dt/2 k2                         (:174-177)
+//   k3 = f(t+dt/2, aux); aux = y + dt k3;
+//                        acc += 2/3 (aux_in - y) + dt/3 k3         (:174-182)
+//   k4 = f(t+dt, aux);   y = acc + dt/6 k4                         (:184-185)
+// The reference adds dt/3 k2 to acc in stage 2; here stage 3 recovers it as
+// 2/3 (aux_in - y) = dt/3 k2 from the two arrays it reads anyway, so stage 2
+// neither reads nor writes acc (a step moves 26 plane sets instead of 28).
+// Stage 4 carries the per-step trace / Hermiticity monitor of the new y.
+void Engine::rk4_stages(double dt, StageUpdate u[4]) const
 {
-    TimedLaunch t;
-    tbegin("monitor", t);
-    launch_monitor(rho, nb, Nk, layout, double(nf), mon, s);
-    tend(t);
+    for (int i = 0; i < 4; ++i) u[i] = StageUpdate{};
+    u[0].x1 = y; u[0].out1 = acc; u[0].b1 = dt / 6.;
+    u[0].x2 = y; u[0].out2 = aux; u[0].b2 = dt / 2.;
+    u[1].x2 = y; u[1].out2 = aux; u[1].b2 = dt / 2.;
+    u[2].x1 = acc; u[2].out1 = acc; u[2].b1 = dt / 3.; u[2].g1 = 2. / 3.;
+    u[2].x2 = y; u[2].out2 = aux; u[2].b2 = dt;
+    u[3].x1 = acc; u[3].out1 = y; u[3].b1 = dt / 6.;
+    u[3].mon = mon; u[3].nf = double(nf);
 }
 
 void Engine::rk4_step(double dt, const double* E3x3, int record_order)
 {
-    StageUpdate u{};
-    // k1 = f(t, y);   aux = y + dt/2 k1;  acc = y + dt/6 k1      (DESolver.hpp:166-171)
-    u.x1 = y; u.out1 = acc; u.b1 = dt / 6.;
-    u.x2 = y; u.out2 = aux; u.b2 = dt / 2.;
+    StageUpdate u[4];
+    rk4_stages(dt, u);
     if (record_order > 0) {
         ensure_hist();
-        u.out_f = hist[abidx[0]];
+        u[0].out_f = hist[abidx[0]];
     }
-    evaluate(y, E3x3 + 0, u);
-    // k2 = f(t+dt/2, aux); aux = y + dt/2 k2; acc += dt/3 k2     (:174-177)
-    u = StageUpdate{};
-    u.x1 = acc; u.out1 = acc; u.b1 = dt / 3.;
-    u.x2 = y; u.out2 = aux; u.b2 = dt / 2.;
-    evaluate(aux, E3x3 + 3, u);
-    // k3 = f(t+dt/2, aux); aux = y + dt k3; acc += dt/3 k3       (:179-182)
-    u = StageUpdate{};
-    u.x1 = acc; u.out1 = acc; u.b1 = dt / 3.;
-    u.x2 = y; u.out2 = aux; u.b2 = dt;
-    evaluate(aux, E3x3 + 3, u);
-    // k4 = f(t+dt, aux); y = acc + dt/6 k4                       (:184-185)
-    // with the per-step trace / Hermiticity monitor fused into the same kernel
-    u = StageUpdate{};
-    u.x1 = acc; u.out1 = y; u.b1 = dt / 6.;
-    u.mon = mon; u.mon_src = y; u.nf = double(nf);
-    evaluate(aux, E3x3 + 6, u);
+    evaluate(y, E3x3 + 0, u[0]);
+    evaluate(aux, E3x3 + 3, u[1]);
+    evaluate(aux, E3x3 + 3, u[2]);
+    evaluate(aux, E3x3 + 6, u[3]);
     if (record_order > 0) {  // the same rotation as ab_step
         const int tmp = abidx[record_order - 1];
         for (int i = record_order - 1; i > 0; --i) abidx[i] = abidx[i - 1];
@@ -833,23 +834,12 @@ void Engine::evaluate_tab(const double2* in, int ts, int j, const bool use_axis[
 
 void Engine::rk4_step_tab(double dt, int ts, const bool use_axis[2], const bool use_xi[3])
 {
-    // the stage sequence of rk4_step (DESolver.hpp:155-187)
-    StageUpdate u{};
-    u.x1 = y; u.out1 = acc; u.b1 = dt / 6.;
-    u.x2 = y; u.out2 = aux; u.b2 = dt / 2.;
-    evaluate_tab(y, ts, 0, use_axis, use_xi, u);
-    u = StageUpdate{};
-    u.x1 = acc; u.out1 = acc; u.b1 = dt / 3.;
-    u.x2 = y; u.out2 = aux; u.b2 = dt / 2.;
-    evaluate_tab(aux, ts, 1, use_axis, use_xi, u);
-    u = StageUpdate{};
-    u.x1 = acc; u.out1 = acc; u.b1 = dt / 3.;
-    u.x2 = y; u.out2 = aux; u.b2 = dt;
-    evaluate_tab(aux, ts, 1, use_axis, use_xi, u);
-    u = StageUpdate{};
-    u.x1 = acc; u.out1 = y; u.b1 = dt / 6.;
-    u.mon = mon; u.mon_src = y; u.nf = double(nf);
-    evaluate_tab(aux, ts, 2, use_axis, use_xi, u);
+    StageUpdate u[4];
+    rk4_stages(dt, u);
+    evaluate_tab(y, ts, 0, use_axis, use_xi, u[0]);
+    evaluate_tab(aux, ts, 1, use_axis, use_xi, u[1]);
+    evaluate_tab(aux, ts, 1, use_axis, use_xi, u[2]);
+    evaluate_tab(aux, ts, 2, use_axis, use_xi, u[3]);
 }
 
 // F^-1 diag(exp(i sign A.R)) F applied to (in + sum_d addc_d add_d), one
@@ -916,22 +906,12 @@ void Engine::evaluate_peierls(const double2* in, const double E[3], const double
 
 void Engine::rk4_step_peierls(double dt, const double* E3x3, const double* A3x3)
 {
-    StageUpdate u{};
-    u.x1 = y; u.out1 = acc; u.b1 = dt / 6.;
-    u.x2 = y; u.out2 = aux; u.b2 = dt / 2.;
-    evaluate_peierls(y, E3x3 + 0, A3x3 + 0, u);
-    u = StageUpdate{};
-    u.x1 = acc; u.out1 = acc; u.b1 = dt / 3.;
-    u.x2 = y; u.out2 = aux; u.b2 = dt / 2.;
-    evaluate_peierls(aux, E3x3 + 3, A3x3 + 3, u);
-    u = StageUpdate{};
-    u.x1 = acc; u.out1 = acc; u.b1 = dt / 3.;
-    u.x2 = y; u.out2 = aux; u.b2 = dt;
-    evaluate_peierls(aux, E3x3 + 3, A3x3 + 3, u);
-    u = StageUpdate{};
-    u.x1 = acc; u.out1 = y; u.b1 = dt / 6.;
-    u.mon = mon; u.mon_src = y; u.nf = double(nf);
-    evaluate_peierls(aux, E3x3 + 6, A3x3 + 6, u);
+    StageUpdate u[4];
+    rk4_stages(dt, u);
+    evaluate_peierls(y, E3x3 + 0, A3x3 + 0, u[0]);
+    evaluate_peierls(aux, E3x3 + 3, A3x3 + 3, u[1]);
+    evaluate_peierls(aux, E3x3 + 3, A3x3 + 3, u[2]);
+    evaluate_peierls(aux, E3x3 + 6, A3x3 + 6, u[3]);
 }
 
 void Engine::ab_step_peierls(double dt, int order, const double* E3, const double* A3)
@@ -948,7 +928,7 @@ void Engine::ab_step_peierls(double dt, int order, const double* E3, const doubl
         u.ab_c[i] = dt * beta[i];
         u.hist[i] = hist[abidx[i]];
     }
-    u.mon = mon; u.mon_src = y; u.nf = double(nf);
+    u.mon = mon; u.nf = double(nf);
     evaluate_peierls(y, E3, A3, u);
     const int tmp = abidx[order - 1];
     for (int i = order - 1; i > 0; --i) abidx[i] = abidx[i - 1];
@@ -976,7 +956,7 @@ void Engine::ab_step(double dt, int order, const double* E3)
         u.ab_c[i] = dt * beta[i];
         u.hist[i] = hist[abidx[i]];
     }
-    u.mon = mon; u.mon_src = y; u.nf = double(nf);
+    u.mon = mon; u.nf = double(nf);
     evaluate(y, E3, u);
     const int tmp = abidx[order - 1];
     for (int i = order - 1; i > 0; --i) abidx[i] = abidx[i - 1];

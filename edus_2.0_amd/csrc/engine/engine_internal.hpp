@@ -117,6 +117,8 @@ class Engine {
     // record_order > 0: stage 1's right-hand side f(t_n, y_n) is also stored as
     // the newest Adams-Bashforth history entry of that order (RK4 start-up)
     void rk4_step(double dt, const double* E3x3, int record_order = 0);
+    // the stage updates of one RK4 step on y / acc / aux
+    void rk4_stages(double dt, StageUpdate u[4]) const;
     // stacked instances: fields from the device tables at table step `ts`;
     // use_axis / use_xi switch off gradient axes and E.xi terms that are zero
     // for every instance of the chunk
@@ -130,7 +132,6 @@ class Engine {
     int graph_steps = 0;
     Batch* self_batch = nullptr;
     void ab_step(double dt, int order, const double* E3);
-    void monitor(const double2* rho);
     void observables(double* pop_bloch, double* pop_wannier, double* vel, int inst = 0,
                      const double2* rho_override = nullptr);
     void collect_timing();

@@ -47,17 +47,20 @@ struct AxisFFTArgs {
 //   AB:    y  = y + sum_i c_i h_i  with h_0 = f, h_i = hist[i]  (if ab_order)
 struct StageUpdate {
     double2* out_f;
+    // RK: out1 = x1 + g1 (rho - x2) + b1 f and out2 = x2 + b2 f, rho the
+    // stage input (g1 lets a stage recover an earlier stage's b f term from
+    // rho - x2 instead of carrying it through another array)
     const double2* x1; double2* out1; double b1;
     const double2* x2; double2* out2; double b2;
+    double g1;
     int ab_order;
     const double2* hist[5];  // hist[0] unused (= f)
     double ab_c[5];
     double2* y;               // AB target
-    // invariant monitor on the array this stage wrote (mon_src): max over the
-    // tile of |Tr - nf| and of the Hermiticity defect into the work-group's
-    // monitor slots (mon_slot)
+    // invariant monitor on the state this stage writes (RK: out1, AB: y):
+    // max over the tile of |Tr - nf| and of the Hermiticity defect into the
+    // work-group's monitor slots (mon_slot)
     unsigned long long* mon;
-    const double2* mon_src;
     double nf;
 };
 

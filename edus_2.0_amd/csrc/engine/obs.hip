@@ -1,5 +1,5 @@
-// Layout conversion at the C-ABI boundary, observable reductions, invariant
-// monitors and the small element-wise kernels of the Hartree-Fock term.
+// Layout conversion at the C-ABI boundary, observable reductions and the
+// small element-wise kernels of the Hartree-Fock term.
 #include "common.hpp"
 #include "obs.hpp"
 
@@ -206,50 +206,6 @@ __global__ void __launch_bounds__(kObsBlock) hartree_kernel(const double2* __res
     }
 }
 
-// Invariant monitors: max_k |Tr rho(k) - nf| and the Hermiticity defect
-// max_k max_{n<=m} |rho_nm - conj(rho_mn)| (HERMITIAN layout: the diagonal's
-// imaginary part, the only part storage does not fix), folded into two device
-// maxima.  Bit patterns of non-negative doubles order like the values, so an
-// integer atomicMax is exact and independent of order.
-__global__ void __launch_bounds__(kObsBlock) monitor_kernel(const double2* __restrict__ rho, int nb, int Nk,
-                                                            int layout, double nf, unsigned long long* __restrict__ mon)
-{
-    __shared__ double s_tr[kObsBlock], s_h[kObsBlock];
-    double mt = 0., mh = 0.;
-    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < Nk; k += gridDim.x * blockDim.x) {
-        double2 tr = make_double2(-nf, 0.);
-        for (int n = 0; n < nb; ++n) {
-            const double2 v = rho[long(diag_plane(n, nb, layout)) * Nk + k];
-            tr = cadd(tr, v);
-            mh = fmax(mh, 2.0 * fabs(v.y));
-        }
-        if (layout == kFull)
-            for (int n = 0; n < nb; ++n)
-                for (int m = n + 1; m < nb; ++m) {
-                    const double2 a = rho[long(n * nb + m) * Nk + k];
-                    const double2 b = rho[long(m * nb + n) * Nk + k];
-                    mh = fmax(mh, sqrt((a.x - b.x) * (a.x - b.x) + (a.y + b.y) * (a.y + b.y)));
-                }
-        mt = fmax(mt, sqrt(tr.x * tr.x + tr.y * tr.y));
-    }
-    s_tr[threadIdx.x] = mt;
-    s_h[threadIdx.x] = mh;
-    __syncthreads();
-    for (int st = kObsBlock / 2; st > 0; st >>= 1) {
-        if (threadIdx.x < st) {
-            s_tr[threadIdx.x] = fmax(s_tr[threadIdx.x], s_tr[threadIdx.x + st]);
-            s_h[threadIdx.x] = fmax(s_h[threadIdx.x], s_h[threadIdx.x + st]);
-        }
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-        double a = s_tr[0], b = s_h[0];
-        if (a != a) a = __builtin_huge_val();  // NaN trips the monitor
-        if (b != b) b = __builtin_huge_val();
-        mon_slot(mon, blockIdx.x, a, b);
-    }
-}
-
 __global__ void axpy_planes_kernel(double2* __restrict__ y, const double2* __restrict__ x, double a, long n)
 {
     const long i = long(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -310,11 +266,6 @@ void launch_hartree(const double2* rho, int nb, int Nk, int layout, int P, int B
     hipLaunchKernelGGL(diag_sum_kernel, dim3(nblk, nb, B), dim3(kObsBlock), 0, s, rho, nb, Nk, layout, P, partial);
     hipLaunchKernelGGL(hartree_kernel, dim3(B), dim3(kObsBlock), 0, s, partial, nblk, hart, nb, 1.0 / double(Nk),
                        hdiag);
-}
-void launch_monitor(const double2* rho, int nb, int Nk, int layout, double nf, unsigned long long* mon, hipStream_t s)
-{
-    const int blocks = int(std::min<long>(grid1d(Nk, kObsBlock), 512));
-    hipLaunchKernelGGL(monitor_kernel, dim3(blocks), dim3(kObsBlock), 0, s, rho, nb, Nk, layout, nf, mon);
 }
 void launch_axpy(double2* y, const double2* x, double a, long n, hipStream_t s)
 {

@@ -1,4 +1,4 @@
-// Launchers of the conversion, observable and monitor kernels.
+// Launchers of the conversion and observable kernels.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -24,8 +24,6 @@ int diag_blocks(int Nk);
 // Hartree shift of B stacked instances ([B][P][Nk]); partial [B][nb][diag_blocks]
 void launch_hartree(const double2* rho, int nb, int Nk, int layout, int P, int B, const double* hart,
                     double2* partial, double2* hdiag, hipStream_t s);
-void launch_monitor(const double2* rho, int nb, int Nk, int layout, double nf, unsigned long long* mon,
-                    hipStream_t s);
 void launch_axpy(double2* y, const double2* x, double a, long n, hipStream_t s);
 void launch_add_diag(double2* H, const double2* d, double scale, int nb, int Nk, int layout, hipStream_t s);
 
