@@ -155,7 +155,8 @@ int edus_graph(edus_ctx* ctx, int steps_per_graph);
 /* Kernel timing with HIP events on the context stream.  When enabled, every
  * launch of the named kernel class is bracketed by events; edus_timing_get
  * returns the summed milliseconds and launch count since the last reset.
- * Classes: "grad_col", "grad_row", "comm", "hf_row", "hf_col", "monitor". */
+ * Classes: "grad_col", "grad_row", "kgrad", "comm", "hf_row", "hf_col",
+ * "peierls_col", "peierls_row". */
 int edus_timing_enable(edus_ctx* ctx, int on);
 int edus_timing_get(edus_ctx* ctx, const char* kernel_class, double* total_ms, long* count);
 
