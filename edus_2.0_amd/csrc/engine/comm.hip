@@ -37,6 +37,13 @@ typedef float flt4 __attribute__((ext_vector_type(4)));
 
 constexpr int kCommThreads = 256;
 
+// work-group size: 256 threads, or 512 for MFMA tile 2 (one k-point per wave)
+template <int KIND, int TILE>
+constexpr int comm_threads()
+{
+    return (KIND != kCommScalar && TILE == 2) ? 2 * kCommThreads : kCommThreads;
+}
+
 template <int NB, int LAYOUT>
 __device__ inline void decode_plane(int p, int& n, int& m)
 {
@@ -85,12 +92,13 @@ constexpr int scalar_tile()
 
 // MFMA engines: tile 0 holds 8 k-points (two per wave; 8 x 16 B = one
 // 128-byte line per plane), tile 1 holds 4 (one per wave; half the LDS, so
-// more work-groups per CU); plane stride padded by one against LDS bank
+// more work-groups per CU), tile 2 holds 8 with 512 threads (one per wave;
+// half the elements per thread); plane stride padded by one against LDS bank
 // conflicts.  The per-k arithmetic does not depend on the tile.
 template <int NB, int LAYOUT, int KIND, int TILE>
 constexpr int tile_k()
 {
-    return KIND == kCommScalar ? scalar_tile<NB, LAYOUT>() : (TILE == 0 ? 8 : 4);
+    return KIND == kCommScalar ? scalar_tile<NB, LAYOUT>() : (TILE == 1 ? 4 : 8);
 }
 
 template <int KIND, int TK>
@@ -114,7 +122,7 @@ constexpr size_t comm_lds_bytes()
     // rho and H tiles plus the Hartree diagonal; the monitor's block
     // reduction reuses the tile: 2 doubles per thread
     return std::max((size_t(2) * tile_slots<NB, LAYOUT, KIND, TILE>() + kMaxNb) * sizeof(double2),
-                    size_t(2) * kCommThreads * sizeof(double));
+                    size_t(2) * comm_threads<KIND, TILE>() * sizeof(double));
 }
 
 // element (r, c) of a tile in LDS (zero outside the nb x nb block)
@@ -199,9 +207,9 @@ __device__ inline void fold_g1(double g1, const double2* sR, const int (&slot_of
 }
 
 template <int NB, int LAYOUT, int KIND, int TILE, bool AB>
-__global__ void __launch_bounds__(kCommThreads) comm_kernel(CommArgs a)
+__global__ void __launch_bounds__((comm_threads<KIND, TILE>())) comm_kernel(CommArgs a)
 {
-    constexpr int NT = kCommThreads;
+    constexpr int NT = comm_threads<KIND, TILE>();
     constexpr int NPL = npl_of<NB, LAYOUT>();
     constexpr int TK = tile_k<NB, LAYOUT, KIND, TILE>();
     constexpr int PS = plane_stride<KIND, TK>();
@@ -398,6 +406,7 @@ __global__ void __launch_bounds__(kCommThreads) comm_kernel(CommArgs a)
         // registers until all waves are done reading rho and H, then replace
         // rho in LDS
         constexpr int KPW = TK / (NT / 64);
+        static_assert(KPW >= 1 && KPW * (NT / 64) == TK, "every wave takes the same number of k-points");
         const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
         using acc_t = typename std::conditional<KIND == kCommMfma64, dbl4, flt4>::type;
         acc_t cre[KPW], cim[KPW];
@@ -566,7 +575,8 @@ void launch_one(const CommArgs& a, hipStream_t s)
                                   int(lds));
         raised = true;
     }
-    hipLaunchKernelGGL(kern, dim3(((a.Nk + TK - 1) / TK) * std::max(a.B, 1)), dim3(kCommThreads), lds, s, a);
+    hipLaunchKernelGGL(kern, dim3(((a.Nk + TK - 1) / TK) * std::max(a.B, 1)), dim3(comm_threads<KIND, TILE>()), lds,
+                       s, a);
 }
 
 template <int NB, int LAYOUT, int KIND, int TILE>
@@ -584,11 +594,15 @@ void launch_layout(const CommArgs& a, int kind, int tile, hipStream_t s)
     if (kind == kCommMfma64) {
         if (tile == 1)
             launch_kind<NB, LAYOUT, kCommMfma64, 1>(a, s);
+        else if (tile == 2)
+            launch_kind<NB, LAYOUT, kCommMfma64, 2>(a, s);
         else
             launch_kind<NB, LAYOUT, kCommMfma64, 0>(a, s);
     } else if (kind == kCommMfma32) {
         if (tile == 1)
             launch_kind<NB, LAYOUT, kCommMfma32, 1>(a, s);
+        else if (tile == 2)
+            launch_kind<NB, LAYOUT, kCommMfma32, 2>(a, s);
         else
             launch_kind<NB, LAYOUT, kCommMfma32, 0>(a, s);
     } else {

@@ -470,9 +470,9 @@ void Engine::autotune()
     } else {
         std::vector<std::pair<int, int>> cands;
         if (prec == EDUS_PREC_MIXED)
-            cands = {{kCommMfma32, 0}, {kCommMfma32, 1}};
+            cands = {{kCommMfma32, 0}, {kCommMfma32, 1}, {kCommMfma32, 2}};
         else
-            cands = {{kCommScalar, 0}, {kCommMfma64, 0}, {kCommMfma64, 1}};
+            cands = {{kCommScalar, 0}, {kCommMfma64, 0}, {kCommMfma64, 1}, {kCommMfma64, 2}};
         if (force_kind >= 0) {  // keep the members' engine: only the tile is free
             std::vector<std::pair<int, int>> keep;
             for (const auto& kt : cands)
@@ -1226,8 +1226,8 @@ int edus_set_tuning(edus_ctx* ctx, const edus_tuning* in)
                                       : (in->comm_kind != edus::kCommScalar && in->comm_kind != edus::kCommMfma64))
             throw EngineError(EDUS_ERR_ARG, "comm_kind does not match the context precision");
         for (int t : {in->comm_tile, in->comm_tile_ab})
-            if (t != 0 && (t != 1 || in->comm_kind == edus::kCommScalar))
-                throw EngineError(EDUS_ERR_ARG, "comm_tile and comm_tile_ab must be 0, or 1 for the MFMA engines");
+            if (t != 0 && (t < 1 || t > 2 || in->comm_kind == edus::kCommScalar))
+                throw EngineError(EDUS_ERR_ARG, "comm_tile and comm_tile_ab must be 0, or 1 or 2 for the MFMA engines");
         e.fft_choice[0] = next[0];
         e.fft_choice[1] = next[1];
         e.comm_kind = in->comm_kind;

@@ -171,8 +171,9 @@ int edus_timing_get(edus_ctx* ctx, const char* kernel_class, double* total_ms, l
  *   fft_lines[a]   lines per work-group
  *   comm_kind      0: FP64 FMAs, 1: FP64 MFMA (EDUS_PREC_FP64);
  *                  2: FP32 MFMA (EDUS_PREC_MIXED, not settable)
- *   comm_tile      k-points per commutator work-group: 0: 8, 1: 4 (MFMA
- *                  engines; the FMA engine takes 0)
+ *   comm_tile      commutator work-group: 0: 8 k-points, 256 threads;
+ *                  1: 4 k-points, 256 threads; 2: 8 k-points, 512 threads
+ *                  (1 and 2: MFMA engines only; the FMA engine takes 0)
  *   comm_tile_ab   the same for the Adams-Bashforth steps
  * Within one family, FFT tiles give bit-identical results, and so do the
  * commutator tiles; the two FP64 commutator engines differ in summation order

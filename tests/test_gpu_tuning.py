@@ -100,7 +100,7 @@ def test_fp64_commutator_engines(model_dir, name, grid, layout):
     f_ref = o.rhs_E(E, rho)
     scale = max(1.0, np.abs(f_ref).max())
     outs = {}
-    for kind, tile in ((0, 0), (1, 0), (1, 1)):
+    for kind, tile in ((0, 0), (1, 0), (1, 1), (1, 2)):
         t = eng.get_tuning()
         t.comm_kind = kind
         t.comm_tile = tile
@@ -113,6 +113,7 @@ def test_fp64_commutator_engines(model_dir, name, grid, layout):
         outs[(kind, tile)] = f
     assert np.abs(outs[(0, 0)] - outs[(1, 0)]).max() <= 1e-13 * scale
     assert np.array_equal(outs[(1, 0)], outs[(1, 1)])  # tiles: same arithmetic per k-point
+    assert np.array_equal(outs[(1, 0)], outs[(1, 2)])
 
 
 @pytest.mark.parametrize("name,grid", [("hsex8_hex90", [9, 11]), ("ipa14_hex120", [10, 9])])
@@ -126,7 +127,7 @@ def test_adams_bashforth_tiles_identical(model_dir, name, grid, order):
     o = oracle_py.Oracle(p)
     coul = bool(p["coulomb"])
     states = []
-    for tile in (0, 1):
+    for tile in (0, 1, 2):
         eng = _engine(o, FULL, p["filledbands"], coulomb=coul)
         eng.set_state(o.rho)
         t = eng.get_tuning()
@@ -140,6 +141,7 @@ def test_adams_bashforth_tiles_identical(model_dir, name, grid, order):
         tr, herm = eng.check()
         _say(name, grid, "AB", order, "tile", tile, "monitors", tr, herm)
     assert np.array_equal(states[0], states[1])
+    assert np.array_equal(states[0], states[2])
 
 
 @pytest.mark.parametrize("name,grid", [("ipa2_square30", [12, 15]), ("ipa14_hex120", [10, 9]),
@@ -157,6 +159,11 @@ def test_mixed_precision_rhs_error(model_dir, name, grid):
     rel = np.abs(f - f_ref).max() / np.abs(f_ref).max()
     _say(name, grid, "mixed precision: max|df| / max|f| =", rel)
     assert rel <= 2e-6
+    for tile in (0, 1, 2):  # the FP32 engine's tiles: same arithmetic per k-point
+        t = eng.get_tuning()
+        t.comm_tile = tile
+        eng.set_tuning(t)
+        assert np.array_equal(eng.rhs(E, rho), f)
 
 
 def test_mixed_precision_propagation_error():
