@@ -76,20 +76,25 @@ def cpu_baseline(params: dict, steps: int, integ: str, target_s: float = 15.0):
                       f"{steps} {integ} steps after warm-up, {dt:.2f} s"}
 
 
-def measured_traffic(cls: str, workload: str, layout: str, precision: str):
+def measured_traffic(cls: str, workload: str, layout: str, precision: str, gradient: str = "R",
+                     gauge: str = "length"):
     """HBM bytes per launch of a kernel class from the committed rocprofv3
     counter summaries under profiles/ (FETCH_SIZE and WRITE_SIZE runs scaled
-    by the copy-kernel calibration); None when no summary covers the case."""
+    by the copy-kernel calibration); None when no summary covers the case.
+    A summary without a gradient= or gauge= key covers the defaults (the
+    spectral gradient, the length gauge) only."""
     import glob
 
-    want = {"class": cls, "workload": workload, "layout": layout, "precision": precision}
+    want = {"class": cls, "workload": workload, "layout": layout, "precision": precision,
+            "gradient": gradient, "gauge": gauge}
+    default = {"gradient": "R", "gauge": "length"}
     for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.txt"))):
         with open(path) as f:
             for line in f:
                 if not line.startswith("traffic:"):
                     continue
                 kv = dict(x.split("=", 1) for x in line.split()[1:])
-                if all(kv.get(k) == v for k, v in want.items()):
+                if all(kv.get(k, default.get(k)) == v for k, v in want.items()):
                     return float(kv["bytes_per_launch"]), os.path.relpath(path, ROOT)
     return None, None
 
@@ -196,7 +201,8 @@ def main():
     avg_s = ms / 1e3 / max(cnt, 1)
     byts = kb[dom]
     achieved = byts / avg_s / 1e9
-    traffic, traffic_src = measured_traffic(dom, args.workload, args.layout, args.precision)
+    traffic, traffic_src = measured_traffic(dom, args.workload, args.layout, args.precision, args.gradient_space,
+                                            "peierls" if args.peierls else "length")
     roofline = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
                 "bytes_per_launch": byts,

@@ -1,0 +1,43 @@
+"""bench.py host logic (no GPU): the algorithmic byte model and the choice of
+the committed profile summary whose measured traffic a run reports."""
+import os
+import subprocess
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, ROOT)
+
+bench = pytest.importorskip("bench")
+
+
+def test_comm_bytes_per_stage_model():
+    """RK4 stages move (6, 6, 8, 6) S at n_E = 1: 6.5 S on average; AB4 with
+    n_E = 2 and exchange: 4 + 2 + 4 + 1 = 11 S; Peierls: 4.5 S."""
+    npl, nk = 196, 14400
+    s = 16.0 * npl * nk
+    assert bench.kernel_bytes("comm", npl, nk, 14, 1) == pytest.approx(6.5 * s)
+    assert bench.kernel_bytes("comm", 64, 8100, 8, 2, "AB", 4, True) == pytest.approx(11 * 16.0 * 64 * 8100)
+    assert bench.kernel_bytes("comm", npl, nk, 14, 1, peierls=True) == pytest.approx(4.5 * s)
+    assert bench.kernel_bytes("grad_col", npl, nk, 14, 1) == pytest.approx(2 * s)
+    assert bench.kernel_bytes("grad_row", npl, nk, 14, 1) == pytest.approx(3 * s)
+
+
+def test_traffic_summary_matches_the_run_variant():
+    t, src = bench.measured_traffic("comm", "ipa14_hex120", "full", "fp64")
+    assert src == os.path.join("profiles", "comm_kernel_ipa14_hex120_rk4_full_fp64.txt")
+    assert t == pytest.approx(bench.kernel_bytes("comm", 196, 14400, 14, 1), rel=0.05)
+    # a Peierls or k-gradient run is not covered by the spectral-gradient summary
+    assert bench.measured_traffic("comm", "ipa14_hex120", "full", "fp64", gauge="peierls") == (None, None)
+    assert bench.measured_traffic("comm", "ipa14_hex120", "full", "fp64", gradient="k") == (None, None)
+    t, src = bench.measured_traffic("kgrad", "ipa14_hex120", "full", "fp64", gradient="k")
+    assert src is not None and t == pytest.approx(2 * 16.0 * 196 * 14400, rel=0.05)
+    assert bench.measured_traffic("comm", "ipa14_hex120", "hermitian", "fp64") == (None, None)
+
+
+def test_more_than_one_gpu_is_refused():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode != 0
+    assert "one GPU" in (r.stdout + r.stderr)
