@@ -28,12 +28,17 @@ def test_traffic_summary_matches_the_run_variant():
     t, src = bench.measured_traffic("comm", "ipa14_hex120", "full", "fp64")
     assert src == os.path.join("profiles", "comm_kernel_ipa14_hex120_rk4_full_fp64.txt")
     assert t == pytest.approx(bench.kernel_bytes("comm", 196, 14400, 14, 1), rel=0.05)
-    # a Peierls or k-gradient run is not covered by the spectral-gradient summary
-    assert bench.measured_traffic("comm", "ipa14_hex120", "full", "fp64", gauge="peierls") == (None, None)
+    # each variant reads its own summary, or none
+    t, src = bench.measured_traffic("comm", "ipa14_hex120", "full", "fp64", gauge="peierls")
+    assert src.endswith("_peierls.txt")
+    assert t == pytest.approx(bench.kernel_bytes("comm", 196, 14400, 14, 1, peierls=True), rel=0.05)
     assert bench.measured_traffic("comm", "ipa14_hex120", "full", "fp64", gradient="k") == (None, None)
     t, src = bench.measured_traffic("kgrad", "ipa14_hex120", "full", "fp64", gradient="k")
     assert src is not None and t == pytest.approx(2 * 16.0 * 196 * 14400, rel=0.05)
-    assert bench.measured_traffic("comm", "ipa14_hex120", "hermitian", "fp64") == (None, None)
+    assert bench.measured_traffic("kgrad", "ipa14_hex120", "full", "fp64") == (None, None)
+    t, src = bench.measured_traffic("comm", "ipa14_hex120", "hermitian", "fp64")
+    assert t == pytest.approx(bench.kernel_bytes("comm", 105, 14400, 14, 1), rel=0.05)
+    assert bench.measured_traffic("comm", "ipa14_hex120", "full", "mixed") == (None, None)
 
 
 def test_more_than_one_gpu_is_refused():
