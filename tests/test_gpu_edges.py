@@ -57,13 +57,19 @@ def test_rhs_edge_shapes(tmp_path, nb, grid, kind, layout):
         a = rng.standard_normal((o.Nk, nb, nb)) + 1j * rng.standard_normal((o.Nk, nb, nb))
         rho = 0.5 * (a + a.conj().transpose(0, 2, 1))
         f_ref = o.rhs_E(E, rho)
-        f = eng.rhs(E, rho)
-        if layout == HERM:
-            iu = np.triu_indices(nb)
-            f, f_ref = f[:, iu[0], iu[1]], f_ref[:, iu[0], iu[1]]
-        err, scale = np.abs(f - f_ref).max(), max(1.0, np.abs(f_ref).max())
-        _say("  E", E, "max|df|", err, "scale", scale)
-        assert err <= 1e-11 * scale
+        # every FP64 commutator engine and tile at this shape
+        for kind, tile in ((0, 0), (1, 0), (1, 1), (1, 2)):
+            t = eng.get_tuning()
+            t.comm_kind, t.comm_tile, t.comm_tile_ab = kind, tile, tile
+            eng.set_tuning(t)
+            f = eng.rhs(E, rho)
+            fr = f_ref
+            if layout == HERM:
+                iu = np.triu_indices(nb)
+                f, fr = f[:, iu[0], iu[1]], f_ref[:, iu[0], iu[1]]
+            err, scale = np.abs(f - fr).max(), max(1.0, np.abs(fr).max())
+            _say("  E", E, "engine", kind, "tile", tile, "max|df|", err, "scale", scale)
+            assert err <= 1e-11 * scale
 
 
 @pytest.mark.parametrize("nb,grid,kind", [c for c in CASES if c[0] != 3] + [(3, [3, 4], "hex")])
