@@ -32,13 +32,17 @@ def test_traffic_summary_matches_the_run_variant():
     t, src = bench.measured_traffic("comm", "ipa14_hex120", "full", "fp64", gauge="peierls")
     assert src.endswith("_peierls.txt")
     assert t == pytest.approx(bench.kernel_bytes("comm", 196, 14400, 14, 1, peierls=True), rel=0.05)
-    assert bench.measured_traffic("comm", "ipa14_hex120", "full", "fp64", gradient="k") == (None, None)
+    t, src = bench.measured_traffic("comm", "ipa14_hex120", "full", "fp64", gradient="k")
+    assert src.endswith("_kgradient.txt")
     t, src = bench.measured_traffic("kgrad", "ipa14_hex120", "full", "fp64", gradient="k")
     assert src is not None and t == pytest.approx(2 * 16.0 * 196 * 14400, rel=0.05)
     assert bench.measured_traffic("kgrad", "ipa14_hex120", "full", "fp64") == (None, None)
     t, src = bench.measured_traffic("comm", "ipa14_hex120", "hermitian", "fp64")
     assert t == pytest.approx(bench.kernel_bytes("comm", 105, 14400, 14, 1), rel=0.05)
-    assert bench.measured_traffic("comm", "ipa14_hex120", "full", "mixed") == (None, None)
+    t, src = bench.measured_traffic("comm", "ipa14_hex120", "full", "mixed")
+    assert src.endswith("_mixed.txt")
+    assert bench.measured_traffic("comm", "hsex8_hex90", "hermitian", "fp64") == (None, None)
+    assert bench.measured_traffic("comm", "ipa14_hex120", "full", "fp64", gradient="k", gauge="peierls") == (None, None)
 
 
 def test_more_than_one_gpu_is_refused():
