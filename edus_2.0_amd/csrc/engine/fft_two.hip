@@ -93,19 +93,28 @@ __global__ void __launch_bounds__(NT) two_step_kernel(AxisFFTArgs a)
     const int Nk = a.N0 * a.N1;
     const int tid = threadIdx.x;
 
+    // Work-groups are dealt to the 8 XCDs round-robin (block b runs on XCD
+    // b % 8); tile tb gives each XCD a contiguous range of tiles, so that
+    // neighbouring column groups of a plane run side by side in one XCD.
+    int tb;
+    {
+        const int nblk = gridDim.x, q = nblk / 8, r = nblk % 8;
+        const int x = blockIdx.x % 8, i = blockIdx.x / 8;
+        tb = x * q + min(x, r) + i;
+    }
     long base, cstride, rowstride;
     long row0 = 0, plane0 = 0;  // first global row (axis 1), plane (axis 0)
     int nlines;
     if (a.axis == 1) {
-        row0 = long(blockIdx.x) * L;
+        row0 = long(tb) * L;
         nlines = int(min(long(L), long(a.P) * a.N0 - row0));
         base = row0 * a.N1;
         cstride = a.N1;
         rowstride = 1;
     } else {
         const int nblk = (a.N1 + L - 1) / L;
-        plane0 = blockIdx.x / nblk;
-        const int first = int(blockIdx.x - plane0 * nblk) * L;
+        plane0 = tb / nblk;
+        const int first = int(tb - plane0 * nblk) * L;
         nlines = min(L, a.N1 - first);
         base = plane0 * Nk + first;
         cstride = 1;
