@@ -220,17 +220,11 @@ __global__ void __launch_bounds__((comm_threads<KIND, TILE>())) comm_kernel(Comm
     double2* sH = lds + NSLOT;
     double2* sHd = lds + 2 * NSLOT;
     const int Nk = a.Nk;
-    // Work-groups are dealt to the 8 XCDs round-robin (block b runs on XCD
-    // b % 8); give each XCD a contiguous range of tiles so that tiles sharing
-    // a cache line (TK = 4) meet in the same L2.  t covers the tiles of
-    // instance 0, then instance 1, ...
+    // each XCD takes a contiguous range of tiles, so that tiles sharing a
+    // cache line (TK = 4) meet in the same L2; t covers the tiles of instance
+    // 0, then instance 1, ...
     const int ntile = (Nk + TK - 1) / TK;
-    int t;
-    {
-        const int nblk = gridDim.x, q = nblk / 8, r = nblk % 8;
-        const int x = blockIdx.x % 8, i = blockIdx.x / 8;
-        t = x * q + min(x, r) + i;
-    }
+    const int t = xcd_contiguous_block();
     const int inst = t / ntile;
     const int k0 = (t - inst * ntile) * TK;
     const long ioff = long(inst) * NPL * Nk;

@@ -64,6 +64,16 @@ __host__ __device__ inline int gamma_centred(int j, int n) { return (2 * j >= n)
 constexpr int kMonSlots = 256, kMonStride = 16, kMonHalf = kMonSlots * kMonStride;
 constexpr int kMonWords = 2 * kMonHalf;
 
+// Work-groups are dealt to the 8 XCDs round-robin (block b runs on XCD b % 8).
+// This index gives each XCD a contiguous range of [0, gridDim.x), so that
+// neighbouring tiles run side by side in one XCD (and its L2).
+__device__ inline int xcd_contiguous_block()
+{
+    const int nblk = gridDim.x, q = nblk / 8, r = nblk % 8;
+    const int x = blockIdx.x % 8, i = blockIdx.x / 8;
+    return x * q + min(x, r) + i;
+}
+
 // running maximum of a monitor: a NaN value becomes +inf, so it trips the
 // monitor instead of being dropped by fmax
 __device__ inline double mon_max(double m, double v) { return (v != v) ? __builtin_huge_val() : fmax(m, v); }

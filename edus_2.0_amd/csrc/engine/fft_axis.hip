@@ -196,19 +196,20 @@ __global__ void __launch_bounds__(NT) axis_fft_kernel(AxisFFTArgs a)
     const int Nk = a.N0 * a.N1;
 
     // global offset of value j of line c: base + c * cstride + j * rowstride
+    const int tb = xcd_contiguous_block();  // neighbouring line groups side by side in one XCD
     long base, cstride, rowstride;
     long row0 = 0, plane0 = 0;  // first global row (axis 1), plane (axis 0)
     int nlines;
     if (a.axis == 1) {
-        row0 = long(blockIdx.x) * L;
+        row0 = long(tb) * L;
         nlines = int(min(long(L), long(a.P) * a.N0 - row0));
         base = row0 * a.N1;
         cstride = a.N1;
         rowstride = 1;
     } else {
         const int nblk = (a.N1 + L - 1) / L;
-        plane0 = blockIdx.x / nblk;
-        const int first = int(blockIdx.x - plane0 * nblk) * L;
+        plane0 = tb / nblk;
+        const int first = int(tb - plane0 * nblk) * L;
         nlines = min(L, a.N1 - first);
         base = plane0 * Nk + first;
         cstride = 1;

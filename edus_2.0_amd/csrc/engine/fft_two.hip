@@ -93,15 +93,7 @@ __global__ void __launch_bounds__(NT) two_step_kernel(AxisFFTArgs a)
     const int Nk = a.N0 * a.N1;
     const int tid = threadIdx.x;
 
-    // Work-groups are dealt to the 8 XCDs round-robin (block b runs on XCD
-    // b % 8); tile tb gives each XCD a contiguous range of tiles, so that
-    // neighbouring column groups of a plane run side by side in one XCD.
-    int tb;
-    {
-        const int nblk = gridDim.x, q = nblk / 8, r = nblk % 8;
-        const int x = blockIdx.x % 8, i = blockIdx.x / 8;
-        tb = x * q + min(x, r) + i;
-    }
+    const int tb = xcd_contiguous_block();  // neighbouring line groups side by side in one XCD
     long base, cstride, rowstride;
     long row0 = 0, plane0 = 0;  // first global row (axis 1), plane (axis 0)
     int nlines;
