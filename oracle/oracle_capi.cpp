@@ -1,6 +1,8 @@
 // C entry points of the FP64 CPU oracle -- TEST INFRASTRUCTURE ONLY.
 // Loaded by tests/ (ctypes), by the smoke check and by bench.py's CPU-baseline
 // leg; never by the product library.
+#include <chrono>
+#include <memory>
 #include <cstring>
 #include <exception>
 #include <string>
@@ -105,6 +107,13 @@ static std::vector<cplx>* named(Simulation* s, const char* name)
     if (n == "Wscr") return &s->Wscr;
     if (n == "Vbare") return &s->Vbare;
     if (n == "hartree") return &s->hartree;
+    // Adams-Bashforth history slots aux[i] (DESolver.hpp:45; the step reads
+    // f_{n-i} from aux[abindex[i]], abindex = identity before the first step)
+    if (n.size() == 5 && n.compare(0, 4, "hist") == 0 && n[4] >= '0' && n[4] <= '4') {
+        auto& v = s->aux[n[4] - '0'];
+        if (v.empty()) v.assign(size_t(s->Nk) * s->M, 0.);
+        return &v;
+    }
     return nullptr;
 }
 
@@ -306,6 +315,29 @@ int oracle_kgradient(void* h, const double* u, const double* in, double* out)
     sim->kgradient(o, a, u);
     std::copy(o.begin(), o.end(), reinterpret_cast<cplx*>(out));
     return 0;
+}
+
+// Host memory bandwidth in bytes/s (bytes read + written) of a parallel copy
+// between two arrays of `bytes` each, the best of `reps` passes after one
+// untimed pass: the bandwidth context for the CPU baseline's byte model.
+double oracle_host_bandwidth(long bytes, int reps)
+{
+    const size_t n = size_t(bytes) / sizeof(double);
+    std::unique_ptr<double[]> pa(new double[n]), pb(new double[n]);  // left uninitialised here
+    double* a = pa.get();
+    double* b = pb.get();
+#pragma omp parallel for schedule(static)
+    for (size_t i = 0; i < n; ++i) a[i] = double(i & 1023), b[i] = 0.0;  // first touch by the copying thread
+    double best = 0.0;
+    for (int r = 0; r <= reps; ++r) {
+        const auto t0 = std::chrono::steady_clock::now();
+#pragma omp parallel for schedule(static)
+        for (size_t i = 0; i < n; ++i) b[i] = a[i];
+        const double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        if (r > 0 && s > 0) best = std::max(best, 2.0 * double(n) * sizeof(double) / s);
+        std::swap(a, b);
+    }
+    return best;
 }
 
 }  // extern "C"

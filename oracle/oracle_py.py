@@ -90,6 +90,8 @@ def lib():
         L.oracle_fft2d.argtypes = [C.c_int, C.c_int, dp, C.c_int]
         L.oracle_struve0.restype = C.c_double
         L.oracle_struve0.argtypes = [C.c_double]
+        L.oracle_host_bandwidth.restype = C.c_double
+        L.oracle_host_bandwidth.argtypes = [C.c_long, C.c_int]
         _LIB = L
     return _LIB
 
@@ -303,3 +305,9 @@ def lindhard(U: np.ndarray, eps: np.ndarray, N0: int, N1: int, filled: int) -> n
     if lib().oracle_lindhard(nb, N0, N1, int(filled), _ptr(U.view(np.float64)), _ptr(e), _ptr(X.view(np.float64))) != 0:
         raise RuntimeError(lib().oracle_last_error().decode())
     return X
+
+
+def host_bandwidth(nbytes: int = 1 << 30, reps: int = 3) -> float:
+    """Host memory bandwidth (bytes read + written per second) of an OpenMP
+    parallel copy between two arrays of nbytes each."""
+    return float(lib().oracle_host_bandwidth(int(nbytes), int(reps)))
