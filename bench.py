@@ -52,28 +52,42 @@ def kernel_bytes(cls: str, npl: int, nk: int, nb: int, n_e: int, solver: str = "
             "peierls_row": 2 * s}.get(cls, 0.0)
 
 
-def cpu_baseline(params: dict, steps: int, integ: str, target_s: float = 15.0):
+def cpu_baseline(params: dict, steps: int, integ: str, bytes_per_kstep: float = 0.0, budget_s: float = 18.0):
     """The FP64 CPU restatement (oracle/) on this host's cores, same workload
-    and grid.  steps <= 0: as many steps as fill about target_s seconds
-    (timed from one step after the warm-up step)."""
+    and grid: warm-up steps, then three timed runs, the median of which is
+    reported (steps <= 0: each run as many steps as fill budget_s / 3).  The
+    host's copy bandwidth is measured beside it, with the rate the byte model
+    (bytes_per_kstep per k-point and step) allows at that bandwidth."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import statistics
+
     import oracle_py
 
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
     os.environ["OMP_NUM_THREADS"] = str(threads)
     o = oracle_py.Oracle(params)
-    o.step(1)  # warm-up
-    if steps <= 0:
-        t0 = time.perf_counter()
-        o.step(1)
-        one = time.perf_counter() - t0
-        steps = int(max(3, min(2000, target_s / max(one, 1e-6))))
     t0 = time.perf_counter()
-    o.step(steps)
-    dt = time.perf_counter() - t0
-    return {"value": o.Nk * steps / dt, "unit": f"k-point·{integ}-steps/s", "cores": threads, "kind": "port",
-            "sample": f"oracle/ FP64 restatement (3 FFTs + 2 zgemm per k per RHS), same workload and grid, "
-                      f"{steps} {integ} steps after warm-up, {dt:.2f} s"}
+    o.step(1)
+    one = time.perf_counter() - t0
+    warm = int(max(1, min(20, 3.0 / max(one, 1e-6))))
+    o.step(warm - 1)
+    if steps <= 0:
+        steps = int(max(3, min(2000, budget_s / 3 / max(one, 1e-6))))
+    runs = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        o.step(steps)
+        runs.append(time.perf_counter() - t0)
+    med = statistics.median(runs)
+    bw = oracle_py.host_bandwidth(1 << 28, 3)
+    out = {"value": o.Nk * steps / med, "unit": f"k-point·{integ}-steps/s", "cores": threads, "kind": "port",
+           "sample": f"oracle/ FP64 restatement (3 FFTs + 2 zgemm per k per RHS), same workload and grid: "
+                     f"{warm} warm-up steps, then the median of 3 runs of {steps} {integ} steps "
+                     f"({', '.join(f'{r:.2f}' for r in runs)} s)",
+           "runs_s": runs, "host_copy_GBs": bw / 1e9}
+    if bytes_per_kstep > 0:
+        out["byte_model_bound"] = bw / bytes_per_kstep
+    return out
 
 
 def measured_traffic(cls: str, workload: str, layout: str, precision: str, gradient: str = "R",
@@ -99,6 +113,25 @@ def measured_traffic(cls: str, workload: str, layout: str, precision: str, gradi
     return None, None
 
 
+def graph_chunk(steps: int, graph: int) -> int:
+    """Steps per replayed graph: the largest divisor of the timed step count
+    that is at most `graph` (0: direct launches)."""
+    if graph <= 0 or steps <= 0:
+        return 0
+    for g in range(min(graph, steps), 0, -1):
+        if steps % g == 0:
+            return g
+    return 1
+
+
+def warmup_steps(warmup: int, gsteps: int) -> int:
+    """Untimed steps before t0: at least `warmup`, rounded up to whole graphs
+    and at least one graph, so the timed steps only replay."""
+    if gsteps <= 0:
+        return warmup
+    return max(gsteps, -(-warmup // gsteps) * gsteps)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -121,6 +154,10 @@ def main():
     ap.add_argument("--epochs", type=int, default=20, help="parareal: training epochs")
     ap.add_argument("--tol", type=float, default=1e-6, help="parareal: convergence tolerance")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg of the default line")
+    ap.add_argument("--no-parareal", action="store_true", help="skip the learned-parareal leg of the default line")
+    ap.add_argument("--parareal-train-runs", type=int, default=16,
+                    help="random-pulse training runs of the parareal leg in the default line")
     ap.add_argument("--tuning", default=None,
                     help="JSON of edus_tuning fields to force after the autotuner (e.g. for counter runs)")
     args = ap.parse_args()
@@ -137,6 +174,9 @@ def main():
         return
     if args.workload == "screened_ipa14_hex120":
         run_screened(args)
+        return
+    if args.workload == "hbm_copy":
+        run_copy(args)
         return
 
     import numpy as np  # noqa: F401
@@ -164,9 +204,13 @@ def main():
                 setattr(tun, key, int(val))
         eng.set_tuning(tun)
 
-    # timed region: K steps, graph replay unless --graph 0
-    eng.graph(args.graph)
-    sim.steps(args.warmup)
+    # timed region: K steps, graph replay unless --graph 0.  The graph holds
+    # the largest divisor of K up to --graph steps, and the warm-up runs whole
+    # graphs (at least one), so capture and instantiation stay outside t0..t1.
+    gsteps = graph_chunk(args.steps, args.graph)
+    warm = warmup_steps(args.warmup, gsteps)
+    eng.graph(gsteps)
+    sim.steps(warm)
     eng.synchronize()
     eng.check()
     t0 = time.perf_counter()
@@ -221,14 +265,67 @@ def main():
         "config": {"workload": args.workload, "nb": nb, "grid": [sim.N0, sim.N1], "dt_au": sim.dt,
                    "integrator": integ, "layout": args.layout, "precision": args.precision,
                    "gradient_space": args.gradient_space, "peierls": bool(args.peierls),
-                   "graph_steps": 0 if args.peierls else args.graph,
+                   "graph_steps": 0 if args.peierls else gsteps,
+                   "warmup_steps_run": warm,
                    "model": "seeded random tight-binding (edus.models)", "tuning": eng.get_tuning().as_dict()},
         "roofline": roofline,
         "monitors": {"trace_dev": trace_dev, "herm_dev": herm_dev},
     }
+    if args.precision == "fp64" and not args.peierls and not args.no_mixed:
+        out["mixed"] = mixed_leg(params, layout, args)
+        _progress("mixed precision", out["mixed"])
+    if args.workload == "ipa14_hex120" and not args.no_parareal:
+        out["parareal"] = parareal_leg(args.parareal_train_runs, args.k_samples, args.epochs, args.tol,
+                                       args.coarse_steps)
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(params, args.cpu_steps, integ)
+        bps = 16.0 * nb * nb * (36 + 4 * n_e) if (solver == "RK" and not coul) else 0.0
+        out["cpu_baseline"] = cpu_baseline(params, args.cpu_steps, integ, bps)
     print(json.dumps(out), flush=True)
+
+
+def mixed_leg(params: dict, layout: int, args) -> dict:
+    """The same workload in mixed precision (FP32-MFMA commutator; FP64
+    gradient, state and stage updates) against FP64, both started at the
+    pulse's peak (laser t0 moved back by half the pulse) from the ground
+    state: the mixed run's throughput (graph replay, as the headline) and,
+    after the same warm-up + K steps, max|d rho| and the relative current
+    error against the FP64 engine (which the tests pin to the oracle)."""
+    import copy
+
+    import numpy as np
+
+    import edus
+
+    p = copy.deepcopy(params)
+    for las in p["lasers"]:
+        period_fs = 4.135667696 / float(las["frequency"])  # h / (hbar omega in eV), fs
+        las["t0"] = -0.5 * float(las["cycles"]) * period_fs
+        las["t0_units"] = "fs"
+    res = {}
+    gsteps = graph_chunk(args.steps, args.graph)
+    warm = warmup_steps(args.warmup, gsteps)
+    for prec in (edus.PREC_MIXED, edus.PREC_FP64):
+        sim = edus.Simulation(p, layout=layout, precision=prec)
+        eng = sim.engine
+        eng.graph(gsteps)
+        sim.steps(warm)
+        eng.synchronize()
+        t0 = time.perf_counter()
+        sim.steps(args.steps)
+        eng.synchronize()
+        el = time.perf_counter() - t0
+        _, _, vel = eng.observables()
+        E = sim.field(sim.time)
+        res[prec] = (eng.get_state(), np.asarray(vel), el, sim.Nk, E)
+        sim.close()
+    rm, vm, el, nk, E = res[edus.PREC_MIXED]
+    rf, vf, elf, _, _ = res[edus.PREC_FP64]
+    return {"value": nk * args.steps / el, "ms_per_step": el / args.steps * 1e3,
+            "fp64_value_same_leg": nk * args.steps / elf, "steps_compared": warm + args.steps,
+            "field_at_end_au": [float(x) for x in E],
+            "max_drho_vs_fp64": float(np.abs(rm - rf).max()),
+            "rel_current_err_vs_fp64": float(np.abs(vm - vf).max() / max(np.abs(vf).max(), 1e-300)),
+            "note": "commutator on FP32 MFMA with FP32 accumulation; gradient, state and stage updates FP64"}
 
 
 def _progress(*a):
@@ -339,6 +436,32 @@ def run_batch(args):
 
 
 
+def run_copy(args):
+    """Counter calibration: a plain elementwise copy (b = 1.0 * a, one read
+    and one write of 256 MiB per launch) through torch, so that rocprofv3's
+    FETCH_SIZE / WRITE_SIZE per dispatch can be scaled to bytes moved."""
+    import torch
+
+    n = (256 << 20) // 8
+    a = torch.arange(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(max(args.warmup, 1)):
+        torch.mul(a, 1.0, out=b)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(args.steps):
+        torch.mul(a, 1.0, out=b)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / args.steps
+    nbytes = 2.0 * n * 8
+    print(json.dumps({"metric": "plain copy (counter calibration)", "value": nbytes / (ms * 1e-3) / 1e9,
+                      "unit": "GB/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
+                      "higher_is_better": True, "dtype": "f64", "data": "synthetic",
+                      "config": {"workload": "hbm_copy", "read_bytes": n * 8, "write_bytes": n * 8}}), flush=True)
+
+
 def lindhard_flops(nk: int, nb: int, nf: int) -> float:
     """Algorithmic FP64 flops of one Lindhard response: for every pair (q, k),
     element a <= b (X_ba = conj X_ab) and conduction band c: B = conj(u_a) u_b
@@ -407,26 +530,26 @@ def run_screened(args):
     print(json.dumps(out), flush=True)
 
 
-def run_parareal(args):
+def parareal_leg(train_runs: int, k_samples: int, epochs: int, tol: float, coarse: int) -> dict:
     """Learned parareal on the 8-band HSEX model, 96^2 k: 32 slices of 200
     fine RK4 steps (dt 0.05 a.u.) of the pulse of the hsex8_hex96 workload.
-    Training data: 64 runs with random pulses (2048 slice trajectories) of the
-    plain method; the network is trained on the GPU.  Reported: wall time of
-    sequential GPU RK4 (graph replay), of parareal with the coarse RK4 alone
-    (the method without the network) and with the network, iterations, and
-    max|d rho| of both against sequential RK4 at every slice boundary."""
+    Training data: `train_runs` runs with random pulses of the plain method;
+    the network is trained on the GPU.  Reported: wall time of sequential GPU
+    RK4 (graph replay), of parareal with the coarse RK4 alone (the method
+    without the network) and with the network, iterations, and max|d rho| of
+    both against sequential RK4 at every slice boundary."""
     import numpy as np
 
     import edus
     from edus import models
     from edus import parareal as pr
 
-    N, FINE, COARSE, DT = 32, 200, args.coarse_steps, 0.05
+    N, FINE, COARSE, DT = 32, 200, coarse, 0.05
     tmp = tempfile.mkdtemp(prefix="edus_para_")
     params = models.workload("hsex8_hex96", tmp)
     sim = edus.Simulation(params)
     eng = sim.engine
-    _progress("set up")
+    _progress("parareal: set up")
 
     def field(t):
         t = np.asarray(t, dtype=float)
@@ -440,35 +563,36 @@ def run_parareal(args):
     seq.set_state(eng.get_state())
     seq.synchronize()
     t0 = time.perf_counter()
-    bounds = [None] * (N + 1)
     for n in range(N):
         seq.rk4_steps(DT, Ef[n])
     seq.synchronize()
     t_seq = time.perf_counter() - t0
-    _progress("sequential RK4", t_seq, "s")
+    _progress("parareal: sequential RK4", t_seq, "s")
     # slice boundaries of the sequential run (untimed)
     chk = eng.clone()
-    bounds[0] = chk.get_state()
+    bounds = [chk.get_state()]
     for n in range(N):
         chk.rk4_steps(DT, Ef[n])
-        bounds[n + 1] = chk.get_state()
+        bounds.append(chk.get_state())
+    chk.close()
+    seq.close()
     # training data and training (one-off costs, reported separately)
     t1 = time.perf_counter()
-    pulses = pr.random_pulses(args.train_runs, seed=11)
+    pulses = pr.random_pulses(train_runs, seed=11)
     tabs = [pr.slice_tables(pl, N, FINE, COARSE, DT) for pl in pulses]
-    P = pr.Params(N, FINE, COARSE, DT, args.tol, 6, 50)
+    P = pr.Params(N, FINE, COARSE, DT, tol, 6, 50)
     X, T = pr.collect(eng, P, np.stack([t[0] for t in tabs]), np.stack([t[1] for t in tabs]),
-                      np.stack([t[2] for t in tabs]), k_samples=args.k_samples, seed=5)
+                      np.stack([t[2] for t in tabs]), k_samples=k_samples, seed=5)
     t_collect = time.perf_counter() - t1
-    _progress("training data", X.shape, t_collect, "s")
+    _progress("parareal: training data", X.shape, t_collect, "s")
     t2 = time.perf_counter()
     stats = pr.normalisation(X, T)
-    net, hist = pr.train_network(X, T, stats, hidden=256, epochs=args.epochs, batch=1024, lr=1e-3, seed=7)
+    net, hist = pr.train_network(X, T, stats, hidden=256, epochs=epochs, batch=1024, lr=1e-3, seed=7)
     t_train = time.perf_counter() - t2
-    _progress("training", t_train, "s, loss", float(hist[0]), "->", float(hist[-1]))
+    _progress("parareal: training", t_train, "s, loss", float(hist[0]), "->", float(hist[-1]))
 
     def run(with_net):
-        para = pr.Parareal(eng, N, FINE, COARSE, DT, tol=args.tol, max_iter=6, graph_steps=50)
+        para = pr.Parareal(eng, N, FINE, COARSE, DT, tol=tol, max_iter=6, graph_steps=50)
         para.set_fields(Ef, Ec, Es)
         if with_net:
             para.set_network(net, stats)
@@ -476,26 +600,36 @@ def run_parareal(args):
         res = para.run()
         err = max(float(np.abs(para.state(n) - bounds[n]).max()) for n in range(N + 1))
         res["max_drho_vs_sequential"] = err
+        para.close()
         _progress("parareal", "with" if with_net else "without", "network", res)
         return res
 
     plain = run(False)
     learned = run(True)
+    return {"workload": "parareal_hsex8_96", "slices": N, "fine_steps": FINE, "coarse_steps": COARSE, "dt_au": DT,
+            "tol": tol, "max_iter": 6, "train_runs": train_runs, "k_samples": k_samples, "epochs": epochs,
+            "network": f"{X.shape[1]}-256-256-{T.shape[1]}",
+            "sequential_rk4_s": t_seq, "parareal_plain": plain, "parareal_learned": learned,
+            "learned_over_sequential": learned["seconds"] / t_seq,
+            "learned_over_plain_iterations": [learned["iterations"], plain["iterations"]],
+            "training": {"rows": int(X.shape[0]), "collect_s": t_collect, "train_s": t_train,
+                         "loss_first": float(hist[0]), "loss_last": float(hist[-1])}}
+
+
+def run_parareal(args):
+    """The learned-parareal leg alone, as its own line (time-like metric)."""
+    r = parareal_leg(args.train_runs, args.k_samples, args.epochs, args.tol, args.coarse_steps)
+    learned = r["parareal_learned"]
     out = {
         "metric": "learned-parareal wall time vs plain GPU RK4 (8-band HSEX, 96^2 k, 32 x 200 steps)",
-        "value": learned["seconds"], "unit": "s", "n_gpus": 1, "steps": N * FINE, "warmup": 0,
-        "ms_per_step": learned["seconds"] / (N * FINE) * 1e3, "higher_is_better": False, "scaling": "strong",
-        "vs_baseline": None, "dtype": "f64 (network bf16/f32)", "data": "synthetic",
-        "config": {"workload": "parareal_hsex8_96", "slices": N, "fine_steps": FINE, "coarse_steps": COARSE,
-                   "dt_au": DT, "tol": args.tol, "max_iter": 6, "train_runs": args.train_runs,
-                   "k_samples": args.k_samples, "epochs": args.epochs, "network": f"{X.shape[1]}-256-256-{T.shape[1]}"},
-        "sequential_rk4_s": t_seq,
-        "parareal_plain": plain,
-        "parareal_learned": learned,
-        "speedup_learned_vs_sequential": t_seq / learned["seconds"],
-        "training": {"rows": int(X.shape[0]), "collect_s": t_collect, "train_s": t_train,
-                     "loss_first": float(hist[0]), "loss_last": float(hist[-1])},
+        "value": learned["seconds"], "unit": "s", "n_gpus": 1, "steps": r["slices"] * r["fine_steps"], "warmup": 0,
+        "ms_per_step": learned["seconds"] / (r["slices"] * r["fine_steps"]) * 1e3, "higher_is_better": False,
+        "scaling": "strong", "vs_baseline": None, "dtype": "f64 (network bf16/f32)", "data": "synthetic",
+        "config": {k: r[k] for k in ("workload", "slices", "fine_steps", "coarse_steps", "dt_au", "tol", "max_iter",
+                                     "train_runs", "k_samples", "epochs", "network")},
     }
+    out.update({k: r[k] for k in ("sequential_rk4_s", "parareal_plain", "parareal_learned", "training")})
+    out["speedup_learned_vs_sequential"] = r["sequential_rk4_s"] / learned["seconds"]
     print(json.dumps(out), flush=True)
 
 

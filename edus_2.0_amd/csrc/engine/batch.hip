@@ -32,10 +32,29 @@
 
 namespace edus {
 
+namespace {
+
+// Members that can share one stacked engine: same shape and arithmetic, so
+// that each ends exactly where it ends alone -- nb, grid, layout, precision,
+// exchange, filled bands, gradient space (and, for the k-space gradient, the
+// same stencil, which depends on the lattice) and commutator engine.
+bool same_group(const Engine& a, const Engine& b)
+{
+    if (a.nb != b.nb || a.N0 != b.N0 || a.N1 != b.N1 || a.layout != b.layout || a.prec != b.prec ||
+        a.coulomb != b.coulomb || a.nf != b.nf || a.kgrad != b.kgrad || a.comm_kind != b.comm_kind)
+        return false;
+    if (!a.kgrad) return true;
+    if (a.kst.size() != b.kst.size()) return false;
+    for (size_t i = 0; i < a.kst.size(); ++i)
+        if (std::memcmp(&a.kst[i], &b.kst[i], sizeof(KStencilPoint)) != 0) return false;
+    return true;
+}
+
+}  // namespace
+
 Batch::Batch(int nmem, edus_ctx* const* ctxs) : n(nmem)
 {
     if (n < 1) throw EngineError(EDUS_ERR_ARG, "a batch needs at least one member");
-    std::map<std::tuple<int, int, int, int, int, int, int>, int> key_to_group;
     std::vector<std::vector<const Engine*>> gm;
     std::vector<std::vector<int>> gidx;
     where.resize(n);
@@ -44,16 +63,13 @@ Batch::Batch(int nmem, edus_ctx* const* ctxs) : n(nmem)
         const Engine& e = *ctxs[i]->e;
         if (e.B != 1) throw EngineError(EDUS_ERR_ARG, "batch members must be single contexts");
         ctx.push_back(ctxs[i]);
-        const auto key = std::make_tuple(e.nb, e.N0, e.N1, e.layout, e.prec, int(e.coulomb), e.nf);
-        auto it = key_to_group.find(key);
-        int g;
-        if (it == key_to_group.end()) {
+        int g = -1;
+        for (size_t q = 0; q < gm.size() && g < 0; ++q)
+            if (same_group(*gm[q][0], e)) g = int(q);
+        if (g < 0) {
             g = int(gm.size());
-            key_to_group[key] = g;
             gm.emplace_back();
             gidx.emplace_back();
-        } else {
-            g = it->second;
         }
         where[i] = {g, int(gm[g].size())};
         gm[g].push_back(&e);

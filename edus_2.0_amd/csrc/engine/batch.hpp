@@ -42,6 +42,13 @@ class Batch {
     void copy_state(int i, bool to_member);
     void check(double* trace_dev, double* herm_dev);
     void sync_all();
+    // forget the captured graph (the members' kernels or tiles changed),
+    // after the replays in flight have finished
+    void invalidate()
+    {
+        sync_all();
+        drop_graph();
+    }
 
   private:
     hipGraph_t graph = nullptr;

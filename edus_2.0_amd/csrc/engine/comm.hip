@@ -23,6 +23,7 @@
 // LAYOUT = kFull keeps every element; kHermitian keeps the upper triangle and
 // expands the lower one by conjugation while reading LDS.
 #include <algorithm>
+#include <mutex>
 #include <type_traits>
 
 #include "common.hpp"
@@ -563,12 +564,11 @@ void launch_one(const CommArgs& a, hipStream_t s)
     constexpr int TK = tile_k<NB, LAYOUT, KIND, TILE>();
     constexpr size_t lds = comm_lds_bytes<NB, LAYOUT, KIND, TILE>();
     auto kern = &comm_kernel<NB, LAYOUT, KIND, TILE, AB>;
-    static bool raised = false;
-    if (!raised) {
+    static std::once_flag raised;  // contexts may be created from several host threads
+    std::call_once(raised, [&] {
         (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
                                   int(lds));
-        raised = true;
-    }
+    });
     hipLaunchKernelGGL(kern, dim3(((a.Nk + TK - 1) / TK) * std::max(a.B, 1)), dim3(comm_threads<KIND, TILE>()), lds,
                        s, a);
 }

@@ -20,12 +20,14 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <tuple>
 #include <vector>
 
 #include "../../../include/edus_gpu.h"
+#include "batch.hpp"
 #include "engine_internal.hpp"
 #include "kernels.hpp"
 #include "obs.hpp"
@@ -83,6 +85,37 @@ std::map<std::tuple<int, int, int, int, int>, std::tuple<int, int, int>>& tuned_
     return m;
 }
 
+// Guards both caches: contexts may be created from several host threads.
+// Held for a whole tuning pass, so one shape is timed once.
+std::mutex& tuning_mutex()
+{
+    static std::mutex m;
+    return m;
+}
+
+// The HERMITIAN layout stores the upper triangle of the exchange self energy
+// and conjugates it into the lower one.  Sigma(k) is Hermitian for Hermitian
+// rho only if the real W satisfies W_mn(R) = W_nm(-R) with -R taken on the
+// periodic FFT grid (index (N - j) mod N, so the even-grid Nyquist index is
+// its own mirror).  W: [Nk][nb][nb] interleaved complex (real part used).
+bool exchange_is_hermitian(const double* W, int nb, int N0, int N1)
+{
+    const size_t M = size_t(nb) * nb;
+    double asym = 0., scale = 0.;
+    for (int j0 = 0; j0 < N0; ++j0)
+        for (int j1 = 0; j1 < N1; ++j1) {
+            const size_t iR = size_t(j0) * N1 + j1;
+            const size_t mR = size_t((N0 - j0) % N0) * N1 + (N1 - j1) % N1;
+            for (int n = 0; n < nb; ++n)
+                for (int m = 0; m < nb; ++m) {
+                    const double a = W[2 * (iR * M + size_t(m) * nb + n)], b = W[2 * (mR * M + size_t(n) * nb + m)];
+                    asym = std::max(asym, std::fabs(a - b));
+                    scale = std::max(scale, std::fabs(a));
+                }
+        }
+    return asym <= 1e-12 * std::max(scale, 1e-300);
+}
+
 }  // namespace
 
 void Engine::alloc(void** p, size_t bytes)
@@ -119,6 +152,11 @@ Engine::Engine(const edus_desc& d, const EngineOptions& opt)
     if (prec != EDUS_PREC_FP64 && prec != EDUS_PREC_MIXED) throw EngineError(EDUS_ERR_ARG, "unknown precision");
     if (coulomb && (!d.W || !d.hartree || !d.rho0k))
         throw EngineError(EDUS_ERR_ARG, "coulomb requires W, hartree and rho0k");
+    if (coulomb && layout == EDUS_LAYOUT_HERMITIAN && !exchange_is_hermitian(d.W, nb, N0, N1))
+        throw EngineError(EDUS_ERR_UNSUPPORTED,
+                          "HERMITIAN layout with exchange needs W_mn(R) = W_nm(-R) on the FFT grid; this W breaks it "
+                          "(even grids put the unpaired Nyquist R in it, and so does the reference's Wannier-centre "
+                          "accumulation): use the FULL layout");
     npl = num_planes(nb, layout);
 
     HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
@@ -416,6 +454,7 @@ void launch_fft(const AxisFFTArgs& a, const FFTChoice& c, hipStream_t s)
 // Results are cached per shape for the life of the process.
 void Engine::autotune()
 {
+    std::lock_guard<std::mutex> lock(tuning_mutex());
     for (int axis = 0; axis < 2; ++axis) {
         const int n = axis == 0 ? N0 : N1;
         const int other = axis == 0 ? N1 : N0;
@@ -1237,6 +1276,8 @@ int edus_set_tuning(edus_ctx* ctx, const edus_tuning* in)
             e.comm_kind64 = in->comm_kind;
             e.comm_tile64 = in->comm_tile;
         }
+        // a captured graph holds the old kernels: capture again on next use
+        if (e.self_batch) e.self_batch->invalidate();
     });
 }
 

@@ -20,6 +20,7 @@
 // LDS write); the passes ping-pong between two LDS buffers, one barrier each.
 // (NT, MAXV) is picked per grid axis by the engine's autotuner.
 #include <algorithm>
+#include <mutex>
 
 #include "common.hpp"
 #include "kernels.hpp"
@@ -351,12 +352,11 @@ template <int NT, int MAXV>
 void launch_cfg(const AxisFFTArgs& a, int blocks, size_t lds, hipStream_t s)
 {
     // dynamic LDS above 64 KB must be allowed explicitly, once per instantiation
-    static bool raised = false;
-    if (!raised) {
+    static std::once_flag raised;  // contexts may be created from several host threads
+    std::call_once(raised, [] {
         (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&axis_fft_kernel<NT, MAXV>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, int((2048 + 2 * MAXV) * 16));
-        raised = true;
-    }
+    });
     hipLaunchKernelGGL((axis_fft_kernel<NT, MAXV>), dim3(blocks), dim3(NT), lds, s, a);
 }
 

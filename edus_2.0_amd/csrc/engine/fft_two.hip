@@ -22,6 +22,7 @@
 // the first arithmetic and the field loads of later phases (addend, multiplier)
 // are issued one phase ahead.
 #include <algorithm>
+#include <mutex>
 
 #include "common.hpp"
 #include "fft_reg.hpp"
@@ -317,13 +318,13 @@ void launch_mode(const AxisFFTArgs& a, int blocks, size_t lds, hipStream_t s)
 {
     using Sh = TwoStepShape<RA, RB>;
     auto kern = &two_step_kernel<RA, RB, NT, MODE, EARLY>;
-    static bool raised = false;
-    if (!raised && lds > 0) {
-        const int maxl = NT / std::max(RA, RB);
-        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  int((Sh::n + maxl * Sh::LS) * sizeof(double2)));
-        raised = true;
-    }
+    static std::once_flag raised;  // contexts may be created from several host threads
+    if (lds > 0)
+        std::call_once(raised, [&] {
+            const int maxl = NT / std::max(RA, RB);
+            (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      int((Sh::n + maxl * Sh::LS) * sizeof(double2)));
+        });
     hipLaunchKernelGGL(kern, dim3(blocks), dim3(NT), lds, s, a);
 }
 

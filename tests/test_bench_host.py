@@ -50,3 +50,14 @@ def test_more_than_one_gpu_is_refused():
                        text=True, timeout=120)
     assert r.returncode != 0
     assert "one GPU" in (r.stdout + r.stderr)
+
+
+def test_graph_capture_stays_out_of_the_timed_region():
+    """The timed steps are whole graphs, and the warm-up runs at least one
+    whole graph, so capture and instantiation happen before t0."""
+    for steps, graph, warmup in ((20, 20, 5), (200, 20, 20), (25, 20, 3), (7, 20, 0), (30, 8, 1)):
+        g = bench.graph_chunk(steps, graph)
+        assert 1 <= g <= graph and steps % g == 0
+        w = bench.warmup_steps(warmup, g)
+        assert w >= warmup and w >= g and w % g == 0
+    assert bench.graph_chunk(20, 0) == 0 and bench.warmup_steps(5, 0) == 5
