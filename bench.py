@@ -30,7 +30,8 @@ def kernel_bytes(cls: str, npl: int, nk: int, nb: int, n_e: int, solver: str = "
                  coulomb: bool = False, peierls: bool = False) -> float:
     """Algorithmic HBM bytes per launch: one array of npl planes is
     S = npl*nk*16 B.  grad_col reads rho, writes G (2S); grad_row reads rho
-    and G, writes D (3S); kgrad (finite-difference stencil, gradient_space
+    and G, writes D (3S); grad_fused (both axes in one pass per plane) reads
+    rho, writes D (2S); kgrad (finite-difference stencil, gradient_space
     "k") reads rho, writes D (2S); hf_row (forward or inverse row transform) 2S;
     hf_col (column convolution with the real W) 2.5S.  comm reads rho_s, H0,
     n_E xi, D (+ the exchange self energy) and the stage's state arrays and
@@ -47,7 +48,8 @@ def kernel_bytes(cls: str, npl: int, nk: int, nb: int, n_e: int, solver: str = "
     comm = (5.5 + n_e + sig) if solver == "RK" else (4 + n_e + order + sig)
     if peierls:
         comm -= 1 + n_e
-    return {"grad_col": 2 * s, "grad_row": 3 * s, "kgrad": 2 * s, "comm": comm * s, "hf_row": 2 * s,
+    return {"grad_col": 2 * s, "grad_row": 3 * s, "grad_fused": 2 * s, "kgrad": 2 * s, "comm": comm * s,
+            "hf_row": 2 * s,
             "hf_col": 2.5 * s, "peierls_col": (2 + n_e) * s,
             "peierls_row": 2 * s}.get(cls, 0.0)
 
@@ -228,7 +230,8 @@ def main():
     sim.steps(args.steps)
     eng.synchronize()
     elapsed_direct = time.perf_counter() - t1
-    classes = ["grad_col", "grad_row", "kgrad", "comm", "hf_row", "hf_col", "peierls_col", "peierls_row"]
+    classes = ["grad_col", "grad_row", "grad_fused", "kgrad", "comm", "hf_row", "hf_col", "peierls_col",
+               "peierls_row"]
     times = {c: eng.timing_get(c) for c in classes}
     eng.timing_enable(False)
 

@@ -37,11 +37,13 @@ namespace {
 // Members that can share one stacked engine: same shape and arithmetic, so
 // that each ends exactly where it ends alone -- nb, grid, layout, precision,
 // exchange, filled bands, gradient space (and, for the k-space gradient, the
-// same stencil, which depends on the lattice) and commutator engine.
+// same stencil, which depends on the lattice), commutator engine and
+// spectral-gradient form.
 bool same_group(const Engine& a, const Engine& b)
 {
     if (a.nb != b.nb || a.N0 != b.N0 || a.N1 != b.N1 || a.layout != b.layout || a.prec != b.prec ||
-        a.coulomb != b.coulomb || a.nf != b.nf || a.kgrad != b.kgrad || a.comm_kind != b.comm_kind)
+        a.coulomb != b.coulomb || a.nf != b.nf || a.kgrad != b.kgrad || a.comm_kind != b.comm_kind ||
+        a.grad_fused != b.grad_fused)
         return false;
     if (!a.kgrad) return true;
     if (a.kst.size() != b.kst.size()) return false;
@@ -281,6 +283,9 @@ void Batch::rk4_steps(double dt, int nsteps, const double* E)
         fill_tables(groups[gi], nsteps, E, fa, fx);
         flags[gi] = {fa[0], fa[1], fx[0], fx[1], fx[2]};
     }
+    // work queued on the groups' own streams (a state copied in, earlier
+    // direct steps) comes before anything on the batch stream
+    join_to(bs);
     int done = 0;
     const int S = graph_steps;
     if (S > 0 && nsteps >= S) {

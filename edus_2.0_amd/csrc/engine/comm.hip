@@ -24,6 +24,7 @@
 // expands the lower one by conjugation while reading LDS.
 #include <algorithm>
 #include <mutex>
+#include <stdexcept>
 #include <type_traits>
 
 #include "common.hpp"
@@ -251,6 +252,11 @@ __global__ void __launch_bounds__((comm_threads<KIND, TILE>())) comm_kernel(Comm
     // RK operand x2 in ex[0], or the AB history f_{n-1..n-4}.
     constexpr bool ab = AB;
     constexpr int NEX = AB ? 4 : 1;
+    // the FP32 engine reads H0 and xi stored in FP32 (half the bytes) into
+    // registers instead of the direct H0 load, when it is given them (not
+    // with exchange: there H0 holds -Sigma[rho0], which cancels against
+    // Sigma[rho] and must not be rounded first)
+    const bool F32H = KIND == kCommMfma32 && a.n_addf > 0;
     const double2* x1p = ab ? u.y : u.x1;
     const bool ld_x1 = (ab || u.out1) && x1p != a.rho;
     const bool ld_x2 = !ab && u.out2 && u.x2 != a.rho;
@@ -264,7 +270,7 @@ __global__ void __launch_bounds__((comm_threads<KIND, TILE>())) comm_kernel(Comm
             if (kk < TK && p < NPL && k0 + kk < Nk) {
                 const long o = ioff + long(p) * Nk + (k0 + kk);
                 __builtin_amdgcn_global_load_lds(a.rho + o, sR + c * 64, 16, 0, 0);
-                __builtin_amdgcn_global_load_lds(a.H0 + o, sH + c * 64, 16, 0, 0);
+                if (!F32H) __builtin_amdgcn_global_load_lds(a.H0 + o, sH + c * 64, 16, 0, 0);
             }
         }
     }
@@ -290,68 +296,129 @@ __global__ void __launch_bounds__((comm_threads<KIND, TILE>())) comm_kernel(Comm
     }
     double2 dv[PER], x1v[PER], ex[NEX][PER];
     {
-        double2 xa[PER], xb[PER];
+        if (F32H) {
+            // H = H0 + sum_d E_d xi_d from the FP32 fields (addf[0] = H0,
+            // coefficient 1; then the xi present), widened to FP64 in the tile
+            double cff[4];
 #pragma unroll
-        for (int j = 0; j < PER; ++j) xa[j] = xb[j] = zero;
-        if (a.n_add > 0) {
-#pragma unroll
-            for (int j = 0; j < PER; ++j)
-                if (ok[j]) xa[j] = a.add[0][off[j]];
-        }
-        if (a.n_add > 1) {
-#pragma unroll
-            for (int j = 0; j < PER; ++j)
-                if (ok[j]) xb[j] = a.add[1][off[j]];
-        }
-        if (a.hdiag && threadIdx.x < NB) sHd[threadIdx.x] = a.hdiag[inst * NB + threadIdx.x];
-        __builtin_amdgcn_s_waitcnt(0);  // this wave's direct loads are in LDS
-        __syncthreads();                // and every other wave's
-        // H = H0 + sum of the addends (in the order xi_0, xi_1, xi_2, Sigma)
-        // + Hartree diagonal, in place in the H tile
-#pragma unroll
-        for (int j = 0; j < PER; ++j) {
-            if (!in_tile[j]) continue;
-            const int sl = slot_of[j];
-            double2 h = sH[sl];
-            if (a.n_add > 0) {
-                h.x += cf[0] * xa[j].x;
-                h.y += cf[0] * xa[j].y;
+            for (int r = 0; r < 4; ++r) {
+                const int dd = a.addf_dd[r];
+                cff[r] = dd == 0 ? Ev[0] : dd == 1 ? Ev[1] : dd == 2 ? Ev[2] : 1.0;
             }
-            if (a.n_add > 1) {
-                h.x += cf[1] * xb[j].x;
-                h.y += cf[1] * xb[j].y;
-            }
-            sH[sl] = h;
-            // stage operands that are the stage input itself
-            // (if/else rather than ?: on double2 keeps the values out of scratch)
-            x1v[j] = zero;
-            ex[0][j] = zero;
-            if (!ld_x1) x1v[j] = sR[sl];
-            if (!ab && !ld_x2) ex[0][j] = sR[sl];
-        }
-        if (a.n_add > 2) {  // third and fourth addend: a second round
+            float2 fa[PER], fb[PER];
+            const float2 fz = make_float2(0.f, 0.f);
 #pragma unroll
             for (int j = 0; j < PER; ++j) {
-                xa[j] = xb[j] = zero;
-                if (ok[j]) xa[j] = a.add[2][off[j]];
+                fa[j] = fb[j] = fz;
+                if (ok[j]) fa[j] = a.addf[0][off[j]];
             }
-            if (a.n_add > 3) {
+            if (a.n_addf > 1) {
 #pragma unroll
                 for (int j = 0; j < PER; ++j)
-                    if (ok[j]) xb[j] = a.add[3][off[j]];
+                    if (ok[j]) fb[j] = a.addf[1][off[j]];
             }
+            if (a.hdiag && threadIdx.x < NB) sHd[threadIdx.x] = a.hdiag[inst * NB + threadIdx.x];
+            __builtin_amdgcn_s_waitcnt(0);  // this wave's direct loads are in LDS
+            __syncthreads();                // and every other wave's
 #pragma unroll
             for (int j = 0; j < PER; ++j) {
                 if (!in_tile[j]) continue;
                 const int sl = slot_of[j];
-                double2 h = sH[sl];
-                h.x += cf[2] * xa[j].x;
-                h.y += cf[2] * xa[j].y;
-                if (a.n_add > 3) {
-                    h.x += cf[3] * xb[j].x;
-                    h.y += cf[3] * xb[j].y;
+                double2 h = make_double2(double(fa[j].x), double(fa[j].y));
+                if (a.n_addf > 1) {
+                    h.x += cff[1] * double(fb[j].x);
+                    h.y += cff[1] * double(fb[j].y);
                 }
                 sH[sl] = h;
+                x1v[j] = zero;
+                ex[0][j] = zero;
+                if (!ld_x1) x1v[j] = sR[sl];
+                if (!ab && !ld_x2) ex[0][j] = sR[sl];
+            }
+            if (a.n_addf > 2 || a.sigma) {  // the other xi and Sigma: a second round
+                double2 sg[PER];
+#pragma unroll
+                for (int j = 0; j < PER; ++j) {
+                    fa[j] = fb[j] = fz;
+                    sg[j] = zero;
+                    if (ok[j] && a.n_addf > 2) fa[j] = a.addf[2][off[j]];
+                    if (ok[j] && a.n_addf > 3) fb[j] = a.addf[3][off[j]];
+                    if (ok[j] && a.sigma) sg[j] = a.sigma[off[j]];
+                }
+#pragma unroll
+                for (int j = 0; j < PER; ++j) {
+                    if (!in_tile[j]) continue;
+                    const int sl = slot_of[j];
+                    double2 h = sH[sl];
+                    h.x += cff[2] * double(fa[j].x) + cff[3] * double(fb[j].x) + sg[j].x;
+                    h.y += cff[2] * double(fa[j].y) + cff[3] * double(fb[j].y) + sg[j].y;
+                    sH[sl] = h;
+                }
+            }
+        } else {
+            double2 xa[PER], xb[PER];
+    #pragma unroll
+            for (int j = 0; j < PER; ++j) xa[j] = xb[j] = zero;
+            if (a.n_add > 0) {
+    #pragma unroll
+                for (int j = 0; j < PER; ++j)
+                    if (ok[j]) xa[j] = a.add[0][off[j]];
+            }
+            if (a.n_add > 1) {
+    #pragma unroll
+                for (int j = 0; j < PER; ++j)
+                    if (ok[j]) xb[j] = a.add[1][off[j]];
+            }
+            if (a.hdiag && threadIdx.x < NB) sHd[threadIdx.x] = a.hdiag[inst * NB + threadIdx.x];
+            __builtin_amdgcn_s_waitcnt(0);  // this wave's direct loads are in LDS
+            __syncthreads();                // and every other wave's
+            // H = H0 + sum of the addends (in the order xi_0, xi_1, xi_2, Sigma)
+            // + Hartree diagonal, in place in the H tile
+    #pragma unroll
+            for (int j = 0; j < PER; ++j) {
+                if (!in_tile[j]) continue;
+                const int sl = slot_of[j];
+                double2 h = sH[sl];
+                if (a.n_add > 0) {
+                    h.x += cf[0] * xa[j].x;
+                    h.y += cf[0] * xa[j].y;
+                }
+                if (a.n_add > 1) {
+                    h.x += cf[1] * xb[j].x;
+                    h.y += cf[1] * xb[j].y;
+                }
+                sH[sl] = h;
+                // stage operands that are the stage input itself
+                // (if/else rather than ?: on double2 keeps the values out of scratch)
+                x1v[j] = zero;
+                ex[0][j] = zero;
+                if (!ld_x1) x1v[j] = sR[sl];
+                if (!ab && !ld_x2) ex[0][j] = sR[sl];
+            }
+            if (a.n_add > 2) {  // third and fourth addend: a second round
+    #pragma unroll
+                for (int j = 0; j < PER; ++j) {
+                    xa[j] = xb[j] = zero;
+                    if (ok[j]) xa[j] = a.add[2][off[j]];
+                }
+                if (a.n_add > 3) {
+    #pragma unroll
+                    for (int j = 0; j < PER; ++j)
+                        if (ok[j]) xb[j] = a.add[3][off[j]];
+                }
+    #pragma unroll
+                for (int j = 0; j < PER; ++j) {
+                    if (!in_tile[j]) continue;
+                    const int sl = slot_of[j];
+                    double2 h = sH[sl];
+                    h.x += cf[2] * xa[j].x;
+                    h.y += cf[2] * xa[j].y;
+                    if (a.n_add > 3) {
+                        h.x += cf[3] * xb[j].x;
+                        h.y += cf[3] * xb[j].y;
+                    }
+                    sH[sl] = h;
+                }
             }
         }
         if (a.hdiag) {
@@ -569,6 +636,7 @@ void launch_one(const CommArgs& a, hipStream_t s)
         (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
                                   int(lds));
     });
+    if (g_attributes_only) return;
     hipLaunchKernelGGL(kern, dim3(((a.Nk + TK - 1) / TK) * std::max(a.B, 1)), dim3(comm_threads<KIND, TILE>()), lds,
                        s, a);
 }
@@ -619,16 +687,29 @@ void launch_comm(int nb, int layout, int kind, int tile, const CommArgs& args, h
 {
     CommArgs a = args;
     a.n_add = 0;
+    a.n_addf = 0;
     for (int r = 0; r < 4; ++r) {
         a.add[r] = nullptr;
         a.add_dd[r] = -1;
+        a.addf[r] = nullptr;
+        a.addf_dd[r] = -1;
     }
-    for (int d = 0; d < 3; ++d)
-        if (args.xi[d]) {
-            a.add[a.n_add] = args.xi[d];
-            a.add_dd[a.n_add++] = d;
-        }
-    if (args.sigma) a.add[a.n_add++] = args.sigma;
+    if (kind == kCommMfma32 && args.H0f) {
+        if (args.sigma) throw std::runtime_error("FP32 H0 with exchange: H0 holds -Sigma[rho0] and stays FP64");
+        a.addf[a.n_addf++] = args.H0f;
+        for (int d = 0; d < 3; ++d)
+            if (args.xif[d]) {
+                a.addf[a.n_addf] = args.xif[d];
+                a.addf_dd[a.n_addf++] = d;
+            }
+    } else {
+        for (int d = 0; d < 3; ++d)
+            if (args.xi[d]) {
+                a.add[a.n_add] = args.xi[d];
+                a.add_dd[a.n_add++] = d;
+            }
+        if (args.sigma) a.add[a.n_add++] = args.sigma;
+    }
     switch (nb) {
         case 1: launch_nb<1>(a, layout, kind, tile, s); break;
         case 2: launch_nb<2>(a, layout, kind, tile, s); break;

@@ -37,6 +37,7 @@
 namespace edus {
 
 thread_local std::string g_last_error;
+thread_local bool g_attributes_only = false;
 
 void set_error(const std::string& s) { g_last_error = s; }
 
@@ -85,7 +86,14 @@ std::map<std::tuple<int, int, int, int, int>, std::tuple<int, int, int>>& tuned_
     return m;
 }
 
-// Guards both caches: contexts may be created from several host threads.
+// one-pass gradient or not per (N0, N1, planes of all instances)
+std::map<std::tuple<int, int, int>, bool>& tuned_grad()
+{
+    static std::map<std::tuple<int, int, int>, bool> m;
+    return m;
+}
+
+// Guards the caches: contexts may be created from several host threads.
 // Held for a whole tuning pass, so one shape is timed once.
 std::mutex& tuning_mutex()
 {
@@ -150,7 +158,7 @@ Engine::Engine(const edus_desc& d, const EngineOptions& opt)
     if (layout != EDUS_LAYOUT_FULL && layout != EDUS_LAYOUT_HERMITIAN) throw EngineError(EDUS_ERR_ARG, "unknown layout");
     if (!d.H0k) throw EngineError(EDUS_ERR_ARG, "H0k is required");
     if (prec != EDUS_PREC_FP64 && prec != EDUS_PREC_MIXED) throw EngineError(EDUS_ERR_ARG, "unknown precision");
-    if (coulomb && (!d.W || !d.hartree || !d.rho0k))
+    if (coulomb && (!d.W || !d.hartree || (!d.rho0k && opt.initial_filled < 0)))
         throw EngineError(EDUS_ERR_ARG, "coulomb requires W, hartree and rho0k");
     if (coulomb && layout == EDUS_LAYOUT_HERMITIAN && !exchange_is_hermitian(d.W, nb, N0, N1))
         throw EngineError(EDUS_ERR_UNSUPPORTED,
@@ -207,6 +215,7 @@ Engine::Engine(const edus_desc& d, const EngineOptions& opt)
         HIP_CHECK(hipStreamSynchronize(s));
     }
 
+    alloc_f32_fields();
     autotune();
     if (peierls) {
         alloc(reinterpret_cast<void**>(&Hp), pb);
@@ -215,6 +224,10 @@ Engine::Engine(const edus_desc& d, const EngineOptions& opt)
         alloc(reinterpret_cast<void**>(&ptab[1]), size_t(N1) * sizeof(double2));
     }
     if (opt.velocity_det > 0.0) build_velocity(opt.velocity_det);
+    if (opt.initial_filled >= 0) {
+        if (!U) throw EngineError(EDUS_ERR_ARG, "the initial condition needs the eigenvectors U");
+        initial_condition(opt.initial_filled);
+    }
 
     if (coulomb) {
         alloc(reinterpret_cast<void**>(&T), pb);
@@ -229,14 +242,43 @@ Engine::Engine(const edus_desc& d, const EngineOptions& opt)
         HIP_CHECK(hipMemcpyAsync(hart, hr.data(), nb * nb * sizeof(double), hipMemcpyHostToDevice, s));
         // H_ee = -W o (rho - rho0) + Hartree(rho - rho0) (Coulomb.cpp:150-171).  The rho0
         // part does not change in time: fold it into H0 once, H0 <- H0 - HF[rho0].
-        upload(d.rho0k, aux);
+        if (d.rho0k)
+            upload(d.rho0k, aux);
+        else  // the reference state is the initial condition computed above
+            HIP_CHECK(hipMemcpyAsync(aux, y, pb, hipMemcpyDeviceToDevice, s));
         fock(aux);  // T = Sigma[rho0], hdiag = Hartree[rho0]
         launch_axpy(H0, T, -1.0, long(npl) * Nk, s);
         launch_add_diag(H0, hdiag, -1.0, nb, Nk, layout, s);
         HIP_CHECK(hipMemsetAsync(aux, 0, pb, s));
     }
+    make_f32_fields();  // from the final H0
     HIP_CHECK(hipStreamSynchronize(s));
     HIP_CHECK(hipGetLastError());
+}
+
+// FP32 copies of H0 and xi for the mixed-precision commutator, which then
+// reads 8 B instead of 16 B per element of each.  Not with exchange (H0 holds
+// -Sigma[rho0], which cancels against Sigma[rho]: rounding it first would cost
+// that cancellation) nor in the Peierls gauge (H is rebuilt every stage; a
+// conversion pass would move more bytes than it saves).
+bool Engine::uses_f32_fields() const { return prec == EDUS_PREC_MIXED && !coulomb && !peierls; }
+
+void Engine::alloc_f32_fields()
+{
+    if (!uses_f32_fields()) return;
+    const size_t fb = size_t(B) * npl * Nk * sizeof(float2);
+    alloc(reinterpret_cast<void**>(&H0f), fb);
+    for (int d = 0; d < 3; ++d)
+        if (xi[d]) alloc(reinterpret_cast<void**>(&xif[d]), fb);
+}
+
+void Engine::make_f32_fields()
+{
+    if (!uses_f32_fields()) return;
+    const long n = long(B) * npl * Nk;
+    launch_to_f32(H0, H0f, n, s);
+    for (int d = 0; d < 3; ++d)
+        if (xi[d]) launch_to_f32(xi[d], xif[d], n, s);
 }
 
 // V_d = (-i[r_d, H0] + e_d . grad_k H0) / det, evaluated with the same
@@ -277,7 +319,11 @@ void Engine::observables(double* pop_bloch, double* pop_wannier, double* vel_out
     a.Nk = Nk;
     a.layout = layout;
     a.rho = rho_override ? rho_override : y + io;
-    a.U = U ? U + size_t(inst) * nb * nb * Nk : nullptr;
+    a.rho_bloch = nullptr;
+    if (U) {  // Bloch populations from U^dagger rho U (Simulation.cpp:439-470), staged in full planes
+        rotate(a.rho, stage, kFull, true, inst);
+        a.rho_bloch = stage;
+    }
     for (int d = 0; d < 3; ++d) a.vel[d] = vel[d] ? vel[d] + io : nullptr;
     a.partial = obs_partial;
     launch_observables(a, obs_out, s);
@@ -291,6 +337,43 @@ void Engine::observables(double* pop_bloch, double* pop_wannier, double* vel_out
     }
     if (vel_out)
         for (int c = 0; c < 6; ++c) vel_out[c] = h[2 * nb + c];
+}
+
+// out = U^dagger in U (to_bloch) or U in U^dagger for every k-point of
+// instance `inst` (Operator::go_to_bloch / go_to_wannier, Operator.hpp:528-564);
+// in: planes in this engine's layout, out: out_layout (kFull, kHermitian or
+// kRotKMajor).  Mixed-precision engines rotate on FP32 matrix cores unless
+// fp64 is asked for.
+void Engine::rotate(const double2* in, double2* out, int out_layout, bool to_bloch, int inst, bool fp64)
+{
+    if (!U) throw EngineError(EDUS_ERR_STATE, "no eigenvectors: the context was created without U");
+    RotateArgs r{};
+    r.Nk = Nk;
+    r.in = in;
+    r.U = U + size_t(inst) * nb * nb * Nk;
+    r.out = out;
+    r.out_layout = out_layout;
+    r.to_bloch = to_bloch ? 1 : 0;
+    const int kind = (prec == EDUS_PREC_MIXED && !fp64) ? kCommMfma32 : kCommMfma64;
+    TimedLaunch t;
+    tbegin("rotate", t);
+    launch_rotate(nb, layout, kind, r, s);
+    tend(t);
+}
+
+// rho = U diag(1 for n < filled, else 0) U^dagger per k-point into y
+// (Functional_InitialCondition.hpp:12-32), rotated on the FP64 matrix cores
+void Engine::initial_condition(int filled)
+{
+    const size_t pb = plane_bytes();
+    HIP_CHECK(hipMemsetAsync(aux, 0, pb, s));
+    std::vector<double2> one(Nk, make_double2(1.0, 0.0));
+    for (int n = 0; n < std::min(filled, nb); ++n)
+        HIP_CHECK(hipMemcpyAsync(aux + size_t(diag_plane(n, nb, layout)) * Nk, one.data(), Nk * sizeof(double2),
+                                 hipMemcpyHostToDevice, s));
+    rotate(aux, y, layout, false, 0, true);
+    HIP_CHECK(hipStreamSynchronize(s));  // the host vector is read by the copies
+    HIP_CHECK(hipMemsetAsync(aux, 0, pb, s));
 }
 
 Engine::~Engine()
@@ -315,6 +398,14 @@ void Engine::upload(const double* host_full, double2* dst)
 void Engine::download(const double2* src, double* host_full)
 {
     launch_from_planes(src, stage, nb, Nk, layout, s);
+    HIP_CHECK(hipMemcpyAsync(host_full, stage, size_t(nb) * nb * Nk * sizeof(double2), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+}
+
+// U^dagger src U of instance 0 -> host [Nk][nb][nb] (Operator::go_to_bloch)
+void Engine::download_bloch(const double2* src, double* host_full)
+{
+    rotate(src, stage, kRotKMajor, true);
     HIP_CHECK(hipMemcpyAsync(host_full, stage, size_t(nb) * nb * Nk * sizeof(double2), hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipStreamSynchronize(s));
 }
@@ -495,6 +586,39 @@ void Engine::autotune()
         tuned_shapes()[key] = best_c;
     }
 
+    // the spectral derivative: one pass per plane or two axis passes, whichever
+    // is faster here (identical results)
+    grad_fused = false;
+    if (grad_fused_available(N0, N1) && !kgrad) {
+        const auto gkey = std::make_tuple(N0, N1, B * npl);
+        auto git = tuned_grad().find(gkey);
+        if (force_grad >= 0) {  // a batch group keeps its members' form (they differ in rounding)
+            grad_fused = force_grad == 1;
+        } else if (git != tuned_grad().end()) {
+            grad_fused = git->second;
+        } else {
+            hipEvent_t e0, e1;
+            HIP_CHECK(hipEventCreate(&e0));
+            HIP_CHECK(hipEventCreate(&e1));
+            const double cg[2] = {1e-3, 1e-3};
+            float ms[2] = {0.f, 0.f};
+            for (int f = 0; f < 2; ++f) {
+                grad_fused = f == 1;
+                gradient(y, cg, nullptr, true, true);  // warm-up
+                HIP_CHECK(hipEventRecord(e0, s));
+                for (int r = 0; r < 3; ++r) gradient(y, cg, nullptr, true, true);
+                HIP_CHECK(hipEventRecord(e1, s));
+                HIP_CHECK(hipEventSynchronize(e1));
+                HIP_CHECK(hipEventElapsedTime(&ms[f], e0, e1));
+            }
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            HIP_CHECK(hipGetLastError());
+            grad_fused = ms[1] < ms[0];
+            tuned_grad()[gkey] = grad_fused;
+        }
+    }
+
     // commutator engine and tile, whichever is fastest here: for FP64 the FMA
     // engine or the matrix cores (they differ in summation order only), for
     // mixed precision the FP32 matrix cores; tiles give identical results.
@@ -541,9 +665,11 @@ void Engine::autotune()
         c.B = B;
         c.rho = aux;
         c.H0 = H0;
+        c.H0f = H0f;
         for (int d = 0; d < 3; ++d)
             if (xi[d]) {
                 c.xi[d] = xi[d];
+                c.xif[d] = xif[d];
                 break;
             }
         c.D = D;
@@ -598,6 +724,50 @@ void Engine::autotune()
 
     for (double2* p : {G, f, acc, aux}) HIP_CHECK(hipMemsetAsync(p, 0, plane_bytes(), s));
     HIP_CHECK(hipStreamSynchronize(s));
+    prime_launch_attributes();
+}
+
+// Raise the dynamic-LDS limit of every kernel this engine may launch with the
+// choices just made (every FFT mode of each axis' tile, the commutator engines
+// and tiles of both integrators, the rotation, the one-pass gradient) while
+// no stream is being captured: a graph captured later then launches them
+// with their full LDS.  Nothing is launched.
+void Engine::prime_launch_attributes()
+{
+    AttributesOnly guard;
+    for (int axis = 0; axis < 2; ++axis) {
+        if ((axis == 0 ? N0 : N1) < 2) continue;
+        for (int mode = 0; mode <= 4; ++mode) {
+            const AxisFFTArgs a = axis_args(axis, fft_choice[axis], mode, y, nullptr, G, 0.0, 1.0, nullptr);
+            launch_fft(a, fft_choice[axis], s);
+        }
+    }
+    CommArgs c{};
+    c.Nk = Nk;
+    c.P = npl;
+    c.B = B;
+    c.H0 = H0;
+    c.H0f = H0f ? H0f : reinterpret_cast<const float2*>(H0);  // not launched
+    for (int ab = 0; ab < 2; ++ab) {
+        c.up = StageUpdate{};
+        c.up.ab_order = ab ? 4 : 0;
+        for (int tile : {comm_tile, comm_tile_ab, comm_tile64, 0}) {
+            launch_comm(nb, layout, comm_kind, tile, c, s);
+            launch_comm(nb, layout, comm_kind64, tile, c, s);
+        }
+    }
+    if (U) {
+        RotateArgs r{};
+        r.Nk = Nk;
+        launch_rotate(nb, layout, kCommMfma64, r, s);
+        launch_rotate(nb, layout, kCommMfma32, r, s);
+    }
+    if (grad_fused_available(N0, N1)) {
+        GradFusedArgs g{};
+        g.N0 = N0;
+        g.N1 = N1;
+        launch_grad_fused(g, s);
+    }
 }
 
 // D = E . grad_k rho  (null when the field has no component along the grid):
@@ -610,7 +780,25 @@ const double2* Engine::gradient(const double2* in, const double c[2], const doub
     if (!use0 && !use1) return nullptr;
     const double* t0 = ctab ? ctab + 0 : nullptr;
     const double* t1 = ctab ? ctab + 1 : nullptr;
-    if (use0 && use1) {
+    if (use0 && use1 && grad_fused) {
+        GradFusedArgs g{};
+        g.in = in;
+        g.out = D;
+        g.P = B * npl;
+        g.N0 = N0;
+        g.N1 = N1;
+        g.coef[0] = c[0];
+        g.coef[1] = c[1];
+        g.coef_tab = ctab;
+        g.coef_stride = 2;
+        g.ppi = npl;
+        g.tw0 = tw[0];
+        g.tw1 = tw[1];
+        TimedLaunch t;
+        tbegin("grad_fused", t);
+        launch_grad_fused(g, s);
+        tend(t);
+    } else if (use0 && use1) {
         axis_fft(0, 0, in, nullptr, G, c[0], 1.0, nullptr, "grad_col", t0);
         axis_fft(1, 0, in, G, D, c[1], 1.0, nullptr, "grad_row", t1);
     } else if (use0) {
@@ -668,8 +856,10 @@ void Engine::evaluate(const double2* in, const double E[3], const StageUpdate& u
     c.rho = in;
     c.D = Dp;
     c.H0 = H0;
+    c.H0f = H0f;
     for (int d = 0; d < 3; ++d) {
         c.xi[d] = (xi[d] && E[d] != 0.0) ? xi[d] : nullptr;
+        c.xif[d] = (xif[d] && E[d] != 0.0) ? xif[d] : nullptr;
         c.E[d] = E[d];
     }
     c.sigma = coulomb ? T : nullptr;
@@ -826,10 +1016,14 @@ Engine::Engine(const std::vector<const Engine*>& members)
                                      hipMemcpyDeviceToDevice, s));
         }
     }
+    alloc_f32_fields();
+    make_f32_fields();
     HIP_CHECK(hipStreamSynchronize(s));
-    // the commutator engines differ in summation order: a group uses its
-    // members' engine so that every member ends exactly where it ends alone
+    // the commutator engines differ in summation order and the gradient forms
+    // in rounding: a group uses its members' choices so that every member
+    // ends exactly where it ends alone
     force_kind = m0.comm_kind;
+    force_grad = m0.grad_fused ? 1 : 0;
     autotune();
     HIP_CHECK(hipStreamSynchronize(s));
     HIP_CHECK(hipGetLastError());
@@ -860,7 +1054,11 @@ void Engine::evaluate_tab(const double2* in, int ts, int j, const bool use_axis[
     c.rho = in;
     c.D = Dp;
     c.H0 = H0;
-    for (int d = 0; d < 3; ++d) c.xi[d] = (xi[d] && use_xi[d]) ? xi[d] : nullptr;
+    c.H0f = H0f;
+    for (int d = 0; d < 3; ++d) {
+        c.xi[d] = (xi[d] && use_xi[d]) ? xi[d] : nullptr;
+        c.xif[d] = (xif[d] && use_xi[d]) ? xif[d] : nullptr;
+    }
     c.E_tab = E_dev + row * B * 3;
     c.sigma = coulomb ? T : nullptr;
     c.hdiag = coulomb ? hdiag : nullptr;
@@ -1002,6 +1200,35 @@ void Engine::ab_step(double dt, int order, const double* E3)
     abidx[0] = tmp;
 }
 
+void Engine::peek_monitor(double out[2])
+{
+    std::vector<unsigned long long> m(kMonWords);
+    HIP_CHECK(hipMemcpyAsync(m.data(), mon, kMonWords * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    unsigned long long best[2] = {0, 0};
+    for (int h = 0; h < 2; ++h)
+        for (int i = 0; i < kMonSlots; ++i) best[h] = std::max(best[h], m[size_t(h) * kMonHalf + size_t(i) * kMonStride]);
+    std::memcpy(out, best, sizeof(best));
+}
+
+// With limits set: the trace and Hermiticity defects of every state the steps
+// since the last edus_check wrote must stay within them (a limit <= 0 is not
+// checked); a NaN or inf anywhere (the monitors hold +inf then) always fails.
+void Engine::enforce_monitor_limits()
+{
+    if (!mon_limits_on()) return;
+    double v[2];
+    peek_monitor(v);
+    const char* name[2] = {"trace", "Hermiticity"};
+    for (int i = 0; i < 2; ++i) {
+        const bool bad = !std::isfinite(v[i]) || (mon_limit[i] > 0.0 && v[i] > mon_limit[i]);
+        if (bad)
+            throw EngineError(EDUS_ERR_MONITOR, std::string(name[i]) + " monitor " + std::to_string(v[i]) +
+                                                    " exceeds its limit " + std::to_string(mon_limit[i]) +
+                                                    " (or is not finite)");
+    }
+}
+
 // maxima over every monitor slot since the last read; the slots are cleared
 void Engine::read_monitor(double out[2])
 {
@@ -1069,6 +1296,26 @@ int edus_get_state(edus_ctx* ctx, double* rho_k)
     return guarded([&] { ctx->e->download(ctx->e->y, rho_k); });
 }
 
+int edus_rotate(edus_ctx* ctx, int to_bloch, const double* in, double* out)
+{
+    EDUS_REQUIRE(ctx && in && out);
+    return guarded([&] {
+        Engine& e = *ctx->e;
+        if (e.B != 1) throw EngineError(EDUS_ERR_ARG, "edus_rotate applies to single contexts");
+        e.upload(in, e.aux);
+        e.rotate(e.aux, e.stage, edus::kRotKMajor, to_bloch != 0);
+        HIP_CHECK(hipMemcpyAsync(out, e.stage, size_t(e.nb) * e.nb * e.Nk * sizeof(double2), hipMemcpyDeviceToHost,
+                                 e.s));
+        HIP_CHECK(hipStreamSynchronize(e.s));
+    });
+}
+
+int edus_get_state_bloch(edus_ctx* ctx, double* rho_bloch)
+{
+    EDUS_REQUIRE(ctx && rho_bloch);
+    return guarded([&] { ctx->e->download_bloch(ctx->e->y, rho_bloch); });
+}
+
 int edus_rhs(edus_ctx* ctx, const double E[3], const double* in, double* out)
 {
     EDUS_REQUIRE(ctx && E && in && out);
@@ -1089,6 +1336,7 @@ int edus_rk4_steps(edus_ctx* ctx, double dt, int nsteps, const double* E)
     return guarded([&] {
         ctx->e->rk4_steps(dt, nsteps, E);
         HIP_CHECK(hipGetLastError());
+        ctx->e->enforce_monitor_limits();
     });
 }
 
@@ -1103,6 +1351,7 @@ int edus_ab_steps(edus_ctx* ctx, double dt, int order, int nsteps, const double*
     return guarded([&] {
         for (int s = 0; s < nsteps; ++s) ctx->e->ab_step(dt, order, E + 3 * size_t(s));
         HIP_CHECK(hipGetLastError());
+        ctx->e->enforce_monitor_limits();
     });
 }
 
@@ -1139,6 +1388,7 @@ int edus_rk4_steps_peierls(edus_ctx* ctx, double dt, int nsteps, const double* E
     return guarded([&] {
         for (int s = 0; s < nsteps; ++s) ctx->e->rk4_step_peierls(dt, E + 9 * size_t(s), Av + 9 * size_t(s));
         HIP_CHECK(hipGetLastError());
+        ctx->e->enforce_monitor_limits();
     });
 }
 
@@ -1153,6 +1403,7 @@ int edus_ab_steps_peierls(edus_ctx* ctx, double dt, int order, int nsteps, const
     return guarded([&] {
         for (int s = 0; s < nsteps; ++s) ctx->e->ab_step_peierls(dt, order, E + 3 * size_t(s), Av + 3 * size_t(s));
         HIP_CHECK(hipGetLastError());
+        ctx->e->enforce_monitor_limits();
     });
 }
 
@@ -1198,6 +1449,18 @@ int edus_check(edus_ctx* ctx, double* trace_dev, double* herm_dev)
     });
 }
 
+int edus_monitor_limits(edus_ctx* ctx, double trace_limit, double herm_limit)
+{
+    EDUS_REQUIRE(ctx);
+    if (!(trace_limit >= 0.0) || !(herm_limit >= 0.0)) {  // also refuses NaN
+        edus::set_error("monitor limits must be >= 0 (0: not checked)");
+        return EDUS_ERR_ARG;
+    }
+    ctx->e->mon_limit[0] = trace_limit;
+    ctx->e->mon_limit[1] = herm_limit;
+    return EDUS_OK;
+}
+
 int edus_graph(edus_ctx* ctx, int steps_per_graph)
 {
     EDUS_REQUIRE(ctx);
@@ -1231,6 +1494,7 @@ int edus_get_tuning(edus_ctx* ctx, edus_tuning* out)
     out->comm_kind = e.comm_kind;
     out->comm_tile = e.comm_tile;
     out->comm_tile_ab = e.comm_tile_ab;
+    out->grad_fused = e.grad_fused ? 1 : 0;
     return EDUS_OK;
 }
 
@@ -1267,11 +1531,14 @@ int edus_set_tuning(edus_ctx* ctx, const edus_tuning* in)
         for (int t : {in->comm_tile, in->comm_tile_ab})
             if (t != 0 && (t < 1 || t > 2 || in->comm_kind == edus::kCommScalar))
                 throw EngineError(EDUS_ERR_ARG, "comm_tile and comm_tile_ab must be 0, or 1 or 2 for the MFMA engines");
+        if (in->grad_fused != 0 && (in->grad_fused != 1 || !edus::grad_fused_available(e.N0, e.N1) || e.kgrad))
+            throw EngineError(EDUS_ERR_ARG, "grad_fused must be 0, or 1 on the grids with a one-pass gradient kernel");
         e.fft_choice[0] = next[0];
         e.fft_choice[1] = next[1];
         e.comm_kind = in->comm_kind;
         e.comm_tile = in->comm_tile;
         e.comm_tile_ab = in->comm_tile_ab;
+        e.grad_fused = in->grad_fused != 0;
         if (e.prec != EDUS_PREC_MIXED) {
             e.comm_kind64 = in->comm_kind;
             e.comm_tile64 = in->comm_tile;

@@ -40,6 +40,10 @@ struct EngineOptions {
     // the device from H0 and r (Simulation.cpp:523-576) instead of taking it
     // from the descriptor.
     double velocity_det = 0.0;
+    // >= 0: set the state to the initial condition U diag(n < initial_filled) U^dagger
+    // (Functional_InitialCondition.hpp:12-32) on the device; with exchange and no
+    // rho0k in the descriptor, it is also the HSEX reference state
+    int initial_filled = -1;
 };
 
 class Batch;
@@ -80,6 +84,12 @@ class Engine {
 
     double2 *y = nullptr, *acc = nullptr, *aux = nullptr, *G = nullptr, *D = nullptr, *f = nullptr;
     double2 *H0 = nullptr, *xi[3] = {nullptr, nullptr, nullptr}, *vel[3] = {nullptr, nullptr, nullptr};
+    // mixed precision: H0 and xi rounded to FP32 for the FP32 commutator engine
+    // (the FP64 copies stay for the set-up products); Peierls: the phased H
+    float2 *H0f = nullptr, *xif[3] = {nullptr, nullptr, nullptr};
+    bool uses_f32_fields() const;
+    void alloc_f32_fields();
+    void make_f32_fields();
     double2 *U = nullptr, *T = nullptr, *stage = nullptr;
     double *W = nullptr, *hart = nullptr;
     double2 *hdiag = nullptr, *diag_partial = nullptr;
@@ -91,11 +101,20 @@ class Engine {
     double2* tw[2] = {nullptr, nullptr};
     FFTFactors fac[2];
     FFTChoice fft_choice[2];  // autotuned kernel and tile per grid axis
+    bool grad_fused = false;  // spectral derivative in one pass per plane (grad_fused.hip)
+    int force_grad = -1;      // >= 0: the autotuner keeps this gradient form (batch groups)
     int comm_kind = kCommScalar;     // engine of the RHS commutator (precision-dependent)
     int comm_kind64 = kCommScalar;   // FP64 engine, also used for set-up products
     int comm_tile = 0, comm_tile64 = 0;  // tile of each
     int comm_tile_ab = 0;                 // tile of the Adams-Bashforth kernel
     void read_monitor(double out[2]);
+    // the monitors' maxima since the last read, left in place
+    void peek_monitor(double out[2]);
+    // > 0: step calls check the monitors against these limits (edus_monitor_limits)
+    double mon_limit[2] = {0.0, 0.0};
+    bool mon_limits_on() const { return mon_limit[0] > 0.0 || mon_limit[1] > 0.0; }
+    // throws EDUS_ERR_MONITOR when a limit is crossed or a value is not finite
+    void enforce_monitor_limits();
     int tile_for(const StageUpdate& up) const { return up.ab_order > 0 ? comm_tile_ab : comm_tile; }
 
     bool timing = false;
@@ -106,6 +125,7 @@ class Engine {
     size_t plane_bytes() const { return size_t(B) * npl * Nk * sizeof(double2); }  // all instances
     void upload(const double* host_full, double2* dst);       // [Nk][nb][nb] -> planes
     void download(const double2* src, double* host_full);     // planes -> [Nk][nb][nb]
+    void download_bloch(const double2* src, double* host_full);  // U^dagger src U -> [Nk][nb][nb]
     void evaluate(const double2* in, const double E[3], const StageUpdate& up);
     // Peierls gauge: the same stage with field E and vector potential Av
     void evaluate_peierls(const double2* in, const double E[3], const double Av[3], const StageUpdate& up);
@@ -134,6 +154,11 @@ class Engine {
     void ab_step(double dt, int order, const double* E3);
     void observables(double* pop_bloch, double* pop_wannier, double* vel, int inst = 0,
                      const double2* rho_override = nullptr);
+    // gauge rotation of planes `in` (this layout) of instance inst into out
+    // (kFull / kHermitian planes or kRotKMajor); FP32 matrix cores in mixed
+    // precision unless fp64
+    void rotate(const double2* in, double2* out, int out_layout, bool to_bloch, int inst = 0, bool fp64 = false);
+    void initial_condition(int filled);
     void collect_timing();
     // k -> R of every plane of a FULL-layout field, sign -1 and 1/Nk
     // (Operator::go_to_R, Operator.hpp:566-606, fftPair.cpp:89-107); uses G
@@ -152,6 +177,7 @@ class Engine {
     void fock(const double2* in);
     void build_velocity(double det);
     void autotune();
+    void prime_launch_attributes();
     AxisFFTArgs axis_args(int axis, const FFTChoice& c, int mode, const double2* in, const double2* addend,
                           double2* out, double coef, double scale, const double* mult) const;
     void tbegin(const char* cls, TimedLaunch& t);

@@ -357,6 +357,7 @@ void launch_cfg(const AxisFFTArgs& a, int blocks, size_t lds, hipStream_t s)
         (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&axis_fft_kernel<NT, MAXV>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, int((2048 + 2 * MAXV) * 16));
     });
+    if (g_attributes_only) return;
     hipLaunchKernelGGL((axis_fft_kernel<NT, MAXV>), dim3(blocks), dim3(NT), lds, s, a);
 }
 

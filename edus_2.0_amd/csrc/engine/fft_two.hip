@@ -325,6 +325,7 @@ void launch_mode(const AxisFFTArgs& a, int blocks, size_t lds, hipStream_t s)
             (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
                                       int((Sh::n + maxl * Sh::LS) * sizeof(double2)));
         });
+    if (g_attributes_only) return;
     hipLaunchKernelGGL(kern, dim3(blocks), dim3(NT), lds, s, a);
 }
 

@@ -8,6 +8,17 @@
 
 namespace edus {
 
+// While set (by an AttributesOnly guard), the launch functions only raise
+// their kernels' dynamic-LDS limit (hipFuncSetAttribute, once per kernel) and
+// launch nothing.  Engines do this for every kernel they may launch before
+// any hipGraph capture: a limit first raised while a stream is being captured
+// does not reach the captured launch.
+extern thread_local bool g_attributes_only;
+struct AttributesOnly {
+    AttributesOnly() { g_attributes_only = true; }
+    ~AttributesOnly() { g_attributes_only = false; }
+};
+
 struct FFTFactors {
     int count;
     int radix[kMaxFactors];
@@ -82,6 +93,14 @@ struct CommArgs {
     const double2* add[4];
     int add_dd[4];
     int n_add;
+    // the FP32 commutator engine (mixed precision) reads H0 and xi stored in
+    // FP32 instead (H0f, xif; required for it); launch_comm fills addf with
+    // H0f (addf_dd -1) and the xif present, and keeps Sigma in `sigma`
+    const float2* H0f;
+    const float2* xif[3];
+    const float2* addf[4];
+    int addf_dd[4];
+    int n_addf;
 };
 
 // derivative coefficient of the line in plane `plane`
@@ -125,6 +144,41 @@ void launch_phase_table(double2* tab, int n, double coef, double scale, hipStrea
 enum CommKind { kCommScalar = 0, kCommMfma64 = 1, kCommMfma32 = 2 };
 // tile 0: 8 k-points per work-group, 1: 4 (MFMA engines; the FMA engine has one)
 void launch_comm(int nb, int layout, int kind, int tile, const CommArgs& a, hipStream_t s);
+
+// The spectral derivative of every plane in one pass (grad_fused.hip): out =
+// F0^-1 i c0 R0 F0 in + F1^-1 i c1 R1 F1 in, the two-pass axis kernels'
+// result to rounding; one work-group per plane, for the grid shapes
+// grad_fused_available accepts.  coef = c_j / N_j, or per instance from
+// coef_tab[(plane / ppi) * coef_stride + j].
+struct GradFusedArgs {
+    const double2* in;
+    double2* out;
+    int P, N0, N1;
+    double coef[2];
+    const double* coef_tab;
+    int coef_stride, ppi;
+    const double2* tw0;  // exp(-2 pi i j / N0), j < N0
+    const double2* tw1;
+};
+bool grad_fused_available(int N0, int N1);
+void launch_grad_fused(const GradFusedArgs& a, hipStream_t s);
+
+// Gauge rotation per k (Operator::go_to_bloch / go_to_wannier): out =
+// U^dagger in U (to_bloch) or U in U^dagger, in is planes [npl][Nk] of the
+// launch's input layout, U full planes [nb*nb][Nk] (U(a, n) in plane a*nb + n,
+// columns the eigenvectors), out in out_layout: kFull or kHermitian planes, or
+// kRotKMajor ([Nk][nb][nb], the reference's BlockMatrix order).  kind:
+// kCommMfma64 (FP64) or kCommMfma32 (FP32 operands and accumulation).
+constexpr int kRotKMajor = 2;
+struct RotateArgs {
+    int Nk;
+    const double2* in;
+    const double2* U;
+    double2* out;
+    int out_layout;
+    int to_bloch;
+};
+void launch_rotate(int nb, int in_layout, int kind, const RotateArgs& a, hipStream_t s);
 
 // k-space finite-difference gradient (gradient_space "k"): one stencil point
 // per mesh vector b of the shells in use: grid offset, shell weight, Cartesian b

@@ -85,7 +85,7 @@ __global__ void full_to_planes_kernel(const double2* __restrict__ full, double2*
 constexpr int kObsBlock = 128;  // keeps the (2nb+6) x block LDS tile of the observables kernel <= 38 KB
 
 // Per-block partial sums of the observables over k:
-//   vals[0..nb)       sum_k Re (U^dagger rho U)_nn      (Bloch population)
+//   vals[0..nb)       sum_k Re (U^dagger rho U)_nn      (Bloch population, from the rotated field)
 //   vals[nb..2nb)     sum_k Re rho_nn                    (Wannier population)
 //   vals[2nb..2nb+6)  sum_k Tr(V_d rho)  (re, im per d)
 __global__ void __launch_bounds__(kObsBlock) observables_kernel(ObsArgs a)
@@ -97,19 +97,8 @@ __global__ void __launch_bounds__(kObsBlock) observables_kernel(ObsArgs a)
     for (int v = 0; v < nv; ++v) sh[v * kObsBlock + threadIdx.x] = 0.;
     if (k < Nk) {
         for (int n = 0; n < nb; ++n) sh[(nb + n) * kObsBlock + threadIdx.x] = element(a.rho, n, n, nb, L, Nk, k).x;
-        if (a.U) {
-            for (int n = 0; n < nb; ++n) {
-                double pr = 0.;
-                for (int i = 0; i < nb; ++i) {
-                    double2 w = make_double2(0., 0.);
-                    for (int j = 0; j < nb; ++j)
-                        w = cadd(w, cmul(element(a.rho, i, j, nb, L, Nk, k), a.U[long(j * nb + n) * Nk + k]));
-                    const double2 ui = a.U[long(i * nb + n) * Nk + k];
-                    pr += ui.x * w.x + ui.y * w.y;  // Re(conj(u_i) w_i)
-                }
-                sh[n * kObsBlock + threadIdx.x] = pr;
-            }
-        }
+        if (a.rho_bloch)
+            for (int n = 0; n < nb; ++n) sh[n * kObsBlock + threadIdx.x] = a.rho_bloch[long(n * nb + n) * Nk + k].x;
         for (int d = 0; d < 3; ++d) {
             if (!a.vel[d]) continue;
             double2 tr = make_double2(0., 0.);
@@ -212,6 +201,12 @@ __global__ void axpy_planes_kernel(double2* __restrict__ y, const double2* __res
     if (i < n) y[i] = make_double2(y[i].x + a * x[i].x, y[i].y + a * x[i].y);
 }
 
+__global__ void to_f32_kernel(const double2* __restrict__ in, float2* __restrict__ out, long n)
+{
+    const long i = long(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = make_float2(float(in[i].x), float(in[i].y));
+}
+
 // H_nn(k) += scale * d_n for every k
 __global__ void add_diag_kernel(double2* __restrict__ H, const double2* __restrict__ d, double scale, int nb, int Nk,
                                 int layout)
@@ -270,6 +265,10 @@ void launch_hartree(const double2* rho, int nb, int Nk, int layout, int P, int B
 void launch_axpy(double2* y, const double2* x, double a, long n, hipStream_t s)
 {
     hipLaunchKernelGGL(axpy_planes_kernel, dim3(grid1d(n, 256)), dim3(256), 0, s, y, x, a, n);
+}
+void launch_to_f32(const double2* in, float2* out, long n, hipStream_t s)
+{
+    hipLaunchKernelGGL(to_f32_kernel, dim3(grid1d(n, 256)), dim3(256), 0, s, in, out, n);
 }
 void launch_add_diag(double2* H, const double2* d, double scale, int nb, int Nk, int layout, hipStream_t s)
 {

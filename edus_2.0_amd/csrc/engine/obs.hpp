@@ -9,7 +9,7 @@ namespace edus {
 struct ObsArgs {
     int nb, Nk, layout;
     const double2* rho;      // planes in `layout`
-    const double2* U;        // full planes [nb*nb][Nk] or null
+    const double2* rho_bloch;  // U^dagger rho U as full planes [nb*nb][Nk] (launch_rotate), or null
     const double2* vel[3];   // planes in `layout` or null
     double* partial;         // [blocks][2nb+6]
 };
@@ -25,6 +25,8 @@ int diag_blocks(int Nk);
 void launch_hartree(const double2* rho, int nb, int Nk, int layout, int P, int B, const double* hart,
                     double2* partial, double2* hdiag, hipStream_t s);
 void launch_axpy(double2* y, const double2* x, double a, long n, hipStream_t s);
+// out = in rounded to FP32 (nearest), n complex values
+void launch_to_f32(const double2* in, float2* out, long n, hipStream_t s);
 void launch_add_diag(double2* H, const double2* d, double scale, int nb, int Nk, int layout, hipStream_t s);
 
 }  // namespace edus

@@ -95,6 +95,12 @@ __device__ inline void pair_of(int i, int nb, int tri, int& a, int& b)
 }
 
 // U[k][a][n] -> Uv[k][v][a] and the conduction planes Uc[(c nb + a)][k]
+__global__ void lindhard_zero_kernel(double2* __restrict__ X, long n)
+{
+    const long i = long(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) X[i] = make_double2(0.0, 0.0);
+}
+
 __global__ void lindhard_split_kernel(const double2* __restrict__ U, double2* __restrict__ Uv,
                                       double2* __restrict__ Uc, int nb, int nv, int Nk)
 {
@@ -423,8 +429,10 @@ void launch_lindhard(const LindhardPlan& p, const double2* U, const double* eps,
                      hipStream_t s)
 {
     const size_t bytes = size_t(p.nb) * p.nb * p.Nk * sizeof(double2);
-    if (p.nv == 0 || p.nc == 0) {  // no valence-conduction pair: X = 0
-        EDUS_HIP_CHECK(hipMemsetAsync(X, 0, bytes, s));
+    if (p.nv == 0 || p.nc == 0) {  // no valence-conduction pair: X = 0 (a kernel: callers may capture this)
+        const long n = long(bytes / sizeof(double2));
+        hipLaunchKernelGGL(lindhard_zero_kernel, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, X, n);
+        EDUS_HIP_CHECK(hipGetLastError());
         return;
     }
     double2* Uv = work;

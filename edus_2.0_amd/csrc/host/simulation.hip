@@ -271,21 +271,11 @@ Simulation::Simulation(const edus_sim_params& p) : par(p)
         setoflaser.lasers.push_back(L);
     }
 
-    // eigensystem (Simulation.cpp:263-286) and initial condition (Functional_InitialCondition.hpp:12-32)
+    // eigensystem (Simulation.cpp:263-286); the initial condition
+    // (Functional_InitialCondition.hpp:12-32) is rotated on the device below
     U.assign(size_t(Nk) * M, 0.);
     eps.assign(size_t(Nk) * nb, 0.);
-    rho0.assign(size_t(Nk) * M, 0.);
-    parallel_for(Nk, [&](int ik) {
-        hermitian_eigen(nb, &H0k[size_t(ik) * M], &eps[size_t(ik) * nb], &U[size_t(ik) * M]);
-        const cplx* u = &U[size_t(ik) * M];
-        cplx* r = &rho0[size_t(ik) * M];
-        for (int a = 0; a < nb; ++a)
-            for (int b = 0; b < nb; ++b) {
-                cplx s = 0.;
-                for (int n = 0; n < filled; ++n) s += u[a * nb + n] * std::conj(u[b * nb + n]);
-                r[a * nb + b] = s;
-            }
-    });
+    parallel_for(Nk, [&](int ik) { hermitian_eigen(nb, &H0k[size_t(ik) * M], &eps[size_t(ik) * nb], &U[size_t(ik) * M]); });
 
     if (p.coulomb) setup_coulomb(p);
 
@@ -302,7 +292,7 @@ Simulation::Simulation(const edus_sim_params& p) : par(p)
     if (p.coulomb) {
         d.W = reinterpret_cast<const double*>(W.data());
         d.hartree = reinterpret_cast<const double*>(hartree.data());
-        d.rho0k = reinterpret_cast<const double*>(rho0.data());
+        d.rho0k = nullptr;  // the engine's initial condition (Coulomb::set_DM0 of the IC, Simulation.cpp:177-181)
     }
     d.U = reinterpret_cast<const double*>(U.data());
     d.filledbands = filled;
@@ -311,8 +301,10 @@ Simulation::Simulation(const edus_sim_params& p) : par(p)
     peierls = p.peierls != 0;
     EngineOptions opt;
     opt.velocity_det = jacobian();  // velocity operator built on the device (Simulation.cpp:523-576)
+    opt.initial_filled = filled;     // rho(t=0) = U diag(filled) U^dagger on the device
     engine = std::make_unique<Engine>(d, opt);
-    engine->upload(reinterpret_cast<const double*>(rho0.data()), engine->y);
+    rho0.assign(size_t(Nk) * M, 0.);  // host copy of the initial state (DM0.txt, edus_sim_get)
+    engine->download(engine->y, reinterpret_cast<double*>(rho0.data()));
 
     const std::string solver = p.solver;
     desolver.initialize(engine.get(), &setoflaser, solver == "AB" ? DESolver::AB : DESolver::RK, p.order,
@@ -417,25 +409,9 @@ void Simulation::h5_record(double t)
             f.write_doubles(g + "/0/local", reinterpret_cast<const double*>(rho.data()), 2 * rho.size());
         }
         if (par.toprint_dmk_bloch) {
-            // go_to_bloch (Operator.hpp:528-564): O_b = U^dagger O_w U per k
+            // go_to_bloch (Operator.hpp:528-564): O_b = U^dagger O_w U per k, on the device
             std::vector<cplx> rb(rho.size());
-            parallel_for(Nk, [&](int ik) {
-                const cplx* u = &U[size_t(ik) * M];
-                const cplx* r = &rho[size_t(ik) * M];
-                std::vector<cplx> tmp(M);
-                for (int a = 0; a < nb; ++a)
-                    for (int m = 0; m < nb; ++m) {
-                        cplx sum = 0.;
-                        for (int b = 0; b < nb; ++b) sum += r[a * nb + b] * u[b * nb + m];
-                        tmp[a * nb + m] = sum;
-                    }
-                for (int n = 0; n < nb; ++n)
-                    for (int m = 0; m < nb; ++m) {
-                        cplx sum = 0.;
-                        for (int a = 0; a < nb; ++a) sum += std::conj(u[a * nb + n]) * tmp[a * nb + m];
-                        rb[size_t(ik) * M + n * nb + m] = sum;
-                    }
-            });
+            engine->download_bloch(engine->y, reinterpret_cast<double*>(rb.data()));
             const std::string g = "/DensityMatrix_k_bloch/" + node;
             f.group(g);
             f.group(g + "/0");
@@ -835,6 +811,7 @@ int edus_sim_propagate(edus_sim* sim, int nsteps, double* times, double* laser, 
         const int n = nsteps < 0 ? sim->s->nsteps : nsteps;
         const int k = sim->s->propagate(n, times, laser, pop_bloch, pop_wannier, vel);
         if (nprinted) *nprinted = k;
+        sim->s->engine->enforce_monitor_limits();
     });
 }
 
@@ -935,7 +912,10 @@ int edus_sim_write_screened(edus_sim* sim, const char* dir)
 int edus_sim_steps(edus_sim* sim, int nsteps)
 {
     if (!sim) { edus::set_error("null simulation"); return EDUS_ERR_ARG; }
-    return guarded_sim([&] { sim->s->desolver.Propagate(nsteps); });
+    return guarded_sim([&] {
+        sim->s->desolver.Propagate(nsteps);
+        sim->s->engine->enforce_monitor_limits();
+    });
 }
 
 long edus_sim_get(edus_sim* sim, const char* name, double* out)
@@ -955,6 +935,9 @@ long edus_sim_get(edus_sim* sim, const char* name, double* out)
     else if (n == "eps") {
         if (out) std::memcpy(out, s.eps.data(), s.eps.size() * sizeof(double));
         return long(s.eps.size());
+    } else if (n == "A") {  // lattice, A[i][j] = component i of lattice vector j (bohr)
+        if (out) std::memcpy(out, s.material.A, 9 * sizeof(double));
+        return 9;
     } else if (n == "Vel0" || n == "Vel1" || n == "Vel2") {
         const int d = n[3] - '0';
         const long cnt = long(s.Nk) * s.nb * s.nb;

@@ -198,14 +198,15 @@ class Parareal {
     double2* U = nullptr;         // N + 1 slice boundaries
     double2* Gp = nullptr;        // N previous G results
     double2* Gt = nullptr;        // G result of the current slice
-    float *mu = nullptr, *sig = nullptr, *tmu = nullptr, *tsig = nullptr, *Ed = nullptr, *Y = nullptr;
+    float *mu = nullptr, *sig = nullptr, *tmu = nullptr, *tsig = nullptr, *Ed = nullptr;
+    float* Ysweep = nullptr;      // the network's output of every slice in the first (coarse) sweep
     uint16_t* X = nullptr;
     unsigned long long* dmax = nullptr;
     double t_fine = 0.0, t_coarse = 0.0;
 
     void set_fields(const double* Ef, const double* Ec, const float* Ee, int ne_);
     void set_net(Mlp* m, const float* mu_, const float* sig_, const float* tmu_, const float* tsig_);
-    void gop(const double2* in, int n, double2* out);
+    void gop(const double2* in, int n, double2* out, bool first);
     void run(const double* rho0, edus_parareal_result* res);
 
   private:
@@ -268,7 +269,8 @@ void Parareal::set_fields(const double* Ef, const double* Ec, const float* Ee, i
     din = 2 * q + 2 * ne + 4;
     if (din > kMaxDin) throw EngineError(EDUS_ERR_ARG, "too many field samples per slice");
     Ed = static_cast<float*>(alloc(std::max<size_t>(Efeat.size(), 1) * sizeof(float)));
-    HIP_CHECK(hipMemcpy(Ed, Efeat.data(), Efeat.size() * sizeof(float), hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemcpyAsync(Ed, Efeat.data(), Efeat.size() * sizeof(float), hipMemcpyHostToDevice, coarse->s));
+    HIP_CHECK(hipStreamSynchronize(coarse->s));
 }
 
 void Parareal::set_net(Mlp* m, const float* mu_, const float* sig_, const float* tmu_, const float* tsig_)
@@ -282,16 +284,25 @@ void Parareal::set_net(Mlp* m, const float* mu_, const float* sig_, const float*
     sig = static_cast<float*>(alloc(din * sizeof(float)));
     tmu = static_cast<float*>(alloc(q * sizeof(float)));
     tsig = static_cast<float*>(alloc(q * sizeof(float)));
-    HIP_CHECK(hipMemcpy(mu, mu_, din * sizeof(float), hipMemcpyHostToDevice));
-    HIP_CHECK(hipMemcpy(sig, sig_, din * sizeof(float), hipMemcpyHostToDevice));
-    HIP_CHECK(hipMemcpy(tmu, tmu_, q * sizeof(float), hipMemcpyHostToDevice));
-    HIP_CHECK(hipMemcpy(tsig, tsig_, q * sizeof(float), hipMemcpyHostToDevice));
+    // on the stream the network's inputs are built on (see gop)
+    const hipStream_t cs = coarse->s;
+    HIP_CHECK(hipMemcpyAsync(mu, mu_, din * sizeof(float), hipMemcpyHostToDevice, cs));
+    HIP_CHECK(hipMemcpyAsync(sig, sig_, din * sizeof(float), hipMemcpyHostToDevice, cs));
+    HIP_CHECK(hipMemcpyAsync(tmu, tmu_, q * sizeof(float), hipMemcpyHostToDevice, cs));
+    HIP_CHECK(hipMemcpyAsync(tsig, tsig_, q * sizeof(float), hipMemcpyHostToDevice, cs));
+    HIP_CHECK(hipStreamSynchronize(cs));
     X = static_cast<uint16_t*>(alloc(size_t(base->Nk) * din * sizeof(uint16_t)));
-    Y = static_cast<float*>(alloc(size_t(base->Nk) * q * sizeof(float)));
+    Ysweep = static_cast<float*>(alloc(size_t(N) * base->Nk * q * sizeof(float)));
 }
 
-// out = G(in) for slice n
-void Parareal::gop(const double2* in, int n, double2* out)
+// out = G(in) for slice n.  The network is evaluated in the first (coarse)
+// sweep only, on that sweep's slice-start states, and its correction is
+// reused unchanged by every later iteration: it then cancels exactly in
+// G(U_new) - G(U_old), so the iterations see the coarse RK4's sensitivity
+// and the network only moves the starting point.  (Re-evaluating it would
+// add the jumps of its BF16-rounded inputs to that difference, at the scale
+// of the tolerance.)
+void Parareal::gop(const double2* in, int n, double2* out, bool first)
 {
     const auto t0 = std::chrono::steady_clock::now();
     Engine& c = *coarse;
@@ -300,25 +311,28 @@ void Parareal::gop(const double2* in, int n, double2* out)
     HIP_CHECK(hipStreamSynchronize(c.s));
     c.rk4_steps(dtc, prm.coarse_steps, Ecoarse.data() + size_t(n) * prm.coarse_steps * 9);
     if (net) {
-        FeatArgs a{};
-        a.nb = c.nb;
-        a.layout = c.layout;
-        a.N0 = c.N0;
-        a.N1 = c.N1;
-        a.ne = ne;
-        a.din = din;
-        a.U = in;
-        a.R = c.y;
-        a.E = Ed + size_t(n) * ne * 2;
-        a.mu = mu;
-        a.sig = sig;
-        a.X = X;
-        hipLaunchKernelGGL(features_kernel, dim3((c.Nk + 127) / 128), dim3(128), 0, c.s, a);
-        const hipStream_t own = net->s;
-        net->s = c.s;  // keep the network's launches in order with the engine's
-        net->forward(X, c.Nk, Y, false);
-        net->s = own;
-        hipLaunchKernelGGL(apply_kernel, dim3((c.Nk + 255) / 256), dim3(256), 0, c.s, c.y, Y, tmu, tsig, c.nb,
+        float* Yn = Ysweep + size_t(n) * c.Nk * q;
+        if (first) {
+            FeatArgs a{};
+            a.nb = c.nb;
+            a.layout = c.layout;
+            a.N0 = c.N0;
+            a.N1 = c.N1;
+            a.ne = ne;
+            a.din = din;
+            a.U = in;
+            a.R = c.y;
+            a.E = Ed + size_t(n) * ne * 2;
+            a.mu = mu;
+            a.sig = sig;
+            a.X = X;
+            hipLaunchKernelGGL(features_kernel, dim3((c.Nk + 127) / 128), dim3(128), 0, c.s, a);
+            const hipStream_t own = net->s;
+            net->s = c.s;  // keep the network's launches in order with the engine's
+            net->forward(X, c.Nk, Yn, false);
+            net->s = own;
+        }
+        hipLaunchKernelGGL(apply_kernel, dim3((c.Nk + 255) / 256), dim3(256), 0, c.s, c.y, Yn, tmu, tsig, c.nb,
                            c.layout, long(c.Nk));
     }
     HIP_CHECK(hipMemcpyAsync(out, c.y, ie * sizeof(double2), hipMemcpyDeviceToDevice, c.s));
@@ -332,18 +346,25 @@ void Parareal::run(const double* rho0, edus_parareal_result* res)
     const auto t_start = std::chrono::steady_clock::now();
     t_fine = t_coarse = 0.0;
     Engine& c = *coarse;
-    if (rho0)
+    // Every device-to-device copy below is stream-ordered: the engines run on
+    // non-blocking streams, which do not wait for work on the null stream (a
+    // plain hipMemcpy between device buffers may return before it lands).
+    if (rho0) {
         c.upload(rho0, U);
-    else
-        HIP_CHECK(hipMemcpy(U, base->y, ie * sizeof(double2), hipMemcpyDeviceToDevice));
-    // iteration 0: the coarse sweep
-    for (int n = 0; n < N; ++n) {
-        gop(U + n * ie, n, Gp + n * ie);
-        HIP_CHECK(hipMemcpy(U + (n + 1) * ie, Gp + n * ie, ie * sizeof(double2), hipMemcpyDeviceToDevice));
+    } else {
+        HIP_CHECK(hipStreamSynchronize(base->s));
+        HIP_CHECK(hipMemcpyAsync(U, base->y, ie * sizeof(double2), hipMemcpyDeviceToDevice, c.s));
     }
+    // iteration 0: the coarse sweep (gop reads U_n on c.s, behind these copies)
+    for (int n = 0; n < N; ++n) {
+        gop(U + n * ie, n, Gp + n * ie, true);
+        HIP_CHECK(hipMemcpyAsync(U + (n + 1) * ie, Gp + n * ie, ie * sizeof(double2), hipMemcpyDeviceToDevice, c.s));
+    }
+    HIP_CHECK(hipStreamSynchronize(c.s));
     int it = 0;
     bool conv = false;
     double diff = 0.0;
+    double hist[16] = {};
     while (it < prm.max_iter && !conv) {
         ++it;
         // F(U_n) for every slice, one stacked batch
@@ -357,7 +378,7 @@ void Parareal::run(const double* rho0, edus_parareal_result* res)
         // correction sweep; U_0 .. U_{it-1} are already exact
         HIP_CHECK(hipMemsetAsync(dmax, 0, sizeof(unsigned long long), c.s));
         for (int n = it - 1; n < N; ++n) {
-            gop(U + n * ie, n, Gt);
+            gop(U + n * ie, n, Gt, false);
             const long ne_ = long(ie);
             hipLaunchKernelGGL(combine_kernel, dim3(std::min<long>((ne_ + 255) / 256, 1024)), dim3(256), 0, c.s,
                                U + (n + 1) * ie, Gt, fine->state(n), Gp + n * ie, ne_, dmax);
@@ -367,6 +388,7 @@ void Parareal::run(const double* rho0, edus_parareal_result* res)
         HIP_CHECK(hipMemcpyAsync(&m, dmax, sizeof(m), hipMemcpyDeviceToHost, c.s));
         HIP_CHECK(hipStreamSynchronize(c.s));
         std::memcpy(&diff, &m, sizeof(diff));
+        if (it <= 16) hist[it - 1] = diff;
         conv = diff <= prm.tol;
     }
     bool fallback = false;
@@ -378,11 +400,15 @@ void Parareal::run(const double* rho0, edus_parareal_result* res)
             seq = std::make_unique<Engine>(std::vector<const Engine*>{base});
             seq->graph_steps = prm.graph_steps;
         }
-        HIP_CHECK(hipMemcpy(seq->y, U, ie * sizeof(double2), hipMemcpyDeviceToDevice));
+        HIP_CHECK(hipMemcpyAsync(seq->y, U, ie * sizeof(double2), hipMemcpyDeviceToDevice, seq->s));
+        HIP_CHECK(hipStreamSynchronize(seq->s));
         for (int n = 0; n < N; ++n) {
             seq->rk4_steps(prm.dt, prm.fine_steps, Eseq.data() + size_t(n) * prm.fine_steps * 9);
             HIP_CHECK(hipStreamSynchronize(seq->s));
-            HIP_CHECK(hipMemcpy(U + (n + 1) * ie, seq->y, ie * sizeof(double2), hipMemcpyDeviceToDevice));
+            // finished before the next slice's steps overwrite the state
+            HIP_CHECK(hipMemcpyAsync(U + (n + 1) * ie, seq->y, ie * sizeof(double2), hipMemcpyDeviceToDevice,
+                                     seq->s));
+            HIP_CHECK(hipStreamSynchronize(seq->s));
         }
         t_fine += seconds_since(tf);
     }
@@ -396,6 +422,7 @@ void Parareal::run(const double* rho0, edus_parareal_result* res)
         res->seconds = seconds_since(t_start);
         res->fine_seconds = t_fine;
         res->coarse_seconds = t_coarse;
+        std::memcpy(res->updates, hist, sizeof(hist));
     }
 }
 
@@ -418,6 +445,7 @@ void collect(Engine* base, const edus_parareal_params& p, int runs, const double
     Batch crs(std::make_unique<Engine>(std::vector<const Engine*>(runs, base)));
     fine.graph_steps = p.graph_steps;
     crs.graph_steps = p.graph_steps > 0 ? cs : 0;
+    const hipStream_t st = fine.bs;  // this function's own copies and kernels
     const size_t ie = base->inst_elems();
     std::vector<void*> bufs;
     auto dev = [&](size_t bytes) {
@@ -432,7 +460,10 @@ void collect(Engine* base, const edus_parareal_params& p, int runs, const double
         int* kl = static_cast<int*>(dev(size_t(samples) * sizeof(int)));
         float* Xd = static_cast<float*>(dev(size_t(samples) * din * sizeof(float)));
         float* Td = static_cast<float*>(dev(size_t(samples) * q * sizeof(float)));
-        if (ne > 0) HIP_CHECK(hipMemcpy(Ed, Ee, size_t(runs) * N * ne * 2 * sizeof(float), hipMemcpyHostToDevice));
+        if (ne > 0) {
+            HIP_CHECK(hipMemcpyAsync(Ed, Ee, size_t(runs) * N * ne * 2 * sizeof(float), hipMemcpyHostToDevice, st));
+            HIP_CHECK(hipStreamSynchronize(st));
+        }
         std::mt19937_64 rng(seed);
         std::vector<int> all(base->Nk);
         for (int k = 0; k < base->Nk; ++k) all[k] = k;
@@ -451,15 +482,19 @@ void collect(Engine* base, const edus_parareal_params& p, int runs, const double
                             ec[((size_t(s) * 3 + j) * runs + r) * 3 + d] =
                                 Ec[(((size_t(r) * N + n) * cs + s) * 3 + j) * 3 + d];
             }
+            // stream-ordered copies, finished before either batch steps on its own streams
             for (int r = 0; r < runs; ++r) {
-                HIP_CHECK(hipMemcpy(Ustart + r * ie, fine.state(r), ie * sizeof(double2), hipMemcpyDeviceToDevice));
-                HIP_CHECK(hipMemcpy(crs.state(r), fine.state(r), ie * sizeof(double2), hipMemcpyDeviceToDevice));
+                HIP_CHECK(hipMemcpyAsync(Ustart + r * ie, fine.state(r), ie * sizeof(double2), hipMemcpyDeviceToDevice,
+                                         st));
+                HIP_CHECK(hipMemcpyAsync(crs.state(r), fine.state(r), ie * sizeof(double2), hipMemcpyDeviceToDevice,
+                                         st));
             }
+            HIP_CHECK(hipStreamSynchronize(st));
             crs.rk4_steps(p.dt * fs / cs, cs, ec.data());
             fine.rk4_steps(p.dt, fs, ef.data());
             for (int r = 0; r < runs; ++r) {
                 std::shuffle(all.begin(), all.end(), rng);
-                HIP_CHECK(hipMemcpy(kl, all.data(), samples * sizeof(int), hipMemcpyHostToDevice));
+                HIP_CHECK(hipMemcpyAsync(kl, all.data(), samples * sizeof(int), hipMemcpyHostToDevice, st));
                 FeatArgs a{};
                 a.nb = base->nb;
                 a.layout = base->layout;
@@ -475,12 +510,14 @@ void collect(Engine* base, const edus_parareal_params& p, int runs, const double
                 a.rows = samples;
                 a.Xf = Xd;
                 a.Tf = Td;
-                hipLaunchKernelGGL(collect_kernel, dim3((samples + 127) / 128), dim3(128), 0, nullptr, a);
+                hipLaunchKernelGGL(collect_kernel, dim3((samples + 127) / 128), dim3(128), 0, st, a);
                 const size_t row0 = (size_t(r) * N + n) * samples;
-                HIP_CHECK(hipMemcpy(Xout + row0 * din, Xd, size_t(samples) * din * sizeof(float),
-                                    hipMemcpyDeviceToHost));
-                HIP_CHECK(hipMemcpy(Tout + row0 * q, Td, size_t(samples) * q * sizeof(float),
-                                    hipMemcpyDeviceToHost));
+                HIP_CHECK(hipMemcpyAsync(Xout + row0 * din, Xd, size_t(samples) * din * sizeof(float),
+                                         hipMemcpyDeviceToHost, st));
+                HIP_CHECK(hipMemcpyAsync(Tout + row0 * q, Td, size_t(samples) * q * sizeof(float),
+                                         hipMemcpyDeviceToHost, st));
+                // the k list is reshuffled and the buffers refilled for the next run
+                HIP_CHECK(hipStreamSynchronize(st));
             }
         }
         HIP_CHECK(hipDeviceSynchronize());

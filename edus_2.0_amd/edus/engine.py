@@ -42,12 +42,13 @@ class _Desc(C.Structure):
 class Tuning(C.Structure):
     """edus_tuning: kernel and tile choices of a context."""
     _fields_ = [("fft_family", C.c_int * 2), ("fft_cfg", C.c_int * 2), ("fft_lines", C.c_int * 2),
-                ("comm_kind", C.c_int), ("comm_tile", C.c_int), ("comm_tile_ab", C.c_int)]
+                ("comm_kind", C.c_int), ("comm_tile", C.c_int), ("comm_tile_ab", C.c_int), ("grad_fused", C.c_int)]
 
     def as_dict(self):
         return {"fft_family": list(self.fft_family), "fft_cfg": list(self.fft_cfg),
                 "fft_lines": list(self.fft_lines), "comm_kind": int(self.comm_kind),
-                "comm_tile": int(self.comm_tile), "comm_tile_ab": int(self.comm_tile_ab)}
+                "comm_tile": int(self.comm_tile), "comm_tile_ab": int(self.comm_tile_ab),
+                "grad_fused": int(self.grad_fused)}
 
 
 class _Laser(C.Structure):
@@ -85,6 +86,7 @@ class _SimParams(C.Structure):
 
 EXPORTS = [
     "edus_create", "edus_clone", "edus_destroy", "edus_last_error", "edus_set_state", "edus_get_state", "edus_rhs",
+    "edus_rotate", "edus_get_state_bloch", "edus_monitor_limits",
     "edus_rk4_steps", "edus_ab_steps", "edus_ab_reset", "edus_ab_seed", "edus_rhs_peierls",
     "edus_rk4_steps_peierls", "edus_ab_steps_peierls", "edus_observables_peierls", "edus_observables", "edus_check", "edus_synchronize", "edus_graph",
     "edus_timing_enable", "edus_timing_get", "edus_get_tuning", "edus_set_tuning", "edus_kgrad_stencil",
@@ -114,6 +116,8 @@ def lib():
         for name in ("edus_set_state", "edus_get_state"):
             getattr(L, name).argtypes = [vp, dp]
         L.edus_rhs.argtypes = [vp, dp, dp, dp]
+        L.edus_rotate.argtypes = [vp, C.c_int, dp, dp]
+        L.edus_get_state_bloch.argtypes = [vp, dp]
         L.edus_rk4_steps.argtypes = [vp, C.c_double, C.c_int, dp]
         L.edus_ab_steps.argtypes = [vp, C.c_double, C.c_int, C.c_int, dp]
         L.edus_ab_reset.argtypes = [vp]
@@ -124,6 +128,7 @@ def lib():
         L.edus_observables_peierls.argtypes = [vp, dp, dp, dp, dp]
         L.edus_observables.argtypes = [vp, dp, dp, dp]
         L.edus_check.argtypes = [vp, dp, dp]
+        L.edus_monitor_limits.argtypes = [vp, C.c_double, C.c_double]
         L.edus_synchronize.argtypes = [vp]
         L.edus_graph.argtypes = [vp, C.c_int]
         L.edus_timing_enable.argtypes = [vp, C.c_int]
@@ -268,6 +273,18 @@ class Engine:
         _check(lib().edus_get_state(self.h, _dp(out.view(np.float64))))
         return out
 
+    def rotate(self, to_bloch: bool, x) -> np.ndarray:
+        """U^dagger x U (to_bloch) or U x U^dagger per k (edus_rotate)."""
+        a = _c128(x).reshape(self._shape())
+        out = np.empty_like(a)
+        _check(lib().edus_rotate(self.h, int(bool(to_bloch)), _dp(a.view(np.float64)), _dp(out.view(np.float64))))
+        return out
+
+    def get_state_bloch(self) -> np.ndarray:
+        out = np.empty(self._shape(), dtype=np.complex128)
+        _check(lib().edus_get_state_bloch(self.h, _dp(out.view(np.float64))))
+        return out
+
     def rhs(self, E, rho) -> np.ndarray:
         a = _c128(rho).reshape(self._shape())
         e = np.ascontiguousarray(E, dtype=np.float64)
@@ -314,6 +331,10 @@ class Engine:
         t, h = C.c_double(), C.c_double()
         _check(lib().edus_check(self.h, C.byref(t), C.byref(h)))
         return t.value, h.value
+
+    def monitor_limits(self, trace_limit: float, herm_limit: float) -> None:
+        """edus_monitor_limits: step calls fail with EDUS_ERR_MONITOR beyond these (0: off)."""
+        _check(lib().edus_monitor_limits(self.h, float(trace_limit), float(herm_limit)))
 
     def synchronize(self):
         _check(lib().edus_synchronize(self.h))

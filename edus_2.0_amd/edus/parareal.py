@@ -28,10 +28,12 @@ class Params(C.Structure):
 class Result(C.Structure):
     _fields_ = [("iterations", C.c_int), ("converged", C.c_int), ("fallback", C.c_int),
                 ("last_update", C.c_double), ("seconds", C.c_double), ("fine_seconds", C.c_double),
-                ("coarse_seconds", C.c_double)]
+                ("coarse_seconds", C.c_double), ("updates", C.c_double * 16)]
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_}
+        d = {k: getattr(self, k) for k, _ in self._fields_ if k != "updates"}
+        d["updates"] = list(self.updates)[:min(self.iterations, 16)]
+        return d
 
 
 def _bind():

@@ -100,6 +100,18 @@ const char* edus_last_error(void);
 int edus_set_state(edus_ctx* ctx, const double* rho_k);
 int edus_get_state(edus_ctx* ctx, double* rho_k);
 
+/* Gauge rotation of an operator field per k (Operator::go_to_bloch /
+ * go_to_wannier, src/Operator/Operator.hpp:528-564): out = U^dagger in U
+ * (to_bloch = 1) or U in U^dagger (to_bloch = 0) with the context's
+ * eigenvectors U (columns, as the descriptor's U).  in/out host arrays
+ * [Nk][nb][nb]; a HERMITIAN-layout context takes the Hermitian part of in.
+ * FP64 contexts rotate on the FP64 matrix cores, mixed-precision ones on the
+ * FP32 matrix cores (FP32 operands and accumulation). */
+int edus_rotate(edus_ctx* ctx, int to_bloch, const double* in, double* out);
+/* The current state in the Bloch gauge, U^dagger rho U, [Nk][nb][nb]: the
+ * reference's DMk_bloch output (Simulation.cpp:398-403 with Operator.hpp:528-564). */
+int edus_get_state_bloch(edus_ctx* ctx, double* rho_bloch);
+
 /* One right-hand-side evaluation f = SourceTerm(t, rho) with the field E
  * given (the reference evaluates SetOfLaser at t).  in/out host arrays. */
 int edus_rhs(edus_ctx* ctx, const double E[3], const double* in, double* out);
@@ -142,6 +154,14 @@ int edus_observables(edus_ctx* ctx, double* pop_bloch, double* pop_wannier, doub
  * Hermiticity defect max_{n<=m} |rho_nm - conj(rho_mn)| (HERMITIAN layout:
  * twice the largest |Im rho_nn|, the only part its storage leaves free). */
 int edus_check(edus_ctx* ctx, double* trace_dev, double* herm_dev);
+/* Limits on those monitors (no counterpart in the reference, which checks
+ * nothing while it propagates).  With a limit > 0 set, edus_rk4_steps,
+ * edus_ab_steps, their Peierls versions and edus_sim_steps /
+ * edus_sim_propagate wait for their steps and return EDUS_ERR_MONITOR when
+ * the monitors (the maxima since the last edus_check, which they leave in
+ * place) exceed a limit; a NaN or inf in any state then always fails.  Both
+ * 0 (the default): not checked, and the step calls stay asynchronous. */
+int edus_monitor_limits(edus_ctx* ctx, double trace_limit, double herm_limit);
 
 int edus_synchronize(edus_ctx* ctx);
 
@@ -155,8 +175,8 @@ int edus_graph(edus_ctx* ctx, int steps_per_graph);
 /* Kernel timing with HIP events on the context stream.  When enabled, every
  * launch of the named kernel class is bracketed by events; edus_timing_get
  * returns the summed milliseconds and launch count since the last reset.
- * Classes: "grad_col", "grad_row", "kgrad", "comm", "hf_row", "hf_col",
- * "peierls_col", "peierls_row". */
+ * Classes: "grad_col", "grad_row", "grad_fused", "kgrad", "comm", "hf_row",
+ * "hf_col", "peierls_col", "peierls_row", "rotate". */
 int edus_timing_enable(edus_ctx* ctx, int on);
 int edus_timing_get(edus_ctx* ctx, const char* kernel_class, double* total_ms, long* count);
 
@@ -175,9 +195,14 @@ int edus_timing_get(edus_ctx* ctx, const char* kernel_class, double* total_ms, l
  *                  1: 4 k-points, 256 threads; 2: 8 k-points, 512 threads
  *                  (1 and 2: MFMA engines only; the FMA engine takes 0)
  *   comm_tile_ab   the same for the Adams-Bashforth steps
+ *   grad_fused     1: the spectral derivative in one pass per plane (one
+ *                  work-group holds a plane: grids 120 x 120, 96 x 96,
+ *                  90 x 90, 48 x 48); 0: two axis passes through a work plane
+ *                  set.  Used when the field has components along both axes.
  * Within one family, FFT tiles give bit-identical results, and so do the
  * commutator tiles; the two FP64 commutator engines differ in summation order
- * only.
+ * only, and the two gradient forms in rounding only (the compiler contracts
+ * multiply-adds per kernel).
  * edus_set_tuning returns EDUS_ERR_ARG for a choice that does not fit. */
 typedef struct {
     int fft_family[2];
@@ -186,6 +211,7 @@ typedef struct {
     int comm_kind;
     int comm_tile;
     int comm_tile_ab;
+    int grad_fused;
 } edus_tuning;
 int edus_get_tuning(edus_ctx* ctx, edus_tuning* out);
 
@@ -201,7 +227,8 @@ int edus_set_tuning(edus_ctx* ctx, const edus_tuning* in);
 /* ---- batches ------------------------------------------------------------
  * Many contexts advanced together.  No counterpart in the reference (one
  * simulation per process).  Members with the same nb, grid, layout,
- * precision, exchange flag and filledbands are stacked into one group whose
+ * precision, exchange flag, filledbands, gradient space (and k-space stencil),
+ * commutator engine and gradient form are stacked into one group whose
  * kernels advance all of them per launch; groups run on their own streams.
  * The batch copies each member's set-up arrays and state at creation;
  * members must outlive the batch (store/load copy states to and from them). */
@@ -251,7 +278,8 @@ int edus_mlp_train(edus_mlp* mlp, int rows, const float* X, const float* T, int 
  * No counterpart in the reference.  The run of nslices * fine_steps RK4 steps
  * of dt from a context's state is cut into nslices slices; F = RK4 over a
  * slice (fine_steps steps), G = RK4 with coarse_steps steps, plus the per-k
- * network's correction when one is set.  Parareal iterations
+ * network's correction when one is set (evaluated in the first, coarse sweep
+ * and reused unchanged by later iterations).  Parareal iterations
  *     U_{n+1} <- G(U_n new) + F(U_n old) - G(U_n old)
  * run every slice's F as one stacked batch; they stop when the largest change
  * of a slice boundary is <= tol, and after max_iter iterations without that
@@ -267,6 +295,7 @@ typedef struct {
     int iterations, converged, fallback;
     double last_update;  /* max change of the last iteration */
     double seconds, fine_seconds, coarse_seconds;
+    double updates[16];  /* max change of iterations 1 .. min(iterations, 16), in order */
 } edus_parareal_result;
 typedef struct edus_parareal edus_parareal;
 /* clones `base` (its set-up; the run starts from its state unless rho0 is given) */
@@ -360,7 +389,8 @@ int edus_sim_steps(edus_sim* sim, int nsteps);
 int edus_sim_write_outputs(edus_sim* sim, const char* dir, int nsteps, int* nprinted);
 /* Set-up arrays for inspection: "H0k", "rk0", "rk1", "rk2", "U", "Vel0",
  * "Vel1", "Vel2", "W", "rho0" (initial state in k), "hartree" ([nb][nb]),
- * "eps" (real [Nk][nb]). */
+ * "eps" (real [Nk][nb]), "A" (9 reals, the edus_desc lattice A[i][j]).
+ * Returns the element count (complex elements; reals for "eps" and "A"). */
 long edus_sim_get(edus_sim* sim, const char* name, double* out);
 
 /* ---- screened-potential application (src/Screened_potential/) ---- */

@@ -924,6 +924,21 @@ bool Simulation::print_now(double t) const
     return (int(std::round(t / dt)) % par.printresolution_pulse == 0);  // Simulation.cpp:240-257
 }
 
+// Operator::go_to_bloch / go_to_wannier (Operator.hpp:528-564), in the
+// reference's association: to Bloch (U^dagger O) U, to Wannier (U O) U^dagger
+void Simulation::rotate(bool to_bloch, const std::vector<cplx>& in, std::vector<cplx>& out) const
+{
+    out.assign(size_t(Nk) * M, 0.);
+#pragma omp parallel for schedule(static)
+    for (int ik = 0; ik < Nk; ++ik) {
+        std::vector<cplx> tmp(M);
+        const cplx* u = &U[size_t(ik) * M];
+        const cplx* ud = &Udag[size_t(ik) * M];
+        zgemm(nb, 1., to_bloch ? ud : u, &in[size_t(ik) * M], 0., tmp.data());
+        zgemm(nb, 1., tmp.data(), to_bloch ? u : ud, 0., &out[size_t(ik) * M]);
+    }
+}
+
 void Simulation::populations(std::vector<double>& bloch, std::vector<double>& wannier) const
 {
     // Simulation.cpp:439-470: go_to(bloch) = U^dagger rho U, then R=0 element = k-mean

@@ -173,6 +173,7 @@ class Simulation {
     void step();                                    // DESolver::Propagate
     void propagate(Observables* obs, int nsteps_override = -1);
     void populations(std::vector<double>& bloch, std::vector<double>& wannier) const;
+    void rotate(bool to_bloch, const std::vector<cplx>& in, std::vector<cplx>& out) const;
     void velocity(double v[6]) const;
     void rho_R(std::vector<cplx>& out) const;       // FFT k->R of rho (1/N)
     // Wannier centres (Model.hpp:72-78): the diagonal of r(R = 0), bohr

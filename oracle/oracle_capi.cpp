@@ -317,6 +317,17 @@ int oracle_kgradient(void* h, const double* u, const double* in, double* out)
     return 0;
 }
 
+// Gauge rotation with the oracle's eigenvectors (Operator.hpp:528-564):
+// to_bloch: out = U^dagger in U, else U in U^dagger; [k][n][m] interleaved
+void oracle_rotate(void* h, int to_bloch, const double* in, double* out)
+{
+    auto* s = static_cast<Simulation*>(h);
+    const size_t n = size_t(s->Nk) * s->M;
+    std::vector<cplx> a(reinterpret_cast<const cplx*>(in), reinterpret_cast<const cplx*>(in) + n), o;
+    s->rotate(to_bloch != 0, a, o);
+    std::copy(o.begin(), o.end(), reinterpret_cast<cplx*>(out));
+}
+
 // Host memory bandwidth in bytes/s (bytes read + written) of a parallel copy
 // between two arrays of `bytes` each, the best of `reps` passes after one
 // untimed pass: the bandwidth context for the CPU baseline's byte model.

@@ -90,6 +90,7 @@ def lib():
         L.oracle_fft2d.argtypes = [C.c_int, C.c_int, dp, C.c_int]
         L.oracle_struve0.restype = C.c_double
         L.oracle_struve0.argtypes = [C.c_double]
+        L.oracle_rotate.argtypes = [vp, C.c_int, dp, dp]
         L.oracle_host_bandwidth.restype = C.c_double
         L.oracle_host_bandwidth.argtypes = [C.c_long, C.c_int]
         _LIB = L
@@ -189,6 +190,13 @@ class Oracle:
         a = np.ascontiguousarray(rho, dtype=np.complex128)
         out = np.empty_like(a)
         lib().oracle_rhs(self.h, float(t), _ptr(a.view(np.float64)), _ptr(out.view(np.float64)))
+        return out
+
+    def rotate(self, to_bloch: bool, x: np.ndarray) -> np.ndarray:
+        """Operator::go_to_bloch (U^dagger x U) or go_to_wannier (U x U^dagger) per k."""
+        a = np.ascontiguousarray(x, dtype=np.complex128)
+        out = np.empty_like(a)
+        lib().oracle_rotate(self.h, int(bool(to_bloch)), _ptr(a.view(np.float64)), _ptr(out.view(np.float64)))
         return out
 
     def rhs_E(self, E, rho: np.ndarray) -> np.ndarray:

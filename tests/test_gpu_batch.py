@@ -145,3 +145,35 @@ def test_context_graph_replay_and_clone(model_dir, idx):
     assert np.array_equal(a, b)
     tr, _ = replay.check()
     assert tr < 1e-10
+
+
+def test_batch_groups_split_by_gradient_space_and_engine(model_dir):
+    """Members of one shape that differ in gradient space (spectral R vs
+    finite-difference k) or in the forced commutator engine form separate
+    groups, and each still ends where it ends alone."""
+    p, o = _setups(model_dir)[0]
+    pk = dict(p, gradient_space="k")
+    ok = oracle_py.Oracle(pk)
+    m_r = _engine(p, o)
+    m_k = edus.Engine(ok.nb, ok.N0, ok.N1, H0k=ok.get("H0k"), xik=(ok.get("rk0"), ok.get("rk1"), ok.get("rk2")),
+                      A=ok.lattice(), filledbands=pk["filledbands"], U=ok.get("U"), gradient_space="k")
+    m_k.set_state(ok.rho)
+    m_e = _engine(p, o)
+    t = m_e.get_tuning()
+    t.comm_kind = 1 - t.comm_kind  # the other FP64 engine (FMA <-> MFMA)
+    t.comm_tile = t.comm_tile_ab = 0
+    m_e.set_tuning(t)
+    members = [m_r, m_k, m_e]
+    alone = [m.clone() for m in members]
+    n, dt = 3, 0.05
+    E = _fields(6, n, dt, 9)
+    b = edus.Batch(members)
+    ng, mg = b.groups()
+    _say("groups", ng, mg)
+    assert ng == 3 and len(set(mg)) == 3
+    b.rk4_steps(dt, E)
+    for i in range(n):
+        alone[i].rk4_steps(dt, E[:, :, i, :])
+        b.store(i)
+        assert np.array_equal(members[i].get_state(), alone[i].get_state()), i
+    b.close()

@@ -96,3 +96,52 @@ def test_c2_ab4_exchange_steps_at_benchmark_shape(tmp_path):
     ref = o.propagate(16)
     got = sim.propagate(16)
     _compare(ref, got, np.abs(sim.engine.get_state() - o.rho).max())
+
+
+def test_c1_mixed_precision_at_benchmark_shape(c1):
+    """Mixed precision at the 14-band 120 x 120 shape (H0 and xi stored in
+    FP32, FP32-MFMA commutator and rotations; gradient, state and updates
+    FP64) against the FP64 oracle: one right-hand side, then 12 RK4 steps
+    at the pulse peak with populations and current.  Bars: RHS 2e-6 of
+    max|f|; rho 1e-6; populations 1e-6; current 1e-4 relative (measured
+    values printed)."""
+    p = dict(c1[0], printresolution=1, printresolution_pulse=1)
+    p["lasers"] = [dict(p["lasers"][0], t0=-10.0)]
+    o = oracle_py.Oracle(p)
+    sim = edus.Simulation(p, precision=edus.PREC_MIXED)
+    assert sim.engine.get_tuning().comm_kind == 2
+    rng = np.random.default_rng(3)
+    a = rng.standard_normal((o.Nk, o.nb, o.nb)) + 1j * rng.standard_normal((o.Nk, o.nb, o.nb))
+    rho = o.rho + 0.05 * 0.5 * (a + a.conj().transpose(0, 2, 1))
+    E = [0.02, 0.0, 0.0]
+    f_ref = o.rhs_E(E, rho)
+    rel = np.abs(sim.engine.rhs(E, rho) - f_ref).max() / np.abs(f_ref).max()
+    _say("c1 mixed: RHS max|df| / max|f| =", rel)
+    assert 1e-12 < rel <= 2e-6
+    ref = o.propagate(12)
+    got = sim.propagate(12)
+    drho = np.abs(sim.engine.get_state() - o.rho).max()
+    dpop = np.abs(got["pop_bloch"] - ref["pop_bloch"]).max()
+    dvel = np.abs(got["velocity"] - ref["velocity"]).max() / np.abs(ref["velocity"]).max()
+    _say("c1 mixed, 12 RK4 steps: max|d rho|", drho, "max|d pop|", dpop, "rel d current", dvel)
+    assert drho <= 1e-6 and dpop <= 1e-6 and dvel <= 1e-4
+
+
+def test_c2_mixed_precision_at_benchmark_shape(tmp_path):
+    """Mixed precision at the 8-band 90 x 90 AB4 shape with Hartree + screened
+    exchange (H0, which holds the folded ground-state self energy, and the
+    exchange self energy stay FP64; the products run on the FP32 matrix cores)
+    against the FP64 oracle, 16 steps at the pulses' peak.  Bars: rho 1e-6;
+    populations 1e-6; current 1e-4 relative (measured values printed)."""
+    p = models.workload("hsex8_hex90", str(tmp_path))
+    p["printresolution"] = p["printresolution_pulse"] = 1
+    p["lasers"] = [dict(las, t0=-7.5) for las in p["lasers"]]
+    o = oracle_py.Oracle(p)
+    sim = edus.Simulation(p, precision=edus.PREC_MIXED)
+    ref = o.propagate(16)
+    got = sim.propagate(16)
+    drho = np.abs(sim.engine.get_state() - o.rho).max()
+    dpop = np.abs(got["pop_bloch"] - ref["pop_bloch"]).max()
+    dvel = np.abs(got["velocity"] - ref["velocity"]).max() / np.abs(ref["velocity"]).max()
+    _say("c2 mixed, 16 AB4 steps: max|d rho|", drho, "max|d pop|", dpop, "rel d current", dvel)
+    assert drho <= 1e-6 and dpop <= 1e-6 and dvel <= 1e-4

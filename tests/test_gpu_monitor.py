@@ -89,3 +89,48 @@ def test_monitor_trips_on_nan(layout):
     tr, herm = eng.check()
     _say("nan case", layout, tr, herm)
     assert np.isinf(tr) and np.isinf(herm)
+
+
+@pytest.mark.parametrize("solver", ["RK4", "AB4"])
+def test_monitor_limits_stop_the_step_calls(solver):
+    """edus_monitor_limits: a step call whose states exceed a set limit
+    returns EDUS_ERR_MONITOR (5) with the values in the message; within the
+    limits it succeeds; a NaN fails whatever the limits; limits of 0 check
+    nothing."""
+    nb, nf = 5, 2
+    eng = _engine(nb, 16, 12, FULL, edus.PREC_FP64, nf, seed=5)
+    rho = _planted(eng.Nk, nb, nf, seed=6)
+    etr, eherm = _expected(rho, nf)
+
+    def step():
+        if solver == "RK4":
+            eng.rk4_steps(0.0, np.zeros((1, 3, 3)))
+        else:
+            eng.ab_steps(0.0, 4, np.zeros((1, 3)))
+
+    eng.set_state(rho)
+    eng.check()
+    eng.monitor_limits(2.0 * etr, 2.0 * eherm)
+    step()  # within the limits
+    eng.check()
+    eng.monitor_limits(0.5 * etr, 0.0)  # the trace defect is twice this limit
+    with pytest.raises(edus.EdusError, match="trace monitor") as e:
+        step()
+    assert e.value.code == 5
+    eng.check()
+    eng.monitor_limits(0.0, 0.5 * eherm)
+    with pytest.raises(edus.EdusError, match="Hermiticity monitor"):
+        step()
+    eng.check()
+    eng.monitor_limits(0.0, 0.0)  # off
+    step()
+    eng.check()
+    bad = rho.copy()
+    bad[7, 1, 2] = np.nan
+    eng.set_state(bad)
+    eng.monitor_limits(1e300, 1e300)
+    with pytest.raises(edus.EdusError) as e:
+        step()
+    assert e.value.code == 5
+    with pytest.raises(edus.EdusError):
+        eng.monitor_limits(-1.0, 0.0)

@@ -185,3 +185,78 @@ def test_mixed_precision_propagation_error():
     assert drho <= 1e-5
     assert dpop <= 1e-5
     assert dvel <= 1e-3
+
+
+@pytest.mark.parametrize("grid", [(120, 120), (96, 96), (90, 90), (48, 48)])
+def test_fused_gradient_identical_to_two_pass(model_dir, grid):
+    """The one-pass spectral derivative (one work-group per plane) against the
+    two axis passes and the oracle: the same DFT arithmetic in another
+    kernel, so the two forms agree to rounding (the compiler contracts
+    multiply-adds per kernel); the field has components along both lattice
+    vectors so both terms act."""
+    p = _model(model_dir, "ipa2_square30", list(grid))
+    o = oracle_py.Oracle(p)
+    eng = _engine(o, FULL, p["filledbands"])
+    rho = _rho(o.Nk, o.nb, 11, hermitian=False)
+    E = [3e-3, -2e-3, 0.0]
+    f_ref = o.rhs_E(E, rho)
+    outs = []
+    for fused in (0, 1):
+        t = eng.get_tuning()
+        t.grad_fused = fused
+        eng.set_tuning(t)
+        outs.append(eng.rhs(E, rho))
+    err = np.abs(outs[1] - f_ref).max()
+    d = np.abs(outs[1] - outs[0]).max()
+    scale = max(1.0, np.abs(f_ref).max())
+    _say(grid, "fused gradient: max|df| vs oracle", err, "vs two passes", d, "scale", scale)
+    assert d <= 1e-14 * scale
+    assert err <= 1e-11 * scale
+
+
+def test_fused_gradient_with_field_tables(model_dir):
+    """Graph replay reads the field and the gradient coefficients from device
+    tables (the batch path): the one-pass gradient with those tables ends
+    where its direct launches end, bit for bit, and within rounding of the
+    two-pass gradient."""
+    p = _model(model_dir, "ipa2_square30", [96, 96])
+    o = oracle_py.Oracle(p)
+    runs = []
+    for fused, graph in ((1, 0), (1, 5), (0, 0)):
+        eng = _engine(o, FULL, p["filledbands"])
+        t = eng.get_tuning()
+        t.grad_fused = fused
+        eng.set_tuning(t)
+        eng.graph(graph)
+        rng = np.random.default_rng(8)
+        E = rng.uniform(-4e-3, 4e-3, size=(10, 3, 3)) * np.array([1.0, 1.0, 0.0])
+        eng.rk4_steps(0.05, E)
+        runs.append(eng.get_state())
+    assert np.abs(runs[0] - o.rho).max() > 0  # the field moved the state
+    assert np.array_equal(runs[0], runs[1])
+    _say("fused vs two-pass after 10 RK4 steps:", np.abs(runs[0] - runs[2]).max())
+    assert np.abs(runs[0] - runs[2]).max() <= 1e-14
+
+
+def test_set_tuning_after_graph_replay(model_dir):
+    """edus_set_tuning drops a captured graph: after graph replays, forcing
+    the other FP64 commutator engine takes effect in the next replays, which
+    then end where direct launches with that engine end."""
+    p = _model(model_dir, "ipa14_hex120", [10, 9])
+    o = oracle_py.Oracle(p)
+    rng = np.random.default_rng(12)
+    E = rng.uniform(-4e-3, 4e-3, size=(10, 3, 3)) * np.array([1.0, 1.0, 0.0])
+    states = []
+    for replay_after in (True, False):
+        eng = _engine(o, FULL, p["filledbands"])
+        eng.set_state(o.rho)
+        eng.graph(5)
+        eng.rk4_steps(0.05, E[:5])
+        t = eng.get_tuning()
+        t.comm_kind = 1 - t.comm_kind
+        t.comm_tile = t.comm_tile_ab = 0
+        eng.set_tuning(t)
+        eng.graph(5 if replay_after else 0)
+        eng.rk4_steps(0.05, E[5:])
+        states.append(eng.get_state())
+    assert np.array_equal(states[0], states[1])

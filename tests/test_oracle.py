@@ -390,3 +390,33 @@ def test_hartree_shift_closed_form():
     expect = -1j * comm
     assert np.abs((f - f0) - expect).max() < 1e-13, np.abs((f - f0) - expect).max()
     assert np.abs(expect).max() > 1e-3  # the term is not trivially zero here
+
+
+# ---------------------------------------------------------------- gauge rotation
+def test_rotation_library_reduction_and_closed_forms(model_dir):
+    """Operator::go_to_bloch / go_to_wannier (Operator.hpp:528-564) in the
+    oracle against numpy matmul (library reduction) on a general complex
+    operator, and against the closed forms the eigensystem fixes:
+    U^dagger H0 U = diag(eps) and U diag(n < filled) U^dagger = the initial
+    condition (Functional_InitialCondition.hpp:12-32); the two directions
+    are inverse to each other."""
+    from edus import models
+
+    p = models.workload("ipa14_hex120", model_dir, grid=[5, 4, 1])
+    o = oracle_py.Oracle(p)
+    nb, nk = o.nb, o.Nk
+    U = o.get("U")
+    rng = np.random.default_rng(2)
+    x = rng.standard_normal((nk, nb, nb)) + 1j * rng.standard_normal((nk, nb, nb))
+    Ud = U.conj().transpose(0, 2, 1)
+    assert np.abs(o.rotate(True, x) - Ud @ x @ U).max() < 1e-13
+    assert np.abs(o.rotate(False, x) - U @ x @ Ud).max() < 1e-13
+    assert np.abs(o.rotate(False, o.rotate(True, x)) - x).max() < 1e-13
+    hb = o.rotate(True, o.get("H0k"))
+    eps = o.eigenvalues()
+    off = hb - np.einsum("kn,nm->knm", eps, np.eye(nb))
+    assert np.abs(off).max() < 1e-13 * max(1.0, np.abs(eps).max())
+    d = np.zeros((nk, nb, nb), complex)
+    for n in range(p["filledbands"]):
+        d[:, n, n] = 1.0
+    assert np.abs(o.rotate(False, d) - o.rho).max() < 1e-14
