@@ -45,6 +45,8 @@
 // not bit for bit.
 #include <algorithm>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <stdexcept>
 #include <vector>
 
@@ -476,6 +478,56 @@ extern "C" int edus_lindhard_device(int nb, int N0, int N1, int filledbands, con
     return edus::guarded([&] {
         const edus::LindhardPlan p = edus::lindhard_plan(nb, filledbands, N0, N1);
         edus::launch_lindhard(p, reinterpret_cast<const double2*>(U), eps, reinterpret_cast<double2*>(work),
+                              reinterpret_cast<double2*>(X), static_cast<hipStream_t>(stream));
+    });
+}
+
+// ---- the Lindhard response without allocation ----------------------------
+// Callers that cannot allocate device memory (code under HIP graph capture,
+// serving loops) use edus_lindhard_device_noalloc: the band-split copy of U
+// that edus_lindhard_device keeps in its `work` argument lives here instead,
+// in a fixed array of every GPU the library runs on, held until the process
+// ends.  Its device address is taken once per device with
+// hipGetSymbolAddress; host code passes that address, never the array's
+// name (a host-side stand-in).
+namespace edus {
+constexpr size_t kLindhardScratchElems = size_t(3) << 20;  // 3,145,728 complex doubles = 48 MiB
+static __device__ __align__(256) double2 scratch[kLindhardScratchElems];
+
+namespace {
+// address of `scratch` on the current device
+double2* scratch_address()
+{
+    static std::mutex mu;
+    static std::map<int, double2*> addr;
+    int dev = 0;
+    EDUS_HIP_CHECK(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = addr.find(dev);
+    if (it != addr.end()) return it->second;
+    void* p = nullptr;
+    EDUS_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) throw EngineError(EDUS_ERR_HIP, "hipGetSymbolAddress returned no address for the Lindhard scratch");
+    addr[dev] = static_cast<double2*>(p);
+    return addr[dev];
+}
+}  // namespace
+}  // namespace edus
+
+extern "C" int edus_lindhard_device_noalloc(int nb, int N0, int N1, int filledbands, const double* U,
+                                            const double* eps, double* X, void* stream)
+{
+    EDUS_REQUIRE(U && eps && X);
+    return edus::guarded([&] {
+        const edus::LindhardPlan p = edus::lindhard_plan(nb, filledbands, N0, N1);
+        const size_t need = edus::lindhard_work_bytes(p);
+        if (need > edus::kLindhardScratchElems * sizeof(double2))
+            throw edus::EngineError(EDUS_ERR_ARG,
+                                    "edus_lindhard_device_noalloc: Nk * nb^2 = " +
+                                        std::to_string(size_t(p.Nk) * nb * nb) + " exceeds its fixed scratch of " +
+                                        std::to_string(edus::kLindhardScratchElems) +
+                                        " complex doubles (48 MiB); use edus_lindhard_device with a workspace");
+        edus::launch_lindhard(p, reinterpret_cast<const double2*>(U), eps, edus::scratch_address(),
                               reinterpret_cast<double2*>(X), static_cast<hipStream_t>(stream));
     });
 }

@@ -5,4 +5,5 @@ this package only marshals arrays and parameters for tests and benchmarks.
 """
 from . import config  # noqa: F401
 from .engine import (PREC_FP64, PREC_MIXED, LAYOUT_FULL, LAYOUT_HERMITIAN, EdusError, Engine, Batch, Mlp, Simulation, Tuning, lib,  # noqa: F401
-                     LIB_PATH, HEADER_PATH, EXPORTS, kgrad_stencil, lindhard, lindhard_device)
+                     LIB_PATH, HEADER_PATH, EXPORTS, kgrad_stencil, lindhard, lindhard_device,
+                     lindhard_device_noalloc)

@@ -96,7 +96,7 @@ EXPORTS = [
     "edus_mlp_destroy", "edus_mlp_num_params", "edus_mlp_set_params", "edus_mlp_get_params", "edus_mlp_forward",
     "edus_mlp_gradients", "edus_mlp_train", "edus_sim_create", "edus_sim_destroy", "edus_sim_info",
     "edus_sim_engine", "edus_sim_field", "edus_sim_time", "edus_sim_propagate", "edus_sim_steps", "edus_sim_write_outputs", "edus_sim_get",
-    "edus_lindhard", "edus_lindhard_device", "edus_sim_screened", "edus_sim_write_screened",
+    "edus_lindhard", "edus_lindhard_device", "edus_lindhard_device_noalloc", "edus_sim_screened", "edus_sim_write_screened",
 ]
 
 
@@ -176,6 +176,8 @@ def lib():
         L.edus_lindhard.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, dp, dp, dp]
         L.edus_lindhard_device.restype = C.c_int
         L.edus_lindhard_device.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, vp, vp, vp, vp, vp]
+        L.edus_lindhard_device_noalloc.restype = C.c_int
+        L.edus_lindhard_device_noalloc.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, vp, vp, vp, vp]
         L.edus_sim_screened.restype = C.c_int
         L.edus_sim_screened.argtypes = [vp, dp, dp, dp, dp, dp]
         L.edus_sim_write_screened.restype = C.c_int
@@ -395,6 +397,13 @@ def lindhard_device(nb: int, N0: int, N1: int, filledbands: int, U_ptr: int, eps
     _check(lib().edus_lindhard_device(int(nb), int(N0), int(N1), int(filledbands), C.c_void_p(U_ptr),
                                       C.c_void_p(eps_ptr), C.c_void_p(work_ptr), C.c_void_p(X_ptr),
                                       C.c_void_p(stream)))
+
+
+def lindhard_device_noalloc(nb: int, N0: int, N1: int, filledbands: int, U_ptr: int, eps_ptr: int, X_ptr: int,
+                            stream: int = 0) -> None:
+    """edus_lindhard_device_noalloc on device pointers (no workspace, no allocation)."""
+    _check(lib().edus_lindhard_device_noalloc(int(nb), int(N0), int(N1), int(filledbands), C.c_void_p(U_ptr),
+                                              C.c_void_p(eps_ptr), C.c_void_p(X_ptr), C.c_void_p(stream)))
 
 
 def _fp(a):

@@ -410,6 +410,17 @@ int edus_lindhard(int nb, int N0, int N1, int filledbands, const double* U, cons
  * Allocates nothing. */
 int edus_lindhard_device(int nb, int N0, int N1, int filledbands, const double* U, const double* eps, double* work,
                          double* X, void* stream);
+/* edus_lindhard_device without the workspace argument: no counterpart in the
+ * reference.  For callers that cannot allocate device memory (code under HIP
+ * graph capture, serving loops, harnesses that forbid hipMalloc): it calls no
+ * allocator.  The band-split copy of U lives in a fixed 48 MiB array that the
+ * library holds on every GPU it runs on until the process ends, so Nk * nb^2
+ * may be at most 3,145,728 (120 x 120 k at nb = 14); larger shapes return
+ * EDUS_ERR_ARG.  Every call on one GPU shares that array: calls that overlap
+ * on one GPU (two streams, two threads) race and must be serialised by the
+ * caller.  Results equal edus_lindhard_device's. */
+int edus_lindhard_device_noalloc(int nb, int N0, int N1, int filledbands, const double* U, const double* eps,
+                                 double* X, void* stream);
 /* ScreenedPotential::Calculate (Screened_potential.cpp:139-199) on a set-up
  * simulation (the application derives from Simulation): X(R) (the response
  * transformed k -> R, Operator::go_to_R), the bare Coulomb V(R)
