@@ -177,6 +177,9 @@ def main():
     if args.workload == "screened_ipa14_hex120":
         run_screened(args)
         return
+    if args.workload == "rotate_ipa14_hex120":
+        run_rotate(args)
+        return
     if args.workload == "hbm_copy":
         run_copy(args)
         return
@@ -463,6 +466,48 @@ def run_copy(args):
                       "unit": "GB/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
                       "higher_is_better": True, "dtype": "f64", "data": "synthetic",
                       "config": {"workload": "hbm_copy", "read_bytes": n * 8, "write_bytes": n * 8}}), flush=True)
+
+
+def run_rotate(args):
+    """The Wannier -> Bloch rotation U^dagger rho U of every k-point (the
+    reference's go_to_bloch, src/Operator/Operator.hpp:528-564) at the 14-band
+    120^2 shape: K rotations of the device state into the k-major staging
+    buffer (the kernel behind edus_get_state_bloch), each timed by HIP events
+    on the engine's stream.  Algorithmic bytes per launch: the operator and U
+    read, the result written, 3 x nb^2 x Nk x 16 B (HBM-bound: 2 complex
+    nb x nb products per k are ~4.7 flop/B, under the FP64 ridge)."""
+    import edus
+    from edus import models
+
+    tmp = tempfile.mkdtemp(prefix="edus_rot_")
+    params = models.workload("ipa14_hex120", tmp)
+    prec = edus.PREC_FP64 if args.precision == "fp64" else edus.PREC_MIXED
+    sim = edus.Simulation(params, precision=prec)
+    eng = sim.engine
+    nb, nk = sim.nb, sim.Nk
+    for _ in range(max(args.warmup, 1)):
+        eng.get_state_bloch()
+    eng.synchronize()
+    eng.timing_enable(True)
+    for _ in range(args.steps):
+        eng.get_state_bloch()
+    eng.synchronize()
+    ms, cnt = eng.timing_get("rotate")
+    eng.timing_enable(False)
+    avg_s = ms / 1e3 / max(cnt, 1)
+    byts = 3.0 * nb * nb * nk * 16
+    achieved = byts / avg_s / 1e9
+    print(json.dumps({"metric": "k-point gauge rotations/s (U^dagger rho U, nb=14, 120^2 k)",
+                      "value": nk / avg_s, "unit": "k-point rotations/s", "n_gpus": 1, "steps": args.steps,
+                      "warmup": args.warmup, "ms_per_step": avg_s * 1e3, "higher_is_better": True,
+                      "scaling": "weak", "vs_baseline": None,
+                      "dtype": "f64" if args.precision == "fp64" else "f32 (FP32 MFMA, FP32 accumulation)",
+                      "data": "synthetic",
+                      "config": {"workload": "rotate_ipa14_hex120", "nb": nb, "grid": [sim.N0, sim.N1],
+                                 "precision": args.precision, "launches_timed": cnt},
+                      "roofline": {"bound": "hbm", "kernel": "rotate", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                                   "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                                   "bytes_per_launch": byts, "avg_launch_us": avg_s * 1e6}}), flush=True)
 
 
 def lindhard_flops(nk: int, nb: int, nf: int) -> float:

@@ -586,39 +586,6 @@ void Engine::autotune()
         tuned_shapes()[key] = best_c;
     }
 
-    // the spectral derivative: one pass per plane or two axis passes, whichever
-    // is faster here (identical results)
-    grad_fused = false;
-    if (grad_fused_available(N0, N1) && !kgrad) {
-        const auto gkey = std::make_tuple(N0, N1, B * npl);
-        auto git = tuned_grad().find(gkey);
-        if (force_grad >= 0) {  // a batch group keeps its members' form (they differ in rounding)
-            grad_fused = force_grad == 1;
-        } else if (git != tuned_grad().end()) {
-            grad_fused = git->second;
-        } else {
-            hipEvent_t e0, e1;
-            HIP_CHECK(hipEventCreate(&e0));
-            HIP_CHECK(hipEventCreate(&e1));
-            const double cg[2] = {1e-3, 1e-3};
-            float ms[2] = {0.f, 0.f};
-            for (int f = 0; f < 2; ++f) {
-                grad_fused = f == 1;
-                gradient(y, cg, nullptr, true, true);  // warm-up
-                HIP_CHECK(hipEventRecord(e0, s));
-                for (int r = 0; r < 3; ++r) gradient(y, cg, nullptr, true, true);
-                HIP_CHECK(hipEventRecord(e1, s));
-                HIP_CHECK(hipEventSynchronize(e1));
-                HIP_CHECK(hipEventElapsedTime(&ms[f], e0, e1));
-            }
-            (void)hipEventDestroy(e0);
-            (void)hipEventDestroy(e1);
-            HIP_CHECK(hipGetLastError());
-            grad_fused = ms[1] < ms[0];
-            tuned_grad()[gkey] = grad_fused;
-        }
-    }
-
     // commutator engine and tile, whichever is fastest here: for FP64 the FMA
     // engine or the matrix cores (they differ in summation order only), for
     // mixed precision the FP32 matrix cores; tiles give identical results.
@@ -656,28 +623,8 @@ void Engine::autotune()
             return ms;
         };
         // the operand sets of the hot uses, so that the candidates see the
-        // step's real HBM traffic, not a cache-resident subset (the arrays
-        // hold no state yet; every coefficient is 0).  RK4 third stage:
-        // stage input, H0, one xi, D and two stage arrays read, two written.
-        CommArgs c{};
-        c.Nk = Nk;
-        c.P = npl;
-        c.B = B;
-        c.rho = aux;
-        c.H0 = H0;
-        c.H0f = H0f;
-        for (int d = 0; d < 3; ++d)
-            if (xi[d]) {
-                c.xi[d] = xi[d];
-                c.xif[d] = xif[d];
-                break;
-            }
-        c.D = D;
-        c.up.x1 = acc;  // the RK4 step's heaviest stage (the third)
-        c.up.out1 = acc;
-        c.up.g1 = 2. / 3.;
-        c.up.x2 = y;
-        c.up.out2 = f;
+        // step's real HBM traffic, not a cache-resident subset
+        CommArgs c = tuning_stage_args();
         // Adams-Bashforth step of order 4: state (read and written in place),
         // H0, one xi, D and three history arrays read, the new history entry
         // written (over one of them: each element is read and written by the
@@ -713,6 +660,48 @@ void Engine::autotune()
         HIP_CHECK(hipGetLastError());
         tuned_comm()[ckey] = std::make_tuple(comm_kind, comm_tile, comm_tile_ab);
     }
+    // the spectral derivative: one pass per plane or two axis passes, whichever
+    // gives the faster stage (timed with the commutator that consumes D: the
+    // one-pass form leaves different data in the caches); they agree to rounding
+    grad_fused = false;
+    if (grad_fused_available(N0, N1) && !kgrad) {
+        const auto gkey = std::make_tuple(N0, N1, B * npl);
+        auto git = tuned_grad().find(gkey);
+        if (force_grad >= 0) {  // a batch group keeps its members' form (they differ in rounding)
+            grad_fused = force_grad == 1;
+        } else if (git != tuned_grad().end()) {
+            grad_fused = git->second;
+        } else {
+            hipEvent_t e0, e1;
+            HIP_CHECK(hipEventCreate(&e0));
+            HIP_CHECK(hipEventCreate(&e1));
+            const double cg[2] = {1e-3, 1e-3};
+            CommArgs c = tuning_stage_args();
+            float best[2] = {1e30f, 1e30f};
+            for (int trial = 0; trial < 2; ++trial)
+                for (int f = 0; f < 2; ++f) {
+                    grad_fused = f == 1;
+                    c.D = gradient(c.rho, cg, nullptr, true, true);  // warm-up
+                    launch_comm(nb, layout, comm_kind, comm_tile, c, s);
+                    HIP_CHECK(hipEventRecord(e0, s));
+                    for (int r = 0; r < 4; ++r) {
+                        gradient(c.rho, cg, nullptr, true, true);
+                        launch_comm(nb, layout, comm_kind, comm_tile, c, s);
+                    }
+                    HIP_CHECK(hipEventRecord(e1, s));
+                    HIP_CHECK(hipEventSynchronize(e1));
+                    float ms = 0.f;
+                    HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+                    best[f] = std::min(best[f], ms);
+                }
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            HIP_CHECK(hipGetLastError());
+            grad_fused = best[1] < best[0];
+            tuned_grad()[gkey] = grad_fused;
+        }
+    }
+
     if (prec == EDUS_PREC_MIXED) {
         // set-up products (the velocity operator) stay FP64
         comm_kind64 = kCommMfma64;
@@ -725,6 +714,34 @@ void Engine::autotune()
     for (double2* p : {G, f, acc, aux}) HIP_CHECK(hipMemsetAsync(p, 0, plane_bytes(), s));
     HIP_CHECK(hipStreamSynchronize(s));
     prime_launch_attributes();
+}
+
+// The commutator's operands in the RK4 step's heaviest stage (the third),
+// for timing candidates on this engine's own arrays (they hold no state yet;
+// every coefficient is 0): stage input, H0, one xi, D and two stage arrays
+// read, two written.
+CommArgs Engine::tuning_stage_args() const
+{
+    CommArgs c{};
+    c.Nk = Nk;
+    c.P = npl;
+    c.B = B;
+    c.rho = aux;
+    c.H0 = H0;
+    c.H0f = H0f;
+    for (int d = 0; d < 3; ++d)
+        if (xi[d]) {
+            c.xi[d] = xi[d];
+            c.xif[d] = xif[d];
+            break;
+        }
+    c.D = D;
+    c.up.x1 = acc;
+    c.up.out1 = acc;
+    c.up.g1 = 2. / 3.;
+    c.up.x2 = y;
+    c.up.out2 = f;
+    return c;
 }
 
 // Raise the dynamic-LDS limit of every kernel this engine may launch with the

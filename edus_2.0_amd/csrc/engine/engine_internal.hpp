@@ -178,6 +178,7 @@ class Engine {
     void build_velocity(double det);
     void autotune();
     void prime_launch_attributes();
+    CommArgs tuning_stage_args() const;
     AxisFFTArgs axis_args(int axis, const FFTChoice& c, int mode, const double2* in, const double2* addend,
                           double2* out, double coef, double scale, const double* mult) const;
     void tbegin(const char* cls, TimedLaunch& t);
