@@ -27,7 +27,7 @@ HBM_PEAK_GBS = 8000.0
 
 
 def kernel_bytes(cls: str, npl: int, nk: int, nb: int, n_e: int, solver: str = "RK", order: int = 4,
-                 coulomb: bool = False, peierls: bool = False) -> float:
+                 coulomb: bool = False, peierls: bool = False, f32_fields: bool = False) -> float:
     """Algorithmic HBM bytes per launch: one array of npl planes is
     S = npl*nk*16 B.  grad_col reads rho, writes G (2S); grad_row reads rho
     and G, writes D (3S); grad_fused (both axes in one pass per plane) reads
@@ -42,12 +42,16 @@ def kernel_bytes(cls: str, npl: int, nk: int, nb: int, n_e: int, solver: str = "
     entry, (4 + n_E + p) S.  Peierls gauge: peierls_col reads H0 and the
     n_E xi and writes the half-phased
     planes ((2 + n_E) S), peierls_row reads those and writes S_A[H] (2S), and
-    comm reads S_A[H] instead of H0, xi and D (one array in place of 2 + n_E)."""
+    comm reads S_A[H] instead of H0, xi and D (one array in place of 2 + n_E).
+    f32_fields: the mixed-precision commutator reads H0 and the n_E xi from
+    FP32 copies, half an array each."""
     s = 16.0 * npl * nk
     sig = 1 if coulomb else 0
     comm = (5.5 + n_e + sig) if solver == "RK" else (4 + n_e + order + sig)
     if peierls:
         comm -= 1 + n_e
+    if f32_fields:
+        comm -= 0.5 * (1 + n_e)
     return {"grad_col": 2 * s, "grad_row": 3 * s, "grad_fused": 2 * s, "kgrad": 2 * s, "comm": comm * s,
             "hf_row": 2 * s,
             "hf_col": 2.5 * s, "peierls_col": (2 + n_e) * s,
@@ -245,7 +249,8 @@ def main():
         pol += np.abs(np.asarray(las["polarization"], dtype=float))
     n_e = int(np.count_nonzero(pol))
     solver, order, coul = params["solver"], int(params["order"]), bool(params["coulomb"])
-    kb = {c: kernel_bytes(c, npl, nk, nb, n_e, solver, order, coul, bool(args.peierls)) for c in classes}
+    f32 = args.precision == "mixed" and not coul and not args.peierls  # the engine's FP32 copies of H0 and xi
+    kb = {c: kernel_bytes(c, npl, nk, nb, n_e, solver, order, coul, bool(args.peierls), f32) for c in classes}
     dom = max(classes, key=lambda c: times[c][0])
     ms, cnt = times[dom]
     avg_s = ms / 1e3 / max(cnt, 1)
@@ -497,6 +502,7 @@ def run_rotate(args):
     avg_s = ms / 1e3 / max(cnt, 1)
     byts = 3.0 * nb * nb * nk * 16
     achieved = byts / avg_s / 1e9
+    traffic, traffic_src = measured_traffic("rotate", "rotate_ipa14_hex120", "full", args.precision)
     print(json.dumps({"metric": "k-point gauge rotations/s (U^dagger rho U, nb=14, 120^2 k)",
                       "value": nk / avg_s, "unit": "k-point rotations/s", "n_gpus": 1, "steps": args.steps,
                       "warmup": args.warmup, "ms_per_step": avg_s * 1e3, "higher_is_better": True,
@@ -506,8 +512,13 @@ def run_rotate(args):
                       "config": {"workload": "rotate_ipa14_hex120", "nb": nb, "grid": [sim.N0, sim.N1],
                                  "precision": args.precision, "launches_timed": cnt},
                       "roofline": {"bound": "hbm", "kernel": "rotate", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                                   "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                                   "bytes_per_launch": byts, "avg_launch_us": avg_s * 1e6}}), flush=True)
+                                   "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                                   "traffic_source": traffic_src, "bytes_per_launch": byts,
+                                   "avg_launch_us": avg_s * 1e6,
+                                   "timed_over": "HIP events around each launch; each call starts on an idle "
+                                                 "stream (it ends with the copy to the host), so the events "
+                                                 "include the launch gap the profiler's kernel time leaves out"}}),
+          flush=True)
 
 
 def lindhard_flops(nk: int, nb: int, nf: int) -> float:

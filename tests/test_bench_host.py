@@ -22,6 +22,8 @@ def test_comm_bytes_per_stage_model():
     assert bench.kernel_bytes("comm", npl, nk, 14, 1, peierls=True) == pytest.approx(4.5 * s)
     assert bench.kernel_bytes("grad_col", npl, nk, 14, 1) == pytest.approx(2 * s)
     assert bench.kernel_bytes("grad_row", npl, nk, 14, 1) == pytest.approx(3 * s)
+    # mixed precision reads H0 and the one xi from FP32 copies: 5.5 S
+    assert bench.kernel_bytes("comm", npl, nk, 14, 1, f32_fields=True) == pytest.approx(5.5 * s)
 
 
 def test_traffic_summary_matches_the_run_variant():
@@ -41,6 +43,13 @@ def test_traffic_summary_matches_the_run_variant():
     assert t == pytest.approx(bench.kernel_bytes("comm", 105, 14400, 14, 1), rel=0.05)
     t, src = bench.measured_traffic("comm", "ipa14_hex120", "full", "mixed")
     assert src.endswith("_mixed.txt")
+    assert t == pytest.approx(bench.kernel_bytes("comm", 196, 14400, 14, 1, f32_fields=True), rel=0.05)
+    t, src = bench.measured_traffic("grad_fused", "ipa14_hex120", "full", "fp64")
+    assert src.endswith("grad_fused_kernel_ipa14_hex120_rk4_full_fp64.txt")
+    for prec in ("fp64", "mixed"):
+        t, src = bench.measured_traffic("rotate", "rotate_ipa14_hex120", "full", prec)
+        assert src.endswith(f"rotate_kernel_rotate_ipa14_hex120_{prec}.txt")
+        assert t == pytest.approx(3 * 16.0 * 196 * 14400, rel=0.05)
     assert bench.measured_traffic("comm", "hsex8_hex90", "hermitian", "fp64") == (None, None)
     assert bench.measured_traffic("comm", "ipa14_hex120", "full", "fp64", gradient="k", gauge="peierls") == (None, None)
 
