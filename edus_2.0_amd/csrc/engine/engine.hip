@@ -1333,6 +1333,16 @@ int edus_get_state_bloch(edus_ctx* ctx, double* rho_bloch)
     return guarded([&] { ctx->e->download_bloch(ctx->e->y, rho_bloch); });
 }
 
+int edus_get_state_bloch_device(edus_ctx* ctx, double* rho_bloch_dev)
+{
+    EDUS_REQUIRE(ctx && rho_bloch_dev);
+    return guarded([&] {
+        Engine& e = *ctx->e;
+        if (e.B != 1) throw EngineError(EDUS_ERR_ARG, "edus_get_state_bloch_device applies to single contexts");
+        e.rotate(e.y, reinterpret_cast<double2*>(rho_bloch_dev), edus::kRotKMajor, true);
+    });
+}
+
 int edus_rhs(edus_ctx* ctx, const double E[3], const double* in, double* out)
 {
     EDUS_REQUIRE(ctx && E && in && out);

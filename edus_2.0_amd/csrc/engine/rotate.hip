@@ -13,8 +13,11 @@
 //   KIND kMfma64: v_mfma_f64_16x16x4 (FP64 throughout),
 //   KIND kMfma32: v_mfma_f32_16x16x4 on FP32 operands with FP32 accumulation
 //                 (the mixed-precision mode), results widened to FP64.
-// The intermediate product goes through LDS; the result is written back
-// through LDS so that the stores are coalesced, in one of three layouts:
+// The operator and U go straight from HBM into LDS (direct loads, no
+// registers held); the intermediate product and then the result take the
+// operator's place in LDS, so a work-group needs 57 KB at nb = 14 and two fit
+// on a CU.  The result is written back through LDS so that the stores are
+// coalesced, in one of three layouts:
 // FULL planes, HERMITIAN planes (upper triangle) or k-major [Nk][nb][nb]
 // (the reference's BlockMatrix order, for transfers to the host).
 #include <algorithm>
@@ -41,10 +44,18 @@ constexpr int rot_npl()
     return LAYOUT == kFull ? NB * NB : NB * (NB + 1) / 2;
 }
 
+// LDS slots of a tile array of n planes, in whole 64-slot chunks (one chunk
+// = one wave-wide direct load)
+constexpr int rot_slots(int n) { return (n * kRotPS + 63) / 64 * 64; }
+
+// the input operator's tile, later the intermediate product and the result
+// (max of the two sizes), then U's tile: 57 KB at nb = 14, two work-groups
+// per CU
 template <int NB, int LAYOUT>
 constexpr size_t rot_lds_bytes()
 {
-    return size_t(rot_npl<NB, LAYOUT>() + 2 * NB * NB) * kRotPS * sizeof(double2);
+    constexpr int a = rot_slots(rot_npl<NB, LAYOUT>()), m = rot_slots(NB * NB);
+    return size_t((a > m ? a : m) + m) * sizeof(double2);
 }
 
 // element (r, c) of the input tile (zero outside the nb x nb block)
@@ -98,32 +109,33 @@ __global__ void __launch_bounds__(kRotThreads) rotate_kernel(RotateArgs a)
 {
     constexpr int NPL = rot_npl<NB, LAYOUT>();
     constexpr int M = NB * NB;
+    constexpr int SA = rot_slots(NPL) > rot_slots(M) ? rot_slots(NPL) : rot_slots(M);
     using acc_t = typename std::conditional<KIND == kCommMfma64, dbl4, flt4>::type;
     extern __shared__ double2 lds[];
-    double2* sO = lds;                   // input operator, NPL planes
-    double2* sU = sO + NPL * kRotPS;     // U, M planes
-    double2* sT = sU + M * kRotPS;       // intermediate product, then the result
+    double2* sO = lds;       // input operator, NPL planes; then the intermediate product; then the result
+    double2* sU = lds + SA;  // U, M planes
     const int Nk = a.Nk;
     const int k0 = xcd_contiguous_block() * kRotTK;
-    const double2 zero = make_double2(0.0, 0.0);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
 
-    // (if/else rather than ?: on double2 keeps the values out of scratch)
-    for (int idx = threadIdx.x; idx < NPL * kRotTK; idx += kRotThreads) {
-        const int p = idx / kRotTK, kk = idx - p * kRotTK;
-        double2 v = zero;
-        if (k0 + kk < Nk) v = a.in[long(p) * Nk + k0 + kk];
-        sO[p * kRotPS + kk] = v;
+    // every load of the tile goes straight from HBM into LDS: lane l of a
+    // wave fills slot base + l (global_load_lds_dwordx4), slot q holds
+    // element (q / PS, q % PS); the padding column and k-points past Nk are
+    // not loaded (their results are not stored)
+    for (int c = wave; c < rot_slots(NPL) / 64; c += kRotThreads / 64) {
+        const int q = c * 64 + lane, p = q / kRotPS, kk = q - p * kRotPS;
+        if (kk < kRotTK && p < NPL && k0 + kk < Nk)
+            __builtin_amdgcn_global_load_lds(a.in + long(p) * Nk + k0 + kk, sO + c * 64, 16, 0, 0);
     }
-    for (int idx = threadIdx.x; idx < M * kRotTK; idx += kRotThreads) {
-        const int p = idx / kRotTK, kk = idx - p * kRotTK;
-        double2 v = zero;
-        if (k0 + kk < Nk) v = a.U[long(p) * Nk + k0 + kk];
-        sU[p * kRotPS + kk] = v;
+    for (int c = wave; c < rot_slots(M) / 64; c += kRotThreads / 64) {
+        const int q = c * 64 + lane, p = q / kRotPS, kk = q - p * kRotPS;
+        if (kk < kRotTK && p < M && k0 + kk < Nk)
+            __builtin_amdgcn_global_load_lds(a.U + long(p) * Nk + k0 + kk, sU + c * 64, 16, 0, 0);
     }
-    __syncthreads();
+    __builtin_amdgcn_s_waitcnt(0);  // this wave's direct loads are in LDS
+    __syncthreads();                // and every other wave's
 
-    const int kk = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int i = lane & 15, q = lane >> 4;
+    const int kk = wave, i = lane & 15, q = lane >> 4;
     const bool to_bloch = a.to_bloch != 0;
     acc_t cr{}, ci{};
     // first product T = Y O, Y = U^dagger (to Bloch) or U (to Wannier)
@@ -141,10 +153,11 @@ __global__ void __launch_bounds__(kRotThreads) rotate_kernel(RotateArgs a)
         const double2 B = in_elem<NB, LAYOUT>(sO, c, i, kk);
         cgemm_step<KIND>(A, B, cr, ci);
     }
+    __syncthreads();  // every wave has read O before T replaces it
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
         const int r = result_row<KIND>(q, v);
-        if (r < NB && i < NB) sT[(r * NB + i) * kRotPS + kk] = make_double2(double(cr[v]), double(ci[v]));
+        if (r < NB && i < NB) sO[(r * NB + i) * kRotPS + kk] = make_double2(double(cr[v]), double(ci[v]));
     }
     __syncthreads();
     // second product R = T X, X = U (to Bloch) or U^dagger (to Wannier)
@@ -154,7 +167,7 @@ __global__ void __launch_bounds__(kRotThreads) rotate_kernel(RotateArgs a)
     for (int s = 0; s < 4; ++s) {
         if (4 * s >= NB) break;
         const int c = 4 * s + q;
-        const double2 A = full_elem<NB>(sT, i, c, kk);
+        const double2 A = full_elem<NB>(sO, i, c, kk);
         double2 B;
         if (to_bloch) {
             B = full_elem<NB>(sU, c, i, kk);
@@ -168,9 +181,10 @@ __global__ void __launch_bounds__(kRotThreads) rotate_kernel(RotateArgs a)
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
         const int r = result_row<KIND>(q, v);
-        if (r < NB && i < NB) sT[(r * NB + i) * kRotPS + kk] = make_double2(double(cr[v]), double(ci[v]));
+        if (r < NB && i < NB) sO[(r * NB + i) * kRotPS + kk] = make_double2(double(cr[v]), double(ci[v]));
     }
     __syncthreads();
+    double2* sT = sO;
 
     if (a.out_layout == kRotKMajor) {
         // [Nk][nb][nb]: the tile's 8 k-points are one contiguous run

@@ -86,7 +86,7 @@ class _SimParams(C.Structure):
 
 EXPORTS = [
     "edus_create", "edus_clone", "edus_destroy", "edus_last_error", "edus_set_state", "edus_get_state", "edus_rhs",
-    "edus_rotate", "edus_get_state_bloch", "edus_monitor_limits",
+    "edus_rotate", "edus_get_state_bloch", "edus_get_state_bloch_device", "edus_monitor_limits",
     "edus_rk4_steps", "edus_ab_steps", "edus_ab_reset", "edus_ab_seed", "edus_rhs_peierls",
     "edus_rk4_steps_peierls", "edus_ab_steps_peierls", "edus_observables_peierls", "edus_observables", "edus_check", "edus_synchronize", "edus_graph",
     "edus_timing_enable", "edus_timing_get", "edus_get_tuning", "edus_set_tuning", "edus_kgrad_stencil",
@@ -118,6 +118,7 @@ def lib():
         L.edus_rhs.argtypes = [vp, dp, dp, dp]
         L.edus_rotate.argtypes = [vp, C.c_int, dp, dp]
         L.edus_get_state_bloch.argtypes = [vp, dp]
+        L.edus_get_state_bloch_device.argtypes = [vp, vp]
         L.edus_rk4_steps.argtypes = [vp, C.c_double, C.c_int, dp]
         L.edus_ab_steps.argtypes = [vp, C.c_double, C.c_int, C.c_int, dp]
         L.edus_ab_reset.argtypes = [vp]
@@ -286,6 +287,11 @@ class Engine:
         out = np.empty(self._shape(), dtype=np.complex128)
         _check(lib().edus_get_state_bloch(self.h, _dp(out.view(np.float64))))
         return out
+
+    def get_state_bloch_device(self, ptr: int) -> None:
+        """edus_get_state_bloch_device into the device buffer at ptr (Nk*nb*nb
+        complex doubles), queued on the context's stream."""
+        _check(lib().edus_get_state_bloch_device(self.h, C.c_void_p(ptr)))
 
     def rhs(self, E, rho) -> np.ndarray:
         a = _c128(rho).reshape(self._shape())

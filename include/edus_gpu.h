@@ -111,6 +111,10 @@ int edus_rotate(edus_ctx* ctx, int to_bloch, const double* in, double* out);
 /* The current state in the Bloch gauge, U^dagger rho U, [Nk][nb][nb]: the
  * reference's DMk_bloch output (Simulation.cpp:398-403 with Operator.hpp:528-564). */
 int edus_get_state_bloch(edus_ctx* ctx, double* rho_bloch);
+/* The same into a device buffer of Nk*nb*nb complex doubles ([Nk][nb][nb]),
+ * queued on the context's stream (edus_synchronize waits for it): for callers
+ * that keep the output on the GPU.  No counterpart in the reference. */
+int edus_get_state_bloch_device(edus_ctx* ctx, double* rho_bloch_dev);
 
 /* One right-hand-side evaluation f = SourceTerm(t, rho) with the field E
  * given (the reference evaluates SetOfLaser at t).  in/out host arrays. */

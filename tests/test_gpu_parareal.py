@@ -3,6 +3,7 @@
 - plain parareal (coarse RK4 alone) converges to sequential RK4 within the
   max|d rho| <= 1e-6 bar;
 - with max_iter = 0 the fallback is sequential RK4 itself, bit for bit;
+- a network whose output scale is zero gives the plain run bit for bit;
 - the per-k network, trained on the GPU on runs with random pulses, removes
   most of the coarse propagator's error on held-out data, and learned parareal
   converges to sequential RK4 within the bar; the iteration counts with and
@@ -59,6 +60,33 @@ def test_plain_parareal_converges_to_sequential(setup):
     assert res["converged"] == 1 and res["fallback"] == 0
     assert res["iterations"] <= N
     assert max(errs) <= 1e-6
+
+
+def test_zero_scaled_network_is_plain_parareal(setup):
+    """The network's correction enters as y * tsig + tmu: with both zero the
+    learned run is the plain one bit for bit (same updates, same boundaries),
+    whatever the untrained network computes -- the features, the network and
+    the correction's unpacking run, and add exact zeros."""
+    sim, (Ef, Ec, Es), bounds = setup
+    eng = sim.engine
+    q = eng.nb * eng.nb
+    din = 2 * q + 2 * Es.shape[1] + 4
+    net = edus.Mlp(din, 64, q, seed=2)
+    stats = {"mu": np.zeros(din, np.float32), "sig": np.ones(din, np.float32),
+             "tmu": np.zeros(q, np.float32), "tsig": np.zeros(q, np.float32)}
+    runs = []
+    for with_net in (False, True):
+        para = pr.Parareal(eng, N, FINE, COARSE, DT, tol=1e-9, max_iter=N)
+        para.set_fields(Ef, Ec, Es)
+        if with_net:
+            para.set_network(net, stats)
+        res = para.run()
+        runs.append((res, [para.state(n) for n in range(N + 1)]))
+        para.close()
+    (r0, s0), (r1, s1) = runs
+    assert r0["iterations"] == r1["iterations"] and r0["updates"] == r1["updates"]
+    for a, b in zip(s0, s1):
+        assert np.array_equal(a, b)
 
 
 def test_fallback_is_sequential_rk4(setup):

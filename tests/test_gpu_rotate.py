@@ -111,3 +111,21 @@ def test_state_in_bloch_gauge_and_initial_condition(tmp_path):
     derr = np.abs(np.einsum("knn->kn", rb) - np.einsum("knn->kn", o.rotate(True, o.rho))).max()
     _say("DMk_bloch max|d|", err, "diagonal vs the oracle's rotation", derr)
     assert err < 1e-10 and derr < 1e-10
+
+
+def test_state_bloch_into_device_buffer(tmp_path):
+    """edus_get_state_bloch_device writes the same [Nk][nb][nb] Bloch-gauge
+    state as edus_get_state_bloch, into a device buffer on the context's
+    stream."""
+    from edus import hip_rt
+
+    p = models.workload("ipa6_hex96", str(tmp_path), grid=[12, 10, 1])
+    sim = edus.Simulation(p)
+    eng = sim.engine
+    sim.steps(3)
+    ref = eng.get_state_bloch()
+    buf = hip_rt.DeviceBuffer(ref.nbytes)
+    eng.get_state_bloch_device(buf.address)
+    eng.synchronize()
+    got = buf.to_array(np.complex128, ref.shape)
+    assert np.array_equal(got, ref)
