@@ -170,3 +170,29 @@ def test_kgradient_batch_equals_single(model_dir, graph_steps):
     for i, m in enumerate(members):
         b.store(i)
         assert np.array_equal(m.get_state(), alone[i]), i
+
+
+@pytest.mark.parametrize("name,grid", [("ipa14_hex120", [10, 9]), ("hsex8_hex90", [9, 11])])
+def test_rhs_kgradient_mixed_precision(model_dir, name, grid):
+    """Mixed precision with the finite-difference k gradient: H0 and xi from
+    the FP32 copies (IPA) or FP64 (exchange), the commutator on the FP32
+    matrix cores, the stencil FP64; relative RHS error against the FP64
+    oracle <= 2e-6."""
+    p = _model(name, model_dir, grid)
+    o = oracle_py.Oracle(p)
+    args = dict(H0k=o.get("H0k"), xik=(o.get("rk0"), o.get("rk1"), o.get("rk2")), A=o.lattice(),
+                filledbands=p["filledbands"], U=o.get("U"), vel=(o.get("Vel0"), o.get("Vel1"), o.get("Vel2")),
+                layout=FULL, gradient_space="k", precision=edus.PREC_MIXED)
+    if p["coulomb"]:
+        args.update(coulomb=True, W=o.get("Wscr"), hartree=o.get("hartree"), rho0k=o.rho)
+    eng = edus.Engine(o.nb, o.N0, o.N1, **args)
+    assert eng.get_tuning().comm_kind == 2
+    rng = np.random.default_rng(6)
+    a = rng.standard_normal((o.Nk, o.nb, o.nb)) + 1j * rng.standard_normal((o.Nk, o.nb, o.nb))
+    rho = o.rho + 0.1 * 0.5 * (a + a.conj().transpose(0, 2, 1))
+    E = [5e-3, -2e-3, 0.0]
+    f_ref = o.rhs_E(E, rho)
+    f = eng.rhs(E, rho)
+    rel = np.abs(f - f_ref).max() / np.abs(f_ref).max()
+    _say(name, grid, "k-gradient mixed precision: max|df| / max|f| =", rel)
+    assert 1e-12 < rel <= 2e-6

@@ -141,3 +141,29 @@ def test_hsex_peierls_rk4_against_oracle(model_dir):
     o = oracle_py.Oracle(p)
     sim = edus.Simulation(p)
     _compare(o, sim, 15)
+
+
+@pytest.mark.parametrize("name,grid", [("ipa14_hex120", [8, 10]), ("hsex8_hex90", [9, 11])])
+def test_rhs_peierls_mixed_precision(model_dir, name, grid):
+    """Mixed precision in the Peierls gauge: the phased H(k) stays FP64 in HBM
+    (it is rebuilt every stage) and the commutator runs on the FP32 matrix
+    cores; relative RHS error against the FP64 oracle <= 2e-6."""
+    p = _model(name, model_dir, grid)
+    o = oracle_py.Oracle(p)
+    coul = bool(p["coulomb"])
+    args = dict(H0k=o.get("H0k"), xik=(o.get("rk0"), o.get("rk1"), o.get("rk2")), A=o.lattice(),
+                filledbands=p["filledbands"], U=o.get("U"), vel=(o.get("Vel0"), o.get("Vel1"), o.get("Vel2")),
+                layout=FULL, peierls=True, precision=edus.PREC_MIXED)
+    if coul:
+        args.update(coulomb=True, W=o.get("Wscr"), hartree=o.get("hartree"), rho0k=o.rho)
+    eng = edus.Engine(o.nb, o.N0, o.N1, **args)
+    assert eng.get_tuning().comm_kind == 2
+    rng = np.random.default_rng(5)
+    a = rng.standard_normal((o.Nk, o.nb, o.nb)) + 1j * rng.standard_normal((o.Nk, o.nb, o.nb))
+    rho = o.rho + 0.1 * 0.5 * (a + a.conj().transpose(0, 2, 1))
+    E, Av = [3e-3, -2e-3, 0.0], [0.21, -0.13, 0.0]
+    f_ref = o.rhs_EA(E, Av, rho)
+    f = eng.rhs_peierls(E, Av, rho)
+    rel = np.abs(f - f_ref).max() / np.abs(f_ref).max()
+    _say(name, grid, "Peierls mixed precision: max|df| / max|f| =", rel)
+    assert 1e-12 < rel <= 2e-6
