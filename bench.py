@@ -162,6 +162,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg of the default line")
     ap.add_argument("--no-parareal", action="store_true", help="skip the learned-parareal leg of the default line")
+    ap.add_argument("--no-batch", action="store_true", help="skip the heterogeneous-batch leg of the default line")
+    ap.add_argument("--batch-steps", type=int, default=40, help="timed steps of each mode of the batch leg")
     ap.add_argument("--parareal-train-runs", type=int, default=16,
                     help="random-pulse training runs of the parareal leg in the default line")
     ap.add_argument("--tuning", default=None,
@@ -285,6 +287,10 @@ def main():
     if args.precision == "fp64" and not args.peierls and not args.no_mixed:
         out["mixed"] = mixed_leg(params, layout, args)
         _progress("mixed precision", out["mixed"])
+    if args.workload == "ipa14_hex120" and not args.no_batch:
+        out["batch"] = batch_leg(args.batch_steps, 20, 20)
+        _progress("batch", {k: out["batch"][k] for k in ("value", "batch_over_sequential",
+                                                          "batch_over_best_concurrent")})
     if args.workload == "ipa14_hex120" and not args.no_parareal:
         out["parareal"] = parareal_leg(args.parareal_train_runs, args.k_samples, args.epochs, args.tol,
                                        args.coarse_steps)
@@ -345,6 +351,11 @@ def _progress(*a):
 
 
 def run_batch(args):
+    """The c3 workload alone, as its own line (see batch_leg)."""
+    print(json.dumps(batch_leg(args.steps, args.warmup, args.graph)), flush=True)
+
+
+def batch_leg(steps: int, warmup: int, graph: int) -> dict:
     """Heterogeneous batch: 64 pulse runs over nb in {2, 6, 14} and grids
     48^2, 96^2, 120^2 (one seeded model per class), peak fields log-spaced
     over 1e-3 .. 3e-2 a.u., CEP uniform in [0, 2 pi), photon energy uniform in
@@ -376,11 +387,11 @@ def run_batch(args):
         members.append(sims[(w, g)].engine.clone())
         pulses.append(Pulse(e0[i], rng.uniform(0.8, 2.0), 20.0, (1.0, 0.0, 0.0), rng.uniform(0, 2 * np.pi)))
         nks.append(g * g)
-    dt, K, W = 0.05, args.steps, args.warmup
+    dt, K, W = 0.05, steps, max(warmup, graph)
     t_start = 200.0  # inside the pulses
     E = np.stack([stage_table(pl, t_start, dt, 4 * W + 4 * K) for pl in pulses], axis=2)  # [s][j][i][3]
     for m in members:
-        m.graph(args.graph)
+        m.graph(graph)
     # sequential: one member at a time
     for i, m in enumerate(members):
         m.rk4_steps(dt, E[:W, :, i, :])
@@ -420,7 +431,7 @@ def run_batch(args):
     # as one batch, from where the members stand
     batch = edus.Batch(members)
     ng, _ = batch.groups()
-    batch.graph(args.graph)
+    batch.graph(graph)
     batch.rk4_steps(dt, E[3 * W + 3 * K:4 * W + 3 * K])
     batch.synchronize()
     t1 = time.perf_counter()
@@ -434,7 +445,7 @@ def run_batch(args):
         "unit": "k-point·RK4-steps/s", "n_gpus": 1, "steps": K, "warmup": W,
         "ms_per_step": t_batch / K * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64", "data": "synthetic",
-        "config": {"workload": "hetero_batch64", "members": n, "groups": ng, "graph_steps": args.graph,
+        "config": {"workload": "hetero_batch64", "members": n, "groups": ng, "graph_steps": graph,
                    "classes": [f"{w} {g}x{g}" for w, g in classes]},
         "sequential": {"value": work / t_seq, "ms_per_step": t_seq / K * 1e3},
         "concurrent_contexts_graph": {"value": work / t_conc, "ms_per_step": t_conc / K * 1e3},
@@ -443,7 +454,10 @@ def run_batch(args):
         "batch_over_best_concurrent": min(t_conc, t_conc_d) / t_batch,
         "monitors": {"trace_dev": tr, "herm_dev": herm},
     }
-    print(json.dumps(out), flush=True)
+    batch.close()
+    for m in members:
+        m.close()
+    return out
 
 
 
