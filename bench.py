@@ -364,7 +364,11 @@ def batch_leg(steps: int, warmup: int, graph: int) -> dict:
     as 64 independent contexts launched back to back (each on its own stream,
     so they overlap) with graph replay and with direct launches, and as one
     batch (stacked groups, one hipGraph per graph_steps steps); throughput =
-    sum over members of Nk * steps / s."""
+    sum over members of Nk * steps / s.  The direct-launch contexts and the
+    batch are timed in alternating rounds and their medians compared (each
+    advances its own copy of the states)."""
+    import statistics
+
     import numpy as np
 
     import edus
@@ -388,56 +392,65 @@ def batch_leg(steps: int, warmup: int, graph: int) -> dict:
         pulses.append(Pulse(e0[i], rng.uniform(0.8, 2.0), 20.0, (1.0, 0.0, 0.0), rng.uniform(0, 2 * np.pi)))
         nks.append(g * g)
     dt, K, W = 0.05, steps, max(warmup, graph)
+    rounds = 3
     t_start = 200.0  # inside the pulses
-    E = np.stack([stage_table(pl, t_start, dt, 4 * W + 4 * K) for pl in pulses], axis=2)  # [s][j][i][3]
+    E = np.stack([stage_table(pl, t_start, dt, 4 * W + (2 + 2 * rounds) * K) for pl in pulses],
+                 axis=2)  # [s][j][i][3]
+    off = 0
+
+    def chunk(c):  # the next c steps of every member's field table
+        nonlocal off
+        e = E[off:off + c]
+        off += c
+        return e
+
+    def contexts(g, c):  # every member on its own stream, launched back to back (they overlap)
+        for m in members:
+            m.graph(g)
+        e = chunk(c)
+        t0 = time.perf_counter()
+        for i, m in enumerate(members):
+            m.rk4_steps(dt, e[:, :, i, :])
+        for m in members:
+            m.synchronize()
+        return time.perf_counter() - t0
+
+    # sequential: one member at a time, graph replay
     for m in members:
         m.graph(graph)
-    # sequential: one member at a time
+    e = chunk(W)
     for i, m in enumerate(members):
-        m.rk4_steps(dt, E[:W, :, i, :])
+        m.rk4_steps(dt, e[:, :, i, :])
         m.synchronize()
+    e = chunk(K)
     t0 = time.perf_counter()
     for i, m in enumerate(members):
-        m.rk4_steps(dt, E[W:W + K, :, i, :])
+        m.rk4_steps(dt, e[:, :, i, :])
         m.synchronize()
     t_seq = time.perf_counter() - t0
     _progress("sequential", t_seq, "s")
-    # independent contexts launched back to back (they overlap on their streams)
-    for i, m in enumerate(members):
-        m.rk4_steps(dt, E[W + K:2 * W + K, :, i, :])
-    for m in members:
-        m.synchronize()
-    t0 = time.perf_counter()
-    for i, m in enumerate(members):
-        m.rk4_steps(dt, E[2 * W + K:2 * W + 2 * K, :, i, :])
-    for m in members:
-        m.synchronize()
-    t_conc = time.perf_counter() - t0
+    contexts(graph, W)
+    t_conc = contexts(graph, K)
     _progress("concurrent contexts, graph replay", t_conc, "s")
-    # the same with direct launches
-    for m in members:
-        m.graph(0)
-    for i, m in enumerate(members):
-        m.rk4_steps(dt, E[2 * W + 2 * K:3 * W + 2 * K, :, i, :])
-    for m in members:
-        m.synchronize()
-    t0 = time.perf_counter()
-    for i, m in enumerate(members):
-        m.rk4_steps(dt, E[3 * W + 2 * K:3 * W + 3 * K, :, i, :])
-    for m in members:
-        m.synchronize()
-    t_conc_d = time.perf_counter() - t0
-    _progress("concurrent contexts, direct launches", t_conc_d, "s")
-    # as one batch, from where the members stand
+    # concurrent contexts with direct launches and the batch, alternated
+    # (rounds of each, medians reported: a single short run of either varies by ~10%)
+    contexts(0, W)
     batch = edus.Batch(members)
     ng, _ = batch.groups()
     batch.graph(graph)
-    batch.rk4_steps(dt, E[3 * W + 3 * K:4 * W + 3 * K])
+    batch.rk4_steps(dt, chunk(W))
     batch.synchronize()
-    t1 = time.perf_counter()
-    batch.rk4_steps(dt, E[4 * W + 3 * K:4 * W + 4 * K])
-    batch.synchronize()
-    t_batch = time.perf_counter() - t1
+    t_cd, t_b = [], []
+    for _ in range(rounds):  # the batch and the contexts advance their own copies of the states
+        t_cd.append(contexts(0, K))
+        e = chunk(K)
+        t1 = time.perf_counter()
+        batch.rk4_steps(dt, e)
+        batch.synchronize()
+        t_b.append(time.perf_counter() - t1)
+    t_conc_d = statistics.median(t_cd)
+    t_batch = statistics.median(t_b)
+    _progress("concurrent contexts, direct launches", t_cd, "s; batch", t_b, "s")
     tr, herm = batch.check()
     work = float(sum(nks)) * K
     out = {
@@ -447,6 +460,9 @@ def batch_leg(steps: int, warmup: int, graph: int) -> dict:
         "dtype": "f64", "data": "synthetic",
         "config": {"workload": "hetero_batch64", "members": n, "groups": ng, "graph_steps": graph,
                    "classes": [f"{w} {g}x{g}" for w, g in classes]},
+        "rounds": rounds,
+        "rounds_ms_per_step": {"batch": [t / K * 1e3 for t in t_b],
+                               "concurrent_contexts_direct": [t / K * 1e3 for t in t_cd]},
         "sequential": {"value": work / t_seq, "ms_per_step": t_seq / K * 1e3},
         "concurrent_contexts_graph": {"value": work / t_conc, "ms_per_step": t_conc / K * 1e3},
         "concurrent_contexts_direct": {"value": work / t_conc_d, "ms_per_step": t_conc_d / K * 1e3},
