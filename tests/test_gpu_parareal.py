@@ -6,8 +6,9 @@
 - a network whose output scale is zero gives the plain run bit for bit;
 - the per-k network, trained on the GPU on runs with random pulses, removes
   most of the coarse propagator's error on held-out data, and learned parareal
-  converges to sequential RK4 within the bar; the iteration counts with and
-  without the network (the ablation) are printed.
+  converges to sequential RK4 within the bar; against the ablation (coarse RK4
+  alone) its first sweep is closer to the fine solution and it needs no more
+  iterations (at this small shape; the 96^2 c4 measurement is in the bench).
 """
 import os
 import sys
@@ -131,3 +132,7 @@ def test_learned_parareal_and_ablation(setup):
     _say("ablation: iterations without network", r0["iterations"], "with network", r1["iterations"],
          "learned max|d rho|", err, "results", r0, r1)
     assert r1["converged"] == 1 and err <= 1e-6
+    # here the network's first sweep is closer to the fine solution than
+    # coarse RK4's, and the learned run needs no more iterations
+    assert r1["updates"][0] < r0["updates"][0]
+    assert r1["iterations"] <= r0["iterations"]
