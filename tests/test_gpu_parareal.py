@@ -61,6 +61,14 @@ def test_plain_parareal_converges_to_sequential(setup):
     assert res["converged"] == 1 and res["fallback"] == 0
     assert res["iterations"] <= N
     assert max(errs) <= 1e-6
+    # the same run from an explicit initial state (uploaded), and a second run
+    # of the same object, end bit for bit where the first did
+    states = [para.state(n) for n in range(N + 1)]
+    for rho0 in (bounds[0], None):
+        again = para.run(rho0=rho0)
+        assert again["iterations"] == res["iterations"] and again["updates"] == res["updates"]
+        for n in range(N + 1):
+            assert np.array_equal(para.state(n), states[n]), n
 
 
 def test_zero_scaled_network_is_plain_parareal(setup):
