@@ -27,7 +27,8 @@ HBM_PEAK_GBS = 8000.0
 
 
 def kernel_bytes(cls: str, npl: int, nk: int, nb: int, n_e: int, solver: str = "RK", order: int = 4,
-                 coulomb: bool = False, peierls: bool = False, f32_fields: bool = False) -> float:
+                 coulomb: bool = False, peierls: bool = False, f32_fields: bool = False,
+                 tri_fields: bool = False) -> float:
     """Algorithmic HBM bytes per launch: one array of npl planes is
     S = npl*nk*16 B.  grad_col reads rho, writes G (2S); grad_row reads rho
     and G, writes D (3S); grad_fused (both axes in one pass per plane) reads
@@ -44,14 +45,21 @@ def kernel_bytes(cls: str, npl: int, nk: int, nb: int, n_e: int, solver: str = "
     planes ((2 + n_E) S), peierls_row reads those and writes S_A[H] (2S), and
     comm reads S_A[H] instead of H0, xi and D (one array in place of 2 + n_E).
     f32_fields: the mixed-precision commutator reads H0 and the n_E xi from
-    FP32 copies, half an array each."""
+    FP32 copies, half an array each.  tri_fields (FULL layout): it reads
+    them from their upper triangles, nb (nb + 1) / 2 of the nb^2 planes
+    (0.536 of an array each at nb = 14)."""
     s = 16.0 * npl * nk
     sig = 1 if coulomb else 0
     comm = (5.5 + n_e + sig) if solver == "RK" else (4 + n_e + order + sig)
     if peierls:
         comm -= 1 + n_e
+    field = 1.0  # bytes of H0 and of each xi, in arrays
     if f32_fields:
-        comm -= 0.5 * (1 + n_e)
+        field *= 0.5
+    if tri_fields:
+        field *= (nb + 1) / (2.0 * nb)
+    if not peierls:
+        comm -= (1.0 - field) * (1 + n_e)
     return {"grad_col": 2 * s, "grad_row": 3 * s, "grad_fused": 2 * s, "kgrad": 2 * s, "comm": comm * s,
             "hf_row": 2 * s,
             "hf_col": 2.5 * s, "peierls_col": (2 + n_e) * s,
@@ -97,17 +105,18 @@ def cpu_baseline(params: dict, steps: int, integ: str, bytes_per_kstep: float = 
 
 
 def measured_traffic(cls: str, workload: str, layout: str, precision: str, gradient: str = "R",
-                     gauge: str = "length"):
+                     gauge: str = "length", fields: str = "full"):
     """HBM bytes per launch of a kernel class from the committed rocprofv3
     counter summaries under profiles/ (FETCH_SIZE and WRITE_SIZE runs scaled
     by the copy-kernel calibration); None when no summary covers the case.
-    A summary without a gradient= or gauge= key covers the defaults (the
-    spectral gradient, the length gauge) only."""
+    A summary without a gradient=, gauge= or fields= key covers the defaults
+    (the spectral gradient, the length gauge, H0 and xi read from every
+    element) only."""
     import glob
 
     want = {"class": cls, "workload": workload, "layout": layout, "precision": precision,
-            "gradient": gradient, "gauge": gauge}
-    default = {"gradient": "R", "gauge": "length"}
+            "gradient": gradient, "gauge": gauge, "fields": fields}
+    default = {"gradient": "R", "gauge": "length", "fields": "full"}
     for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.txt"))):
         with open(path) as f:
             for line in f:
@@ -252,14 +261,16 @@ def main():
     n_e = int(np.count_nonzero(pol))
     solver, order, coul = params["solver"], int(params["order"]), bool(params["coulomb"])
     f32 = args.precision == "mixed" and not coul and not args.peierls  # the engine's FP32 copies of H0 and xi
-    kb = {c: kernel_bytes(c, npl, nk, nb, n_e, solver, order, coul, bool(args.peierls), f32) for c in classes}
+    tri = bool(eng.get_tuning().tri_fields)  # H0 and xi read from their upper triangles
+    kb = {c: kernel_bytes(c, npl, nk, nb, n_e, solver, order, coul, bool(args.peierls), f32, tri)
+          for c in classes}
     dom = max(classes, key=lambda c: times[c][0])
     ms, cnt = times[dom]
     avg_s = ms / 1e3 / max(cnt, 1)
     byts = kb[dom]
     achieved = byts / avg_s / 1e9
     traffic, traffic_src = measured_traffic(dom, args.workload, args.layout, args.precision, args.gradient_space,
-                                            "peierls" if args.peierls else "length")
+                                            "peierls" if args.peierls else "length", "tri" if tri else "full")
     roofline = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
                 "bytes_per_launch": byts,

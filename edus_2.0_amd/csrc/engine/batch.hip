@@ -37,13 +37,13 @@ namespace {
 // Members that can share one stacked engine: same shape and arithmetic, so
 // that each ends exactly where it ends alone -- nb, grid, layout, precision,
 // exchange, filled bands, gradient space (and, for the k-space gradient, the
-// same stencil, which depends on the lattice), commutator engine and
-// spectral-gradient form.
+// same stencil, which depends on the lattice), commutator engine,
+// spectral-gradient form and the storage the commutator reads H0 and xi from.
 bool same_group(const Engine& a, const Engine& b)
 {
     if (a.nb != b.nb || a.N0 != b.N0 || a.N1 != b.N1 || a.layout != b.layout || a.prec != b.prec ||
         a.coulomb != b.coulomb || a.nf != b.nf || a.kgrad != b.kgrad || a.comm_kind != b.comm_kind ||
-        a.grad_fused != b.grad_fused)
+        a.grad_fused != b.grad_fused || a.tri != b.tri)
         return false;
     if (!a.kgrad) return true;
     if (a.kst.size() != b.kst.size()) return false;

@@ -21,7 +21,10 @@
 //             (v_mfma_f32_16x16x4): the mixed-precision mode.  D, the stage
 //             updates and the state stay FP64.
 // LAYOUT = kFull keeps every element; kHermitian keeps the upper triangle and
-// expands the lower one by conjugation while reading LDS.
+// expands the lower one by conjugation while reading LDS.  In kFull the
+// Hermitian fields H0 and xi may come as upper triangles too (CommArgs::tri):
+// the state keeps every element, and H0 and xi cost 0.54 of their FULL bytes
+// at nb = 14.
 #include <algorithm>
 #include <mutex>
 #include <stdexcept>
@@ -139,13 +142,40 @@ __device__ inline double2 lds_elem(const double2* s, int r, int c, int kk)
     return v;
 }
 
-// [H, rho] of k-point kk of the tile on the matrix cores.  Lane l holds, per
+// LDS slot of element (a, b) of the H tile and whether it must be conjugated:
+// as slot<NB, LAYOUT>, or with tri (FULL layout, upper-triangle fields) the
+// HERMITIAN slot; selects rather than branches, so that both forms share one
+// copy of the code that reads the tile
+template <int NB, int LAYOUT>
+__device__ inline int h_slot(int a, int b, bool tri, bool& conj)
+{
+    if (LAYOUT != kFull) return slot<NB, LAYOUT>(a, b, conj);
+    conj = tri && a > b;
+    const int lo = conj ? b : a, hi = conj ? a : b;
+    return tri ? tri_index(lo, hi, NB) : a * NB + b;
+}
+
+// element (r, c) of the H tile (zero outside the nb x nb block)
+template <int NB, int LAYOUT, int PS>
+__device__ inline double2 lds_h(const double2* s, int r, int c, int kk, bool tri)
+{
+    if (r >= NB || c >= NB) return make_double2(0.0, 0.0);
+    bool cj;
+    const int sl = h_slot<NB, LAYOUT>(r, c, tri, cj);
+    double2 v = s[sl * PS + kk];
+    if (cj) v.y = -v.y;
+    return v;
+}
+
+// [H, rho] of k-point kk of the tile on the matrix cores (the H tile as
+// h_slot lays it out).  Lane l holds, per
 // K step s, A-fragments (row i = l%16, column c = 4s + l/16) and B-fragments
 // (row c, column i); the result element (l/16 + 4v, l%16) is in slot v of the
 // f64 instruction, (4*(l/16) + v, l%16) in slot v of the f32 one.
 //   Re C = Hr.Rr - Hi.Ri - Rr.Hr + Ri.Hi,   Im C = Hr.Ri + Hi.Rr - Rr.Hi - Ri.Hr
 template <int NB, int LAYOUT, int PS>
-__device__ inline void mfma_commutator64(const double2* sH, const double2* sR, int kk, int lane, dbl4& cre, dbl4& cim)
+__device__ inline void mfma_commutator64(const double2* sH, const double2* sR, int kk, int lane, bool tri, dbl4& cre,
+                              dbl4& cim)
 {
     const int i = lane & 15, q = lane >> 4;
     cre = dbl4{0.0, 0.0, 0.0, 0.0};
@@ -154,8 +184,8 @@ __device__ inline void mfma_commutator64(const double2* sH, const double2* sR, i
     for (int s = 0; s < 4; ++s) {
         const int c = 4 * s + q;
         if (4 * s >= NB) break;
-        const double2 hA = lds_elem<NB, LAYOUT, PS>(sH, i, c, kk), rA = lds_elem<NB, LAYOUT, PS>(sR, i, c, kk);
-        const double2 hB = lds_elem<NB, LAYOUT, PS>(sH, c, i, kk), rB = lds_elem<NB, LAYOUT, PS>(sR, c, i, kk);
+        const double2 hA = lds_h<NB, LAYOUT, PS>(sH, i, c, kk, tri), rA = lds_elem<NB, LAYOUT, PS>(sR, i, c, kk);
+        const double2 hB = lds_h<NB, LAYOUT, PS>(sH, c, i, kk, tri), rB = lds_elem<NB, LAYOUT, PS>(sR, c, i, kk);
         cre = __builtin_amdgcn_mfma_f64_16x16x4f64(hA.x, rB.x, cre, 0, 0, 0);
         cre = __builtin_amdgcn_mfma_f64_16x16x4f64(-hA.y, rB.y, cre, 0, 0, 0);
         cre = __builtin_amdgcn_mfma_f64_16x16x4f64(-rA.x, hB.x, cre, 0, 0, 0);
@@ -168,7 +198,8 @@ __device__ inline void mfma_commutator64(const double2* sH, const double2* sR, i
 }
 
 template <int NB, int LAYOUT, int PS>
-__device__ inline void mfma_commutator32(const double2* sH, const double2* sR, int kk, int lane, flt4& cre, flt4& cim)
+__device__ inline void mfma_commutator32(const double2* sH, const double2* sR, int kk, int lane, bool tri, flt4& cre,
+                              flt4& cim)
 {
     const int i = lane & 15, q = lane >> 4;
     cre = flt4{0.f, 0.f, 0.f, 0.f};
@@ -177,8 +208,8 @@ __device__ inline void mfma_commutator32(const double2* sH, const double2* sR, i
     for (int s = 0; s < 4; ++s) {
         const int c = 4 * s + q;
         if (4 * s >= NB) break;
-        const double2 hA = lds_elem<NB, LAYOUT, PS>(sH, i, c, kk), rA = lds_elem<NB, LAYOUT, PS>(sR, i, c, kk);
-        const double2 hB = lds_elem<NB, LAYOUT, PS>(sH, c, i, kk), rB = lds_elem<NB, LAYOUT, PS>(sR, c, i, kk);
+        const double2 hA = lds_h<NB, LAYOUT, PS>(sH, i, c, kk, tri), rA = lds_elem<NB, LAYOUT, PS>(sR, i, c, kk);
+        const double2 hB = lds_h<NB, LAYOUT, PS>(sH, c, i, kk, tri), rB = lds_elem<NB, LAYOUT, PS>(sR, c, i, kk);
         const float hAx = float(hA.x), hAy = float(hA.y), rAx = float(rA.x), rAy = float(rA.y);
         const float hBx = float(hB.x), hBy = float(hB.y), rBx = float(rB.x), rBy = float(rB.y);
         cre = __builtin_amdgcn_mfma_f32_16x16x4f32(hAx, rBx, cre, 0, 0, 0);
@@ -208,9 +239,10 @@ __device__ inline void fold_g1(double g1, const double2* sR, const int (&slot_of
         }
 }
 
-template <int NB, int LAYOUT, int KIND, int TILE, bool AB>
+template <int NB, int LAYOUT, int KIND, int TILE, bool AB, bool TRI>
 __global__ void __launch_bounds__((comm_threads<KIND, TILE>())) comm_kernel(CommArgs a)
 {
+    static_assert(!TRI || (LAYOUT == kFull && KIND != kCommScalar), "upper-triangle fields: FULL, MFMA engines");
     constexpr int NT = comm_threads<KIND, TILE>();
     constexpr int NPL = npl_of<NB, LAYOUT>();
     constexpr int TK = tile_k<NB, LAYOUT, KIND, TILE>();
@@ -257,6 +289,16 @@ __global__ void __launch_bounds__((comm_threads<KIND, TILE>())) comm_kernel(Comm
     // with exchange: there H0 holds -Sigma[rho0], which cancels against
     // Sigma[rho] and must not be rounded first)
     const bool F32H = KIND == kCommMfma32 && a.n_addf > 0;
+    // H0 and xi as upper-triangle planes (FULL layout, Hermitian fields): the
+    // H tile then holds the upper triangle only, in HERMITIAN slot order, and
+    // the products read the lower elements as conjugates; H0 and every
+    // addend are loaded once per element pair
+    constexpr int NPLU = NB * (NB + 1) / 2;
+    // (a separate instantiation, TRI, so that the other kernels keep their
+    // registers; the MFMA engines only: the FMA engine reads FULL fields)
+    constexpr bool tri = TRI;
+    const int nh = (tri ? NPLU : NPL) * TK;  // elements of the H tile
+    const long dh = tri ? long(inst) * (NPLU - NPL) * Nk : 0;  // H0 / xi offset minus the state's
     const double2* x1p = ab ? u.y : u.x1;
     const bool ld_x1 = (ab || u.out1) && x1p != a.rho;
     const bool ld_x2 = !ab && u.out2 && u.x2 != a.rho;
@@ -270,12 +312,15 @@ __global__ void __launch_bounds__((comm_threads<KIND, TILE>())) comm_kernel(Comm
             if (kk < TK && p < NPL && k0 + kk < Nk) {
                 const long o = ioff + long(p) * Nk + (k0 + kk);
                 __builtin_amdgcn_global_load_lds(a.rho + o, sR + c * 64, 16, 0, 0);
-                if (!F32H) __builtin_amdgcn_global_load_lds(a.H0 + o, sH + c * 64, 16, 0, 0);
+                if (!F32H && p * TK < nh) __builtin_amdgcn_global_load_lds(a.H0 + o + dh, sH + c * 64, 16, 0, 0);
             }
         }
     }
+    // element idx = threadIdx.x + j NT of the tile: plane idx / TK, k-point
+    // idx % TK, the same slot in the rho and H tiles; in the H tile (and in
+    // H0 / xi, at off + dh) plane p is the upper plane p when tri
     long off[PER];
-    bool in_tile[PER], ok[PER];
+    bool in_tile[PER], ok[PER], h_in[PER], h_ok[PER];
     int slot_of[PER];
 #pragma unroll
     for (int j = 0; j < PER; ++j) {
@@ -284,6 +329,8 @@ __global__ void __launch_bounds__((comm_threads<KIND, TILE>())) comm_kernel(Comm
         const int kk = idx - p * TK;
         in_tile[j] = idx < NPL * TK;
         ok[j] = in_tile[j] && k0 + kk < Nk;
+        h_in[j] = idx < nh;
+        h_ok[j] = h_in[j] && k0 + kk < Nk;
         off[j] = ioff + long(p) * Nk + (k0 + kk);
         slot_of[j] = p * PS + kk;
     }
@@ -310,12 +357,12 @@ __global__ void __launch_bounds__((comm_threads<KIND, TILE>())) comm_kernel(Comm
 #pragma unroll
             for (int j = 0; j < PER; ++j) {
                 fa[j] = fb[j] = fz;
-                if (ok[j]) fa[j] = a.addf[0][off[j]];
+                if (h_ok[j]) fa[j] = a.addf[0][off[j] + dh];
             }
             if (a.n_addf > 1) {
 #pragma unroll
                 for (int j = 0; j < PER; ++j)
-                    if (ok[j]) fb[j] = a.addf[1][off[j]];
+                    if (h_ok[j]) fb[j] = a.addf[1][off[j] + dh];
             }
             if (a.hdiag && threadIdx.x < NB) sHd[threadIdx.x] = a.hdiag[inst * NB + threadIdx.x];
             __builtin_amdgcn_s_waitcnt(0);  // this wave's direct loads are in LDS
@@ -324,12 +371,14 @@ __global__ void __launch_bounds__((comm_threads<KIND, TILE>())) comm_kernel(Comm
             for (int j = 0; j < PER; ++j) {
                 if (!in_tile[j]) continue;
                 const int sl = slot_of[j];
-                double2 h = make_double2(double(fa[j].x), double(fa[j].y));
-                if (a.n_addf > 1) {
-                    h.x += cff[1] * double(fb[j].x);
-                    h.y += cff[1] * double(fb[j].y);
+                if (h_in[j]) {
+                    double2 h = make_double2(double(fa[j].x), double(fa[j].y));
+                    if (a.n_addf > 1) {
+                        h.x += cff[1] * double(fb[j].x);
+                        h.y += cff[1] * double(fb[j].y);
+                    }
+                    sH[sl] = h;
                 }
-                sH[sl] = h;
                 x1v[j] = zero;
                 ex[0][j] = zero;
                 if (!ld_x1) x1v[j] = sR[sl];
@@ -341,13 +390,13 @@ __global__ void __launch_bounds__((comm_threads<KIND, TILE>())) comm_kernel(Comm
                 for (int j = 0; j < PER; ++j) {
                     fa[j] = fb[j] = fz;
                     sg[j] = zero;
-                    if (ok[j] && a.n_addf > 2) fa[j] = a.addf[2][off[j]];
-                    if (ok[j] && a.n_addf > 3) fb[j] = a.addf[3][off[j]];
+                    if (h_ok[j] && a.n_addf > 2) fa[j] = a.addf[2][off[j] + dh];
+                    if (h_ok[j] && a.n_addf > 3) fb[j] = a.addf[3][off[j] + dh];
                     if (ok[j] && a.sigma) sg[j] = a.sigma[off[j]];
                 }
 #pragma unroll
                 for (int j = 0; j < PER; ++j) {
-                    if (!in_tile[j]) continue;
+                    if (!h_in[j]) continue;
                     const int sl = slot_of[j];
                     double2 h = sH[sl];
                     h.x += cff[2] * double(fa[j].x) + cff[3] * double(fb[j].x) + sg[j].x;
@@ -362,12 +411,12 @@ __global__ void __launch_bounds__((comm_threads<KIND, TILE>())) comm_kernel(Comm
             if (a.n_add > 0) {
     #pragma unroll
                 for (int j = 0; j < PER; ++j)
-                    if (ok[j]) xa[j] = a.add[0][off[j]];
+                    if (h_ok[j]) xa[j] = a.add[0][off[j] + dh];
             }
             if (a.n_add > 1) {
     #pragma unroll
                 for (int j = 0; j < PER; ++j)
-                    if (ok[j]) xb[j] = a.add[1][off[j]];
+                    if (h_ok[j]) xb[j] = a.add[1][off[j] + dh];
             }
             if (a.hdiag && threadIdx.x < NB) sHd[threadIdx.x] = a.hdiag[inst * NB + threadIdx.x];
             __builtin_amdgcn_s_waitcnt(0);  // this wave's direct loads are in LDS
@@ -378,16 +427,18 @@ __global__ void __launch_bounds__((comm_threads<KIND, TILE>())) comm_kernel(Comm
             for (int j = 0; j < PER; ++j) {
                 if (!in_tile[j]) continue;
                 const int sl = slot_of[j];
-                double2 h = sH[sl];
-                if (a.n_add > 0) {
-                    h.x += cf[0] * xa[j].x;
-                    h.y += cf[0] * xa[j].y;
+                if (h_in[j]) {
+                    double2 h = sH[sl];
+                    if (a.n_add > 0) {
+                        h.x += cf[0] * xa[j].x;
+                        h.y += cf[0] * xa[j].y;
+                    }
+                    if (a.n_add > 1) {
+                        h.x += cf[1] * xb[j].x;
+                        h.y += cf[1] * xb[j].y;
+                    }
+                    sH[sl] = h;
                 }
-                if (a.n_add > 1) {
-                    h.x += cf[1] * xb[j].x;
-                    h.y += cf[1] * xb[j].y;
-                }
-                sH[sl] = h;
                 // stage operands that are the stage input itself
                 // (if/else rather than ?: on double2 keeps the values out of scratch)
                 x1v[j] = zero;
@@ -399,16 +450,16 @@ __global__ void __launch_bounds__((comm_threads<KIND, TILE>())) comm_kernel(Comm
     #pragma unroll
                 for (int j = 0; j < PER; ++j) {
                     xa[j] = xb[j] = zero;
-                    if (ok[j]) xa[j] = a.add[2][off[j]];
+                    if (h_ok[j]) xa[j] = a.add[2][off[j] + dh];
                 }
                 if (a.n_add > 3) {
     #pragma unroll
                     for (int j = 0; j < PER; ++j)
-                        if (ok[j]) xb[j] = a.add[3][off[j]];
+                        if (h_ok[j]) xb[j] = a.add[3][off[j] + dh];
                 }
     #pragma unroll
                 for (int j = 0; j < PER; ++j) {
-                    if (!in_tile[j]) continue;
+                    if (!h_in[j]) continue;
                     const int sl = slot_of[j];
                     double2 h = sH[sl];
                     h.x += cf[2] * xa[j].x;
@@ -476,9 +527,9 @@ __global__ void __launch_bounds__((comm_threads<KIND, TILE>())) comm_kernel(Comm
         for (int w = 0; w < KPW; ++w) {
             const int kk = wave + (NT / 64) * w;
             if constexpr (KIND == kCommMfma64)
-                mfma_commutator64<NB, LAYOUT, PS>(sH, sR, kk, lane, cre[w], cim[w]);
+                mfma_commutator64<NB, LAYOUT, PS>(sH, sR, kk, lane, tri, cre[w], cim[w]);
             else
-                mfma_commutator32<NB, LAYOUT, PS>(sH, sR, kk, lane, cre[w], cim[w]);
+                mfma_commutator32<NB, LAYOUT, PS>(sH, sR, kk, lane, tri, cre[w], cim[w]);
         }
         if (!ab && u.g1 != 0.0) fold_g1(u.g1, sR, slot_of, ok, ex[0], x1v);  // rho before the products replace it
         __syncthreads();
@@ -625,12 +676,12 @@ __global__ void __launch_bounds__((comm_threads<KIND, TILE>())) comm_kernel(Comm
     }
 }
 
-template <int NB, int LAYOUT, int KIND, int TILE, bool AB>
+template <int NB, int LAYOUT, int KIND, int TILE, bool AB, bool TRI>
 void launch_one(const CommArgs& a, hipStream_t s)
 {
     constexpr int TK = tile_k<NB, LAYOUT, KIND, TILE>();
     constexpr size_t lds = comm_lds_bytes<NB, LAYOUT, KIND, TILE>();
-    auto kern = &comm_kernel<NB, LAYOUT, KIND, TILE, AB>;
+    auto kern = &comm_kernel<NB, LAYOUT, KIND, TILE, AB, TRI>;
     static std::once_flag raised;  // contexts may be created from several host threads
     std::call_once(raised, [&] {
         (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -641,13 +692,25 @@ void launch_one(const CommArgs& a, hipStream_t s)
                        s, a);
 }
 
+template <int NB, int LAYOUT, int KIND, int TILE, bool TRI>
+void launch_ab(const CommArgs& a, hipStream_t s)
+{
+    if (a.up.ab_order > 0)
+        launch_one<NB, LAYOUT, KIND, TILE, true, TRI>(a, s);
+    else
+        launch_one<NB, LAYOUT, KIND, TILE, false, TRI>(a, s);
+}
+
 template <int NB, int LAYOUT, int KIND, int TILE>
 void launch_kind(const CommArgs& a, hipStream_t s)
 {
-    if (a.up.ab_order > 0)
-        launch_one<NB, LAYOUT, KIND, TILE, true>(a, s);
-    else
-        launch_one<NB, LAYOUT, KIND, TILE, false>(a, s);
+    if constexpr (LAYOUT == kFull && KIND != kCommScalar) {
+        if (a.tri) {
+            launch_ab<NB, LAYOUT, KIND, TILE, true>(a, s);
+            return;
+        }
+    }
+    launch_ab<NB, LAYOUT, KIND, TILE, false>(a, s);
 }
 
 template <int NB, int LAYOUT>
@@ -694,6 +757,8 @@ void launch_comm(int nb, int layout, int kind, int tile, const CommArgs& args, h
         a.addf[r] = nullptr;
         a.addf_dd[r] = -1;
     }
+    if (args.tri && (layout != kFull || args.sigma || args.hdiag || kind == kCommScalar))
+        throw std::runtime_error("upper-triangle H0 and xi: FULL layout without exchange, MFMA engines only");
     if (kind == kCommMfma32 && args.H0f) {
         if (args.sigma) throw std::runtime_error("FP32 H0 with exchange: H0 holds -Sigma[rho0] and stays FP64");
         a.addf[a.n_addf++] = args.H0f;

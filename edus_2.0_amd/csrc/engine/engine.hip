@@ -19,6 +19,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <limits>
 #include <map>
 #include <mutex>
 #include <stdexcept>
@@ -42,6 +43,29 @@ thread_local bool g_attributes_only = false;
 void set_error(const std::string& s) { g_last_error = s; }
 
 namespace {
+
+// Whether a host field [Nk][nb][nb] is Hermitian at every k: the largest
+// |a_nm - conj(a_mn)| within 16 ulp of the largest |a| (the DFT of a
+// Hermitian H(R) leaves 1-2 ulp; r(k) is made exactly Hermitian).  Reading
+// the lower triangle as the conjugate of the upper one then changes H by at
+// most that defect.
+bool hermitian_field(const double* f, int nb, int Nk)
+{
+    double dev = 0.0, mx = 0.0;
+    for (long k = 0; k < Nk; ++k) {
+        const double* a = f + 2 * k * nb * nb;
+        for (int n = 0; n < nb; ++n)
+            for (int m = 0; m < nb; ++m) {
+                const double* x = a + 2 * (n * nb + m);
+                mx = std::max(mx, std::hypot(x[0], x[1]));
+                if (m > n) {
+                    const double* y = a + 2 * (m * nb + n);
+                    dev = std::max(dev, std::hypot(x[0] - y[0], x[1] + y[1]));
+                }
+            }
+    }
+    return dev <= 16.0 * std::numeric_limits<double>::epsilon() * mx;
+}
 
 FFTFactors factorize(int n)
 {
@@ -215,7 +239,12 @@ Engine::Engine(const edus_desc& d, const EngineOptions& opt)
         HIP_CHECK(hipStreamSynchronize(s));
     }
 
+    tri_ok = layout == kFull && !coulomb && !peierls && hermitian_field(d.H0k, nb, Nk);
+    for (int k = 0; k < 3; ++k)
+        if (d.xik[k]) tri_ok = tri_ok && hermitian_field(d.xik[k], nb, Nk);
+    tri = tri_ok;
     alloc_f32_fields();
+    make_tri_fields();  // H0 is final here: the exchange fold below excludes them
     autotune();
     if (peierls) {
         alloc(reinterpret_cast<void**>(&Hp), pb);
@@ -279,6 +308,55 @@ void Engine::make_f32_fields()
     launch_to_f32(H0, H0f, n, s);
     for (int d = 0; d < 3; ++d)
         if (xi[d]) launch_to_f32(xi[d], xif[d], n, s);
+    if (H0uf) {
+        const long nu = long(B) * num_planes(nb, kHermitian) * Nk;
+        launch_to_f32(H0u, H0uf, nu, s);
+        for (int d = 0; d < 3; ++d)
+            if (xiu[d]) launch_to_f32(xiu[d], xiuf[d], nu, s);
+    }
+}
+
+// Upper-triangle copies of H0 and xi (and their FP32 versions, filled by
+// make_f32_fields) from the FULL planes: row n's planes (n, n..nb-1) are
+// consecutive in both layouts.
+void Engine::make_tri_fields()
+{
+    if (!tri_ok) return;
+    const int nu = num_planes(nb, kHermitian);
+    const size_t ub = size_t(B) * nu * Nk * sizeof(double2);
+    auto upper = [&](const double2* full, double2* up) {
+        for (int i = 0; i < B; ++i)
+            for (int n = 0; n < nb; ++n)
+                HIP_CHECK(hipMemcpyAsync(up + (size_t(i) * nu + tri_index(n, n, nb)) * Nk,
+                                         full + (size_t(i) * npl + size_t(n) * (nb + 1)) * Nk,
+                                         size_t(nb - n) * Nk * sizeof(double2), hipMemcpyDeviceToDevice, s));
+    };
+    alloc(reinterpret_cast<void**>(&H0u), ub);
+    upper(H0, H0u);
+    for (int d = 0; d < 3; ++d)
+        if (xi[d]) {
+            alloc(reinterpret_cast<void**>(&xiu[d]), ub);
+            upper(xi[d], xiu[d]);
+        }
+    if (uses_f32_fields()) {
+        const size_t fb = size_t(B) * nu * Nk * sizeof(float2);
+        alloc(reinterpret_cast<void**>(&H0uf), fb);
+        for (int d = 0; d < 3; ++d)
+            if (xi[d]) alloc(reinterpret_cast<void**>(&xiuf[d]), fb);
+    }
+}
+
+void Engine::field_args(CommArgs& c, const bool use_xi[3], int kind) const
+{
+    const bool t = tri && kind != kCommScalar;  // the FMA engine reads every element
+    c.tri = t ? 1 : 0;
+    c.H0 = t ? H0u : H0;
+    c.H0f = t ? H0uf : H0f;
+    for (int d = 0; d < 3; ++d) {
+        const bool on = use_xi[d] && xi[d];
+        c.xi[d] = on ? (t ? xiu[d] : xi[d]) : nullptr;
+        c.xif[d] = on ? (t ? xiuf[d] : xif[d]) : nullptr;
+    }
 }
 
 // V_d = (-i[r_d, H0] + e_d . grad_k H0) / det, evaluated with the same
@@ -623,23 +701,25 @@ void Engine::autotune()
             return ms;
         };
         // the operand sets of the hot uses, so that the candidates see the
-        // step's real HBM traffic, not a cache-resident subset
-        CommArgs c = tuning_stage_args();
+        // step's real HBM traffic, not a cache-resident subset.
         // Adams-Bashforth step of order 4: state (read and written in place),
         // H0, one xi, D and three history arrays read, the new history entry
         // written (over one of them: each element is read and written by the
         // same thread)
-        CommArgs cab = c;
-        cab.up = StageUpdate{};
-        cab.up.ab_order = 4;
-        cab.up.y = aux;
-        cab.up.out_f = G;
-        cab.up.hist[1] = acc;
-        cab.up.hist[2] = G;
-        cab.up.hist[3] = f;
+        auto ab_args = [&](int kind) {
+            CommArgs cab = tuning_stage_args(kind);
+            cab.up = StageUpdate{};
+            cab.up.ab_order = 4;
+            cab.up.y = aux;
+            cab.up.out_f = G;
+            cab.up.hist[1] = acc;
+            cab.up.hist[2] = G;
+            cab.up.hist[3] = f;
+            return cab;
+        };
         float best = 1e30f;
         for (const auto& kt : cands) {
-            const float ms = time_of(kt.first, kt.second, c);
+            const float ms = time_of(kt.first, kt.second, tuning_stage_args(kt.first));
             if (ms < best) {
                 best = ms;
                 comm_kind = kt.first;
@@ -649,7 +729,7 @@ void Engine::autotune()
         best = 1e30f;
         for (const auto& kt : cands) {
             if (kt.first != comm_kind) continue;
-            const float ms = time_of(kt.first, kt.second, cab);
+            const float ms = time_of(kt.first, kt.second, ab_args(kt.first));
             if (ms < best) {
                 best = ms;
                 comm_tile_ab = kt.second;
@@ -676,7 +756,7 @@ void Engine::autotune()
             HIP_CHECK(hipEventCreate(&e0));
             HIP_CHECK(hipEventCreate(&e1));
             const double cg[2] = {1e-3, 1e-3};
-            CommArgs c = tuning_stage_args();
+            CommArgs c = tuning_stage_args(comm_kind);
             float best[2] = {1e30f, 1e30f};
             for (int trial = 0; trial < 2; ++trial)
                 for (int f = 0; f < 2; ++f) {
@@ -720,21 +800,20 @@ void Engine::autotune()
 // for timing candidates on this engine's own arrays (they hold no state yet;
 // every coefficient is 0): stage input, H0, one xi, D and two stage arrays
 // read, two written.
-CommArgs Engine::tuning_stage_args() const
+CommArgs Engine::tuning_stage_args(int kind) const
 {
     CommArgs c{};
     c.Nk = Nk;
     c.P = npl;
     c.B = B;
     c.rho = aux;
-    c.H0 = H0;
-    c.H0f = H0f;
+    bool use_xi[3] = {false, false, false};
     for (int d = 0; d < 3; ++d)
         if (xi[d]) {
-            c.xi[d] = xi[d];
-            c.xif[d] = xif[d];
+            use_xi[d] = true;
             break;
         }
+    field_args(c, use_xi, kind);
     c.D = D;
     c.up.x1 = acc;
     c.up.out1 = acc;
@@ -745,10 +824,11 @@ CommArgs Engine::tuning_stage_args() const
 }
 
 // Raise the dynamic-LDS limit of every kernel this engine may launch with the
-// choices just made (every FFT mode of each axis' tile, the commutator engines
-// and tiles of both integrators, the rotation, the one-pass gradient) while
-// no stream is being captured: a graph captured later then launches them
-// with their full LDS.  Nothing is launched.
+// choices just made (every FFT mode of each axis' tile, every tile of the
+// commutator engines for both integrators and both H0 / xi storages, the
+// rotation, the one-pass gradient) while no stream is being captured: a graph
+// captured later then launches them with their full LDS.  Nothing is
+// launched.  Also run after edus_set_tuning.
 void Engine::prime_launch_attributes()
 {
     AttributesOnly guard;
@@ -765,14 +845,17 @@ void Engine::prime_launch_attributes()
     c.B = B;
     c.H0 = H0;
     c.H0f = H0f ? H0f : reinterpret_cast<const float2*>(H0);  // not launched
-    for (int ab = 0; ab < 2; ++ab) {
-        c.up = StageUpdate{};
-        c.up.ab_order = ab ? 4 : 0;
-        for (int tile : {comm_tile, comm_tile_ab, comm_tile64, 0}) {
-            launch_comm(nb, layout, comm_kind, tile, c, s);
-            launch_comm(nb, layout, comm_kind64, tile, c, s);
+    for (int t = 0; t < (tri_ok ? 2 : 1); ++t)  // both H0 / xi storages (separate kernels)
+        for (int ab = 0; ab < 2; ++ab) {
+            c.up = StageUpdate{};
+            c.up.ab_order = ab ? 4 : 0;
+            for (int kind : {comm_kind, comm_kind64}) {
+                c.tri = t == 1 && kind != kCommScalar;
+                if (t == 1 && !c.tri) continue;
+                for (int tile : {0, 1, 2})
+                    if (tile == 0 || kind != kCommScalar) launch_comm(nb, layout, kind, tile, c, s);
+            }
         }
-    }
     if (U) {
         RotateArgs r{};
         r.Nk = Nk;
@@ -872,13 +955,9 @@ void Engine::evaluate(const double2* in, const double E[3], const StageUpdate& u
     c.P = npl;
     c.rho = in;
     c.D = Dp;
-    c.H0 = H0;
-    c.H0f = H0f;
-    for (int d = 0; d < 3; ++d) {
-        c.xi[d] = (xi[d] && E[d] != 0.0) ? xi[d] : nullptr;
-        c.xif[d] = (xif[d] && E[d] != 0.0) ? xif[d] : nullptr;
-        c.E[d] = E[d];
-    }
+    const bool use_xi[3] = {E[0] != 0.0, E[1] != 0.0, E[2] != 0.0};
+    field_args(c, use_xi, comm_kind);
+    for (int d = 0; d < 3; ++d) c.E[d] = E[d];
     c.sigma = coulomb ? T : nullptr;
     c.hdiag = coulomb ? hdiag : nullptr;
     c.up = up;
@@ -1033,7 +1112,13 @@ Engine::Engine(const std::vector<const Engine*>& members)
                                      hipMemcpyDeviceToDevice, s));
         }
     }
+    // upper-triangle fields when every member has them; the group reads them
+    // as its members do (same_group keeps members with different choices apart)
+    tri_ok = true;
+    for (const Engine* m : members) tri_ok = tri_ok && m->tri_ok;
+    tri = tri_ok && m0.tri;
     alloc_f32_fields();
+    make_tri_fields();
     make_f32_fields();
     HIP_CHECK(hipStreamSynchronize(s));
     // the commutator engines differ in summation order and the gradient forms
@@ -1070,12 +1155,7 @@ void Engine::evaluate_tab(const double2* in, int ts, int j, const bool use_axis[
     c.B = B;
     c.rho = in;
     c.D = Dp;
-    c.H0 = H0;
-    c.H0f = H0f;
-    for (int d = 0; d < 3; ++d) {
-        c.xi[d] = (xi[d] && use_xi[d]) ? xi[d] : nullptr;
-        c.xif[d] = (xif[d] && use_xi[d]) ? xif[d] : nullptr;
-    }
+    field_args(c, use_xi, comm_kind);
     c.E_tab = E_dev + row * B * 3;
     c.sigma = coulomb ? T : nullptr;
     c.hdiag = coulomb ? hdiag : nullptr;
@@ -1522,6 +1602,7 @@ int edus_get_tuning(edus_ctx* ctx, edus_tuning* out)
     out->comm_tile = e.comm_tile;
     out->comm_tile_ab = e.comm_tile_ab;
     out->grad_fused = e.grad_fused ? 1 : 0;
+    out->tri_fields = e.tri ? 1 : 0;
     return EDUS_OK;
 }
 
@@ -1560,16 +1641,21 @@ int edus_set_tuning(edus_ctx* ctx, const edus_tuning* in)
                 throw EngineError(EDUS_ERR_ARG, "comm_tile and comm_tile_ab must be 0, or 1 or 2 for the MFMA engines");
         if (in->grad_fused != 0 && (in->grad_fused != 1 || !edus::grad_fused_available(e.N0, e.N1) || e.kgrad))
             throw EngineError(EDUS_ERR_ARG, "grad_fused must be 0, or 1 on the grids with a one-pass gradient kernel");
+        if (in->tri_fields != 0 && (in->tri_fields != 1 || !e.tri_ok))
+            throw EngineError(EDUS_ERR_ARG, "tri_fields must be 0, or 1 where H0 and r are Hermitian (FULL layout, "
+                                            "no exchange, no Peierls gauge)");
         e.fft_choice[0] = next[0];
         e.fft_choice[1] = next[1];
         e.comm_kind = in->comm_kind;
         e.comm_tile = in->comm_tile;
         e.comm_tile_ab = in->comm_tile_ab;
         e.grad_fused = in->grad_fused != 0;
+        e.tri = in->tri_fields != 0;
         if (e.prec != EDUS_PREC_MIXED) {
             e.comm_kind64 = in->comm_kind;
             e.comm_tile64 = in->comm_tile;
         }
+        e.prime_launch_attributes();
         // a captured graph holds the old kernels: capture again on next use
         if (e.self_batch) e.self_batch->invalidate();
     });

@@ -90,6 +90,21 @@ class Engine {
     bool uses_f32_fields() const;
     void alloc_f32_fields();
     void make_f32_fields();
+    // FULL layout without exchange or the Peierls gauge, with H0(k) and every
+    // r_d(k) Hermitian to a few ulp: their upper triangles (nb (nb + 1) / 2
+    // planes per instance, FP64 and for mixed precision FP32), which the
+    // commutator reads instead when `tri` is set (CommArgs::tri).  The FULL
+    // arrays stay for the set-up products.
+    bool tri_ok = false;
+    bool tri = false;
+    double2 *H0u = nullptr, *xiu[3] = {nullptr, nullptr, nullptr};
+    float2 *H0uf = nullptr, *xiuf[3] = {nullptr, nullptr, nullptr};
+    void make_tri_fields();
+    // H0, xi (where use_xi[d]), their FP32 copies and the tri flag of the
+    // arguments of a commutator launch on engine `kind`
+    void field_args(CommArgs& c, const bool use_xi[3], int kind) const;
+    // the dynamic-LDS attribute of every kernel the current choices may launch
+    void prime_launch_attributes();
     double2 *U = nullptr, *T = nullptr, *stage = nullptr;
     double *W = nullptr, *hart = nullptr;
     double2 *hdiag = nullptr, *diag_partial = nullptr;
@@ -177,8 +192,7 @@ class Engine {
     void fock(const double2* in);
     void build_velocity(double det);
     void autotune();
-    void prime_launch_attributes();
-    CommArgs tuning_stage_args() const;
+    CommArgs tuning_stage_args(int kind) const;
     AxisFFTArgs axis_args(int axis, const FFTChoice& c, int mode, const double2* in, const double2* addend,
                           double2* out, double coef, double scale, const double* mult) const;
     void tbegin(const char* cls, TimedLaunch& t);

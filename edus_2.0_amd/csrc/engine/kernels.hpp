@@ -101,6 +101,11 @@ struct CommArgs {
     const float2* addf[4];
     int addf_dd[4];
     int n_addf;
+    // FULL layout: H0, xi (and H0f, xif) hold only the upper-triangle planes
+    // of Hermitian fields, nb (nb + 1) / 2 per instance in HERMITIAN order;
+    // element (n, m) with n > m is read as the conjugate of (m, n).  Not with
+    // Sigma (a FULL-layout exchange term is not Hermitian on even grids).
+    int tri;
 };
 
 // derivative coefficient of the line in plane `plane`

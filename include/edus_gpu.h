@@ -203,6 +203,14 @@ int edus_timing_get(edus_ctx* ctx, const char* kernel_class, double* total_ms, l
  *                  work-group holds a plane: grids 120 x 120, 96 x 96,
  *                  90 x 90, 48 x 48); 0: two axis passes through a work plane
  *                  set.  Used when the field has components along both axes.
+ *   tri_fields     1: the MFMA commutator engines read H0(k) and r_d(k) from
+ *                  copies of their upper triangles, the lower elements as
+ *                  conjugates (0.54 of the bytes at nb = 14); 0: from every
+ *                  element (as the FMA engine always does).
+ *                  Available (and the default) in the FULL layout without
+ *                  exchange or the Peierls gauge when H0(k) and every r_d(k)
+ *                  are Hermitian to 16 ulp of their largest element; for
+ *                  exactly Hermitian fields both give identical results.
  * Within one family, FFT tiles give bit-identical results, and so do the
  * commutator tiles; the two FP64 commutator engines differ in summation order
  * only, and the two gradient forms in rounding only (the compiler contracts
@@ -216,6 +224,7 @@ typedef struct {
     int comm_tile;
     int comm_tile_ab;
     int grad_fused;
+    int tri_fields;
 } edus_tuning;
 int edus_get_tuning(edus_ctx* ctx, edus_tuning* out);
 

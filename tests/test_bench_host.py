@@ -24,6 +24,13 @@ def test_comm_bytes_per_stage_model():
     assert bench.kernel_bytes("grad_row", npl, nk, 14, 1) == pytest.approx(3 * s)
     # mixed precision reads H0 and the one xi from FP32 copies: 5.5 S
     assert bench.kernel_bytes("comm", npl, nk, 14, 1, f32_fields=True) == pytest.approx(5.5 * s)
+    # upper triangles of H0 and xi: 105 of 196 planes each, 6.5 - 2 (91 / 196) = 5.571 S;
+    # in FP32 as well: 6.5 - 2 (1 - 105 / 392) = 5.036 S
+    assert bench.kernel_bytes("comm", npl, nk, 14, 1, tri_fields=True) == pytest.approx((6.5 - 2 * 91 / 196) * s)
+    assert bench.kernel_bytes("comm", npl, nk, 14, 1, f32_fields=True, tri_fields=True) == \
+        pytest.approx((6.5 - 2 * (1 - 105 / 392)) * s)
+    # the Peierls gauge reads S_A[H] in place of H0 and xi whatever their storage
+    assert bench.kernel_bytes("comm", npl, nk, 14, 1, peierls=True, tri_fields=True) == pytest.approx(4.5 * s)
 
 
 def test_traffic_summary_matches_the_run_variant():

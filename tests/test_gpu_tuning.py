@@ -74,6 +74,7 @@ def test_fft_tiles_give_identical_results(model_dir, name, grid, layout):
         t.comm_kind = base.comm_kind
         t.comm_tile = base.comm_tile
         t.comm_tile_ab = base.comm_tile_ab
+        t.tri_fields = base.tri_fields
         try:
             eng.set_tuning(t)
         except edus.EdusError:
