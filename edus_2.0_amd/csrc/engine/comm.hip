@@ -121,13 +121,23 @@ constexpr int tile_slots()
     return (npl_of<NB, LAYOUT>() * plane_stride<KIND, TK>() + 63) / 64 * 64;
 }
 
-template <int NB, int LAYOUT, int KIND, int TILE>
+// Slots of the H tile: as the rho tile, or the upper triangle's planes when
+// the fields come as upper triangles (TRI)
+template <int NB, int LAYOUT, int KIND, int TILE, bool TRI>
+constexpr int h_tile_slots()
+{
+    constexpr int TK = tile_k<NB, LAYOUT, KIND, TILE>();
+    constexpr int np = TRI ? NB * (NB + 1) / 2 : npl_of<NB, LAYOUT>();
+    return (np * plane_stride<KIND, TK>() + 63) / 64 * 64;
+}
+
+template <int NB, int LAYOUT, int KIND, int TILE, bool TRI>
 constexpr size_t comm_lds_bytes()
 {
     // rho and H tiles plus the Hartree diagonal; the monitor's block
-    // reduction reuses the tile: 2 doubles per thread
-    return std::max((size_t(2) * tile_slots<NB, LAYOUT, KIND, TILE>() + kMaxNb) * sizeof(double2),
-                    size_t(2) * comm_threads<KIND, TILE>() * sizeof(double));
+    // reduction reuses the H tile (2 doubles per wave)
+    return (size_t(tile_slots<NB, LAYOUT, KIND, TILE>()) + h_tile_slots<NB, LAYOUT, KIND, TILE, TRI>() + kMaxNb) *
+           sizeof(double2);
 }
 
 // element (r, c) of a tile in LDS (zero outside the nb x nb block)
@@ -252,7 +262,7 @@ __global__ void __launch_bounds__((comm_threads<KIND, TILE>())) comm_kernel(Comm
     extern __shared__ double2 lds[];
     double2* sR = lds;
     double2* sH = lds + NSLOT;
-    double2* sHd = lds + 2 * NSLOT;
+    double2* sHd = sH + h_tile_slots<NB, LAYOUT, KIND, TILE, TRI>();
     const int Nk = a.Nk;
     // each XCD takes a contiguous range of tiles, so that tiles sharing a
     // cache line (TK = 4) meet in the same L2; t covers the tiles of instance
@@ -472,7 +482,7 @@ __global__ void __launch_bounds__((comm_threads<KIND, TILE>())) comm_kernel(Comm
                 }
             }
         }
-        if (a.hdiag) {
+        if (!TRI && a.hdiag) {  // (exchange only, never with upper-triangle fields)
 #pragma unroll
             for (int j = 0; j < PER; ++j) {
                 if (!ok[j]) continue;
@@ -680,7 +690,7 @@ template <int NB, int LAYOUT, int KIND, int TILE, bool AB, bool TRI>
 void launch_one(const CommArgs& a, hipStream_t s)
 {
     constexpr int TK = tile_k<NB, LAYOUT, KIND, TILE>();
-    constexpr size_t lds = comm_lds_bytes<NB, LAYOUT, KIND, TILE>();
+    constexpr size_t lds = comm_lds_bytes<NB, LAYOUT, KIND, TILE, TRI>();
     auto kern = &comm_kernel<NB, LAYOUT, KIND, TILE, AB, TRI>;
     static std::once_flag raised;  // contexts may be created from several host threads
     std::call_once(raised, [&] {
