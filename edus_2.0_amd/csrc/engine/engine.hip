@@ -1423,6 +1423,35 @@ int edus_get_state_bloch_device(edus_ctx* ctx, double* rho_bloch_dev)
     });
 }
 
+int edus_fourier(edus_ctx* ctx, int sign, const double* in, double* out)
+{
+    EDUS_REQUIRE(ctx && in && out);
+    return guarded([&] {
+        Engine& e = *ctx->e;
+        if (sign != 1 && sign != -1) throw EngineError(EDUS_ERR_ARG, "edus_fourier: sign must be +1 or -1");
+        if (e.layout != EDUS_LAYOUT_FULL || e.B != 1)
+            throw EngineError(EDUS_ERR_UNSUPPORTED, "edus_fourier needs a single FULL-layout context");
+        if (e.N0 < 2 || e.N1 < 2) throw EngineError(EDUS_ERR_UNSUPPORTED, "edus_fourier needs a 2-D grid");
+        e.upload(in, e.aux);  // synchronous: in may equal out
+        if (sign < 0)
+            e.transform_to_R(e.aux, e.D);
+        else
+            e.transform_to_k(e.aux, e.D);
+        e.download(e.D, out);
+    });
+}
+
+int edus_ctx_info(edus_ctx* ctx, int* nb, int* N0, int* N1, int* layout)
+{
+    EDUS_REQUIRE(ctx);
+    const Engine& e = *ctx->e;
+    if (nb) *nb = e.nb;
+    if (N0) *N0 = e.N0;
+    if (N1) *N1 = e.N1;
+    if (layout) *layout = e.layout;
+    return EDUS_OK;
+}
+
 int edus_rhs(edus_ctx* ctx, const double E[3], const double* in, double* out)
 {
     EDUS_REQUIRE(ctx && E && in && out);

@@ -87,7 +87,8 @@ class _SimParams(C.Structure):
 
 EXPORTS = [
     "edus_create", "edus_clone", "edus_destroy", "edus_last_error", "edus_set_state", "edus_get_state", "edus_rhs",
-    "edus_rotate", "edus_get_state_bloch", "edus_get_state_bloch_device", "edus_monitor_limits",
+    "edus_rotate", "edus_get_state_bloch", "edus_get_state_bloch_device", "edus_monitor_limits", "edus_fourier",
+    "edus_ctx_info",
     "edus_rk4_steps", "edus_ab_steps", "edus_ab_reset", "edus_ab_seed", "edus_rhs_peierls",
     "edus_rk4_steps_peierls", "edus_ab_steps_peierls", "edus_observables_peierls", "edus_observables", "edus_check", "edus_synchronize", "edus_graph",
     "edus_timing_enable", "edus_timing_get", "edus_get_tuning", "edus_set_tuning", "edus_kgrad_stencil",
@@ -118,6 +119,9 @@ def lib():
             getattr(L, name).argtypes = [vp, dp]
         L.edus_rhs.argtypes = [vp, dp, dp, dp]
         L.edus_rotate.argtypes = [vp, C.c_int, dp, dp]
+        L.edus_fourier.argtypes = [vp, C.c_int, dp, dp]
+        ip = C.POINTER(C.c_int)
+        L.edus_ctx_info.argtypes = [vp, ip, ip, ip, ip]
         L.edus_get_state_bloch.argtypes = [vp, dp]
         L.edus_get_state_bloch_device.argtypes = [vp, vp]
         L.edus_rk4_steps.argtypes = [vp, C.c_double, C.c_int, dp]
@@ -283,6 +287,20 @@ class Engine:
         out = np.empty_like(a)
         _check(lib().edus_rotate(self.h, int(bool(to_bloch)), _dp(a.view(np.float64)), _dp(out.view(np.float64))))
         return out
+
+    def fourier(self, sign: int, x) -> np.ndarray:
+        """k -> R (sign -1, scaled by 1/Nk) or R -> k (sign +1) of an operator
+        field [Nk][nb][nb] on the context's grid (edus_fourier)."""
+        a = _c128(x).reshape(self._shape())
+        out = np.empty_like(a)
+        _check(lib().edus_fourier(self.h, int(sign), _dp(a.view(np.float64)), _dp(out.view(np.float64))))
+        return out
+
+    def info(self) -> tuple:
+        """(nb, N0, N1, layout) of the context (edus_ctx_info)."""
+        v = [C.c_int() for _ in range(4)]
+        _check(lib().edus_ctx_info(self.h, *[C.byref(x) for x in v]))
+        return tuple(x.value for x in v)
 
     def get_state_bloch(self) -> np.ndarray:
         out = np.empty(self._shape(), dtype=np.complex128)

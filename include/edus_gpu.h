@@ -104,7 +104,8 @@ int edus_get_state(edus_ctx* ctx, double* rho_k);
  * go_to_wannier, src/Operator/Operator.hpp:528-564): out = U^dagger in U
  * (to_bloch = 1) or U in U^dagger (to_bloch = 0) with the context's
  * eigenvectors U (columns, as the descriptor's U).  in/out host arrays
- * [Nk][nb][nb]; a HERMITIAN-layout context takes the Hermitian part of in.
+ * [Nk][nb][nb] (in may equal out); a HERMITIAN-layout context takes the
+ * Hermitian part of in.
  * FP64 contexts rotate on the FP64 matrix cores, mixed-precision ones on the
  * FP32 matrix cores (FP32 operands and accumulation). */
 int edus_rotate(edus_ctx* ctx, int to_bloch, const double* in, double* out);
@@ -115,6 +116,20 @@ int edus_get_state_bloch(edus_ctx* ctx, double* rho_bloch);
  * queued on the context's stream (edus_synchronize waits for it): for callers
  * that keep the output on the GPU.  No counterpart in the reference. */
 int edus_get_state_bloch_device(edus_ctx* ctx, double* rho_bloch_dev);
+
+/* Fourier transform of an operator field between the context's k grid and
+ * its R grid (Operator::go_to_R / go_to_k, src/Operator/Operator.hpp:566-606,
+ * through FourierTransform::fft, src/fftPair/fftPair.cpp:89-107):
+ *   sign -1:  out(R) = 1/Nk sum_k in(k) e^{-2 pi i (k0 R0 / N0 + k1 R1 / N1)}
+ *   sign +1:  out(k) = sum_R in(R) e^{+2 pi i (k0 R0 / N0 + k1 R1 / N1)}
+ * in/out host arrays [N0 N1][nb][nb] in FFT index order (index i of an axis
+ * of length N is R = i for 2 i < N, i - N otherwise; MeshGrid.cpp:458-486);
+ * in may equal out.  FULL-layout contexts with a 2-D grid; the state is left
+ * alone. */
+int edus_fourier(edus_ctx* ctx, int sign, const double* in, double* out);
+/* nb, grid and layout of a context (any pointer may be NULL).  No
+ * counterpart in the reference (its Operator carries its MeshGrid). */
+int edus_ctx_info(edus_ctx* ctx, int* nb, int* N0, int* N1, int* layout);
 
 /* One right-hand-side evaluation f = SourceTerm(t, rho) with the field E
  * given (the reference evaluates SetOfLaser at t).  in/out host arrays. */

@@ -33,3 +33,19 @@ def test_cpp_caller_reproduces_hbn_fixture(tmp_path):
         assert not bad.any(), name
     t = np.loadtxt(tmp_path / "Time.txt")
     assert np.allclose(t, np.loadtxt(os.path.join(GOLDEN, "hBN", "Time.txt")), rtol=5e-6, atol=1e-12)
+
+
+SEAMS = os.path.join(ROOT, "tests", "cpp", "build", "seams_hbn")
+
+
+def test_cpp_seam_classes(tmp_path):
+    """include/edus_operator.hpp's Operator<T> and FourierTransform from C++
+    on the hBN set-up (tests/cpp/seams_hbn.cpp): go_to_R against a direct DFT,
+    the round trips, go_to_bloch against edus_get_state_bloch, copies."""
+    assert os.path.exists(SEAMS), "tests/cpp/build/seams_hbn is not built (__graft_entry__.build())"
+    print("case: hBN 50x50, 30 RK4 steps, then the seam classes from C++", flush=True)
+    r = subprocess.run([SEAMS, os.path.join(GOLDEN, "models", "hBN_gap7.25eV_a2.5A")], capture_output=True,
+                       text=True, timeout=240)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0
+    assert "all checks passed" in r.stdout
