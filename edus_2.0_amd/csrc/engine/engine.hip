@@ -103,10 +103,13 @@ std::map<std::tuple<int, int, int, int>, FFTChoice>& tuned_shapes()
     return m;
 }
 
-// commutator engine and tile per (nb, layout, k-points of all instances, precision)
-std::map<std::tuple<int, int, int, int, int>, std::tuple<int, int, int>>& tuned_comm()
+// commutator engine and tile per (nb, layout, k-points of all instances,
+// precision, kept engine, exchange, upper-triangle fields): the last two
+// change the bytes per launch
+using CommKey = std::tuple<int, int, int, int, int, int, int>;
+std::map<CommKey, std::tuple<int, int, int>>& tuned_comm()
 {
-    static std::map<std::tuple<int, int, int, int, int>, std::tuple<int, int, int>> m;
+    static std::map<CommKey, std::tuple<int, int, int>> m;
     return m;
 }
 
@@ -669,7 +672,7 @@ void Engine::autotune()
     // mixed precision the FP32 matrix cores; tiles give identical results.
     // The Adams-Bashforth kernel (history prefetched, more registers) gets a
     // tile of its own.
-    const auto ckey = std::make_tuple(nb, layout, Nk * B, prec, force_kind);
+    const CommKey ckey = std::make_tuple(nb, layout, Nk * B, prec, force_kind, int(coulomb), int(tri));
     auto cit = tuned_comm().find(ckey);
     if (cit != tuned_comm().end()) {
         comm_kind = std::get<0>(cit->second);
