@@ -580,9 +580,11 @@ std::vector<FFTChoice> fft_candidates(int axis, int n, int other)
             const int maxl = two_step_max_lines(n, variant);
             std::vector<int> Ls{maxl};
             if (axis == 1) {
-                Ls.push_back(std::max(1, maxl / 2));
+                // fewer lines per work-group give small grids (90^2 x 64
+                // planes: 230 work-groups at the most) more work-groups
+                for (int div : {2, 4, 8}) Ls.push_back(std::max(1, maxl / div));
             } else {
-                for (int L : {8 * (maxl / 8), 32, 16, 8})
+                for (int L : {8 * (maxl / 8), 32, 16, 8, 4})
                     if (L >= 1 && L <= maxl) Ls.push_back(L);
                 for (int L = std::min(maxl, other); L >= 1; --L) {
                     if (other % L == 0) {

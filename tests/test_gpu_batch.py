@@ -149,8 +149,9 @@ def test_context_graph_replay_and_clone(model_dir, idx):
 
 def test_batch_groups_split_by_gradient_space_and_engine(model_dir):
     """Members of one shape that differ in gradient space (spectral R vs
-    finite-difference k) or in the forced commutator engine form separate
-    groups, and each still ends where it ends alone."""
+    finite-difference k), in the forced commutator engine or in the storage
+    the commutator reads H0 and xi from (upper triangles or every element)
+    form separate groups, and each still ends where it ends alone."""
     p, o = _setups(model_dir)[0]
     pk = dict(p, gradient_space="k")
     ok = oracle_py.Oracle(pk)
@@ -163,14 +164,20 @@ def test_batch_groups_split_by_gradient_space_and_engine(model_dir):
     t.comm_kind = 1 - t.comm_kind  # the other FP64 engine (FMA <-> MFMA)
     t.comm_tile = t.comm_tile_ab = 0
     m_e.set_tuning(t)
-    members = [m_r, m_k, m_e]
+    m_t = _engine(p, o)
+    t = m_t.get_tuning()
+    assert t.tri_fields == 1 and m_r.get_tuning().tri_fields == 1
+    t.tri_fields = 0
+    m_t.set_tuning(t)
+    members = [m_r, m_k, m_e, m_t]
     alone = [m.clone() for m in members]
-    n, dt = 3, 0.05
+    assert alone[3].get_tuning().tri_fields == 0  # a clone keeps the choice
+    n, dt = 4, 0.05
     E = _fields(6, n, dt, 9)
     b = edus.Batch(members)
     ng, mg = b.groups()
     _say("groups", ng, mg)
-    assert ng == 3 and len(set(mg)) == 3
+    assert ng == 4 and len(set(mg)) == 4
     b.rk4_steps(dt, E)
     for i in range(n):
         alone[i].rk4_steps(dt, E[:, :, i, :])
