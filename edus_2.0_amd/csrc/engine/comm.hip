@@ -642,21 +642,17 @@ __global__ void __launch_bounds__((comm_threads<KIND, TILE>())) comm_kernel(Comm
             const double t = sqrt(tr.x * tr.x + tr.y * tr.y);
             mt = mon_max(mt, t);
         }
-        // |rho_nm - conj(rho_mn)| over all pairs n < m, spread over the block
-        if (LAYOUT == kFull) {
-            constexpr int NPAIR = NB * (NB - 1) / 2;
-            for (int item = threadIdx.x; item < TK * NPAIR; item += NT) {
-                const int q = item / TK;
-                const int kk = item - q * TK;
-                const int k = k0 + kk;
-                if (k >= Nk) continue;
-                int n = 0, r = q;
-                while (r >= NB - 1 - n) {
-                    r -= NB - 1 - n;
-                    ++n;
-                }
-                const int m = n + 1 + r;
-                const double2 x = sR[(n * NB + m) * PS + kk];
+        // |rho_nm - conj(rho_mn)| over all pairs n < m: the owner of (n, m)
+        // holds it in a register and reads its mirror from the tile
+        if constexpr (LAYOUT == kFull) {
+#pragma unroll
+            for (int j = 0; j < PER; ++j) {
+                const int idx = threadIdx.x + j * NT;
+                const int p = idx / TK;
+                const int kk = idx - p * TK;
+                const int n = p / NB, m = p - (p / NB) * NB;
+                if (!ok[j] || n >= m) continue;
+                const double2 x = x1v[j];
                 const double2 z = sR[(m * NB + n) * PS + kk];
                 const double d = sqrt((x.x - z.x) * (x.x - z.x) + (x.y + z.y) * (x.y + z.y));
                 mh = mon_max(mh, d);
