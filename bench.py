@@ -28,7 +28,7 @@ HBM_PEAK_GBS = 8000.0
 
 def kernel_bytes(cls: str, npl: int, nk: int, nb: int, n_e: int, solver: str = "RK", order: int = 4,
                  coulomb: bool = False, peierls: bool = False, f32_fields: bool = False,
-                 tri_fields: bool = False) -> float:
+                 tri_fields: bool = False, hf_fused: bool = False) -> float:
     """Algorithmic HBM bytes per launch: one array of npl planes is
     S = npl*nk*16 B.  grad_col reads rho, writes G (2S); grad_row reads rho
     and G, writes D (3S); grad_fused (both axes in one pass per plane) reads
@@ -47,7 +47,9 @@ def kernel_bytes(cls: str, npl: int, nk: int, nb: int, n_e: int, solver: str = "
     f32_fields: the mixed-precision commutator reads H0 and the n_E xi from
     FP32 copies, half an array each.  tri_fields (FULL layout): it reads
     them from their upper triangles, nb (nb + 1) / 2 of the nb^2 planes
-    (0.536 of an array each at nb = 14)."""
+    (0.536 of an array each at nb = 14).  hf_fused (exchange): grad_row also
+    writes the exchange term's forward row transform (4S), and hf_row then
+    counts only the inverse row passes (2S each)."""
     s = 16.0 * npl * nk
     sig = 1 if coulomb else 0
     comm = (5.5 + n_e + sig) if solver == "RK" else (4 + n_e + order + sig)
@@ -60,7 +62,7 @@ def kernel_bytes(cls: str, npl: int, nk: int, nb: int, n_e: int, solver: str = "
         field *= (nb + 1) / (2.0 * nb)
     if not peierls:
         comm -= (1.0 - field) * (1 + n_e)
-    return {"grad_col": 2 * s, "grad_row": 3 * s, "grad_fused": 2 * s, "kgrad": 2 * s, "comm": comm * s,
+    return {"grad_col": 2 * s, "grad_row": (4 if (coulomb and hf_fused) else 3) * s, "grad_fused": 2 * s, "kgrad": 2 * s, "comm": comm * s,
             "hf_row": 2 * s,
             "hf_col": 2.5 * s, "peierls_col": (2 + n_e) * s,
             "peierls_row": 2 * s}.get(cls, 0.0)
@@ -261,8 +263,10 @@ def main():
     n_e = int(np.count_nonzero(pol))
     solver, order, coul = params["solver"], int(params["order"]), bool(params["coulomb"])
     f32 = args.precision == "mixed" and not coul and not args.peierls  # the engine's FP32 copies of H0 and xi
-    tri = bool(eng.get_tuning().tri_fields)  # H0 and xi read from their upper triangles
-    kb = {c: kernel_bytes(c, npl, nk, nb, n_e, solver, order, coul, bool(args.peierls), f32, tri)
+    tun = eng.get_tuning()
+    tri = bool(tun.tri_fields)  # H0 and xi read from their upper triangles
+    hf = coul and bool(tun.hf_fused) and tun.fft_family[1] == 1  # exchange row pass from the gradient's
+    kb = {c: kernel_bytes(c, npl, nk, nb, n_e, solver, order, coul, bool(args.peierls), f32, tri, hf)
           for c in classes}
     dom = max(classes, key=lambda c: times[c][0])
     ms, cnt = times[dom]

@@ -38,12 +38,14 @@ namespace {
 // that each ends exactly where it ends alone -- nb, grid, layout, precision,
 // exchange, filled bands, gradient space (and, for the k-space gradient, the
 // same stencil, which depends on the lattice), commutator engine,
-// spectral-gradient form and the storage the commutator reads H0 and xi from.
+// spectral-gradient form, the storage the commutator reads H0 and xi from and
+// whether the exchange term's forward row pass comes from the gradient's (the
+// last two forms agree only to rounding).
 bool same_group(const Engine& a, const Engine& b)
 {
     if (a.nb != b.nb || a.N0 != b.N0 || a.N1 != b.N1 || a.layout != b.layout || a.prec != b.prec ||
         a.coulomb != b.coulomb || a.nf != b.nf || a.kgrad != b.kgrad || a.comm_kind != b.comm_kind ||
-        a.grad_fused != b.grad_fused || a.tri != b.tri)
+        a.grad_fused != b.grad_fused || a.tri != b.tri || a.hf_fuse != b.hf_fuse)
         return false;
     if (!a.kgrad) return true;
     if (a.kst.size() != b.kst.size()) return false;

@@ -546,9 +546,12 @@ AxisFFTArgs Engine::axis_args(int axis, const FFTChoice& c, int mode, const doub
 }
 
 void Engine::axis_fft(int axis, int mode, const double2* in, const double2* addend, double2* out, double coef,
-                      double scale, const double* mult, const char* cls, const double* coef_tab)
+                      double scale, const double* mult, const char* cls, const double* coef_tab, double2* out2)
 {
+    if (mode == 5 && (!fft_choice[axis].two_step || !out2))
+        throw EngineError(EDUS_ERR_STATE, "mode 5 needs the two-step kernels and a second output");
     AxisFFTArgs a = axis_args(axis, fft_choice[axis], mode, in, addend, out, coef, scale, mult);
+    a.out2 = out2;
     if (coef_tab) {
         a.coef_tab = coef_tab;
         a.coef_stride = 2;
@@ -839,7 +842,8 @@ void Engine::prime_launch_attributes()
     AttributesOnly guard;
     for (int axis = 0; axis < 2; ++axis) {
         if ((axis == 0 ? N0 : N1) < 2) continue;
-        for (int mode = 0; mode <= 4; ++mode) {
+        for (int mode = 0; mode <= 5; ++mode) {
+            if (mode == 5 && (axis != 1 || !fft_choice[axis].two_step)) continue;
             const AxisFFTArgs a = axis_args(axis, fft_choice[axis], mode, y, nullptr, G, 0.0, 1.0, nullptr);
             launch_fft(a, fft_choice[axis], s);
         }
@@ -878,13 +882,21 @@ void Engine::prime_launch_attributes()
 // D = E . grad_k rho  (null when the field has no component along the grid):
 // c_j = a_j . E for lattice vector j, spectral derivative along grid axis j
 // (Functional_SourceTerm.hpp:20-24, kGradient_definitions.hpp:35-41).
-const double2* Engine::gradient(const double2* in, const double c[2], const double* ctab, bool use0, bool use1)
+const double2* Engine::gradient(const double2* in, const double c[2], const double* ctab, bool use0, bool use1,
+                                double2* fwd1, bool* fwd1_done)
 {
+    if (fwd1_done) *fwd1_done = false;
     use0 = use0 && N0 > 1;
     use1 = use1 && N1 > 1;
     if (!use0 && !use1) return nullptr;
     const double* t0 = ctab ? ctab + 0 : nullptr;
     const double* t1 = ctab ? ctab + 1 : nullptr;
+    // the row pass also stores the forward row transform of `in` (the
+    // exchange term's first pass) when asked to and the row kernels are the
+    // two-step ones
+    const bool fuse = fwd1 && use1 && fft_choice[1].two_step && !(use0 && grad_fused);
+    const int row_mode = fuse ? 5 : 0;
+    if (fuse && fwd1_done) *fwd1_done = true;
     if (use0 && use1 && grad_fused) {
         GradFusedArgs g{};
         g.in = in;
@@ -905,11 +917,11 @@ const double2* Engine::gradient(const double2* in, const double c[2], const doub
         tend(t);
     } else if (use0 && use1) {
         axis_fft(0, 0, in, nullptr, G, c[0], 1.0, nullptr, "grad_col", t0);
-        axis_fft(1, 0, in, G, D, c[1], 1.0, nullptr, "grad_row", t1);
+        axis_fft(1, row_mode, in, G, D, c[1], 1.0, nullptr, "grad_row", t1, fuse ? fwd1 : nullptr);
     } else if (use0) {
         axis_fft(0, 0, in, nullptr, D, c[0], 1.0, nullptr, "grad_col", t0);
     } else {
-        axis_fft(1, 0, in, nullptr, D, c[1], 1.0, nullptr, "grad_row", t1);
+        axis_fft(1, row_mode, in, nullptr, D, c[1], 1.0, nullptr, "grad_row", t1, fuse ? fwd1 : nullptr);
     }
     return D;
 }
@@ -924,9 +936,9 @@ const double2* Engine::kgradient(const double2* in, const double E[3], const dou
 }
 
 // T = F^-1[ -W o (F rho)/N ]  (screened exchange in k),  hdiag = Hartree shift
-void Engine::fock(const double2* in)
+void Engine::fock(const double2* in, bool row_done)
 {
-    axis_fft(1, 1, in, nullptr, T, 0.0, 1.0, nullptr, "hf_row");
+    if (!row_done) axis_fft(1, 1, in, nullptr, T, 0.0, 1.0, nullptr, "hf_row");
     axis_fft(0, 3, T, nullptr, T, 0.0, -1.0 / double(Nk), W, "hf_col");
     axis_fft(1, 2, T, nullptr, T, 0.0, 1.0, nullptr, "hf_row");
     launch_hartree(in, nb, Nk, layout, npl, B, hart, diag_partial, hdiag, s);
@@ -952,9 +964,11 @@ void Engine::evaluate(const double2* in, const double E[3], const StageUpdate& u
 {
     double cg[2];
     for (int j = 0; j < 2; ++j) cg[j] = (A[0][j] * E[0] + A[1][j] * E[1] + A[2][j] * E[2]) / (j == 0 ? N0 : N1);
+    bool hf_row = false;  // the exchange term's forward row pass done by the gradient
     const double2* Dp = kgrad ? ((cg[0] != 0.0 || cg[1] != 0.0) ? kgradient(in, E, nullptr) : nullptr)
-                              : gradient(in, cg, nullptr, cg[0] != 0.0, cg[1] != 0.0);
-    if (coulomb) fock(in);
+                              : gradient(in, cg, nullptr, cg[0] != 0.0, cg[1] != 0.0,
+                                         coulomb && hf_fuse ? T : nullptr, &hf_row);
+    if (coulomb) fock(in, hf_row);
     CommArgs c{};
     c.Nk = Nk;
     c.P = npl;
@@ -1122,6 +1136,7 @@ Engine::Engine(const std::vector<const Engine*>& members)
     tri_ok = true;
     for (const Engine* m : members) tri_ok = tri_ok && m->tri_ok;
     tri = tri_ok && m0.tri;
+    hf_fuse = m0.hf_fuse;  // the two exchange row-pass forms agree only to rounding
     alloc_f32_fields();
     make_tri_fields();
     make_f32_fields();
@@ -1151,9 +1166,11 @@ void Engine::evaluate_tab(const double2* in, int ts, int j, const bool use_axis[
 {
     const size_t row = size_t(ts) * 3 + j;
     const double c0[2] = {0.0, 0.0};
+    bool hf_row = false;  // the exchange term's forward row pass done by the gradient
     const double2* Dp = kgrad ? ((use_axis[0] || use_axis[1]) ? kgradient(in, nullptr, E_dev + row * B * 3) : nullptr)
-                              : gradient(in, c0, coef_dev + row * B * 2, use_axis[0], use_axis[1]);
-    if (coulomb) fock(in);
+                              : gradient(in, c0, coef_dev + row * B * 2, use_axis[0], use_axis[1],
+                                         coulomb && hf_fuse ? T : nullptr, &hf_row);
+    if (coulomb) fock(in, hf_row);
     CommArgs c{};
     c.Nk = Nk;
     c.P = npl;
@@ -1637,6 +1654,7 @@ int edus_get_tuning(edus_ctx* ctx, edus_tuning* out)
     out->comm_tile_ab = e.comm_tile_ab;
     out->grad_fused = e.grad_fused ? 1 : 0;
     out->tri_fields = e.tri ? 1 : 0;
+    out->hf_fused = e.hf_fuse ? 1 : 0;
     return EDUS_OK;
 }
 
@@ -1678,6 +1696,7 @@ int edus_set_tuning(edus_ctx* ctx, const edus_tuning* in)
         if (in->tri_fields != 0 && (in->tri_fields != 1 || !e.tri_ok))
             throw EngineError(EDUS_ERR_ARG, "tri_fields must be 0, or 1 where H0 and r are Hermitian (FULL layout, "
                                             "no exchange, no Peierls gauge)");
+        if (in->hf_fused != 0 && in->hf_fused != 1) throw EngineError(EDUS_ERR_ARG, "hf_fused must be 0 or 1");
         e.fft_choice[0] = next[0];
         e.fft_choice[1] = next[1];
         e.comm_kind = in->comm_kind;
@@ -1685,6 +1704,7 @@ int edus_set_tuning(edus_ctx* ctx, const edus_tuning* in)
         e.comm_tile_ab = in->comm_tile_ab;
         e.grad_fused = in->grad_fused != 0;
         e.tri = in->tri_fields != 0;
+        e.hf_fuse = in->hf_fused != 0;
         if (e.prec != EDUS_PREC_MIXED) {
             e.comm_kind64 = in->comm_kind;
             e.comm_tile64 = in->comm_tile;

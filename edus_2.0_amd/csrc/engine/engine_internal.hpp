@@ -122,6 +122,7 @@ class Engine {
     int comm_kind64 = kCommScalar;   // FP64 engine, also used for set-up products
     int comm_tile = 0, comm_tile64 = 0;  // tile of each
     int comm_tile_ab = 0;                 // tile of the Adams-Bashforth kernel
+    bool hf_fuse = true;  // exchange: its forward row pass taken from the gradient's row pass
     void read_monitor(double out[2]);
     // the monitors' maxima since the last read, left in place
     void peek_monitor(double out[2]);
@@ -185,11 +186,17 @@ class Engine {
     std::vector<void*> owned;
     void alloc(void** p, size_t bytes);
     void axis_fft(int axis, int mode, const double2* in, const double2* addend, double2* out, double coef,
-                  double scale, const double* mult, const char* cls, const double* coef_tab = nullptr);
+                  double scale, const double* mult, const char* cls, const double* coef_tab = nullptr,
+                  double2* out2 = nullptr);
     // D = E.grad rho with c[j] = c_j / N_j, or per instance from ctab [B][2]
-    const double2* gradient(const double2* in, const double c[2], const double* ctab, bool use0, bool use1);
+    // fwd1 (may be null): also store the forward row transform of in there
+    // from the row pass (mode 5) where it can; *fwd1_done says whether it did
+    const double2* gradient(const double2* in, const double c[2], const double* ctab, bool use0, bool use1,
+                            double2* fwd1 = nullptr, bool* fwd1_done = nullptr);
     void ensure_hist();
-    void fock(const double2* in);
+    // exchange term of in into T (+ Hartree diagonal); row_done: T already
+    // holds the forward row transform of in (from the gradient's row pass)
+    void fock(const double2* in, bool row_done = false);
     void build_velocity(double det);
     void autotune();
     CommArgs tuning_stage_args(int kind) const;

@@ -15,7 +15,9 @@
 // instead (the Hartree-Fock convolution along this axis), mode 4 by
 // scale exp(i coef R) (the Peierls phase along this axis, its input a sum
 // in + sum_d c_d in_add_d), mode 1 stops after step B (forward transform),
-// mode 2 starts at step B^-1 (inverse).
+// mode 2 starts at step B^-1 (inverse).  Mode 5 is mode 0 that also stores
+// the forward transform after step B into out2, exactly as mode 1 stores it
+// (the exchange term's first pass, taken from the same loads).
 //
 // Every thread owns at most one A task and one B task (L*max(RA,RB) <= NT,
 // checked on the host), so each thread's global loads are all issued before
@@ -133,7 +135,8 @@ __global__ void __launch_bounds__(NT) two_step_kernel(AxisFFTArgs a)
     const long gA = base + lineA * cstride;
     const long gB = base + lineB * cstride;
     constexpr int mode = MODE;
-    constexpr bool sandwich = mode == 0 || mode == 3 || mode == 4;
+    constexpr bool sandwich = mode == 0 || mode == 3 || mode == 4 || mode == 5;
+    constexpr bool deriv = mode == 0 || mode == 5;
 
     if constexpr (RB == 1) {
         // the whole line in one thread: no exchange
@@ -151,7 +154,13 @@ __global__ void __launch_bounds__(NT) two_step_kernel(AxisFFTArgs a)
             reg::rdft<RA, 1>(v);
         } else {
             reg::rdft<RA, -1>(v);
-            if constexpr (mode == 0) {
+            if constexpr (mode == 5) {
+                sfor<RA>([&](auto k_) {
+                    constexpr int k = decltype(k_)::value;
+                    a.out2[gA + long(k) * rowstride] = v[k];
+                });
+            }
+            if constexpr (deriv) {
                 const double coef = line_coef(a, a.axis == 1 ? (row0 + lineA) / a.N0 : plane0);
                 sfor<RA>([&](auto k_) {
                     constexpr int k = decltype(k_)::value;
@@ -262,7 +271,13 @@ __global__ void __launch_bounds__(NT) two_step_kernel(AxisFFTArgs a)
                         a.out[gB + long(ka) * rowstride + long(k * RA) * rowstride] = x;
                     });
                 } else {
-                    if constexpr (mode == 0) {
+                    if constexpr (mode == 5) {
+                        sfor<RB>([&](auto k_) {
+                            constexpr int k = decltype(k_)::value;
+                            a.out2[gB + long(ka) * rowstride + long(k * RA) * rowstride] = u[k];
+                        });
+                    }
+                    if constexpr (deriv) {
                         const double coef = line_coef(a, a.axis == 1 ? (row0 + lineB) / a.N0 : plane0);
                         sfor<RB>([&](auto k_) {
                             constexpr int k = decltype(k_)::value;
@@ -340,6 +355,12 @@ void launch_nt(const AxisFFTArgs& a, int variant, int blocks, size_t lds, hipStr
                 launch_mode<RA, RB, NT, 0, true>(a, blocks, lds, s);
             else
                 launch_mode<RA, RB, NT, 0, false>(a, blocks, lds, s);
+            break;
+        case 5:
+            if (variant & 1)
+                launch_mode<RA, RB, NT, 5, true>(a, blocks, lds, s);
+            else
+                launch_mode<RA, RB, NT, 5, false>(a, blocks, lds, s);
             break;
         case 1: launch_mode<RA, RB, NT, 1, false>(a, blocks, lds, s); break;
         case 2: launch_mode<RA, RB, NT, 2, false>(a, blocks, lds, s); break;

@@ -32,7 +32,8 @@ struct AxisFFTArgs {
     int axis;               // 0: columns (length N0), 1: rows (length N1)
     int n;                  // line length
     int L;                  // lines per work-group
-    int mode;               // 0 derivative, 1 forward, 2 inverse, 3 convolution, 4 phase
+    int mode;               // 0 derivative, 1 forward, 2 inverse, 3 convolution, 4 phase,
+                            // 5 derivative that also stores the forward transform in out2
     double coef;            // derivative: c_axis / n
     // derivative of stacked instances: plane p uses coef_tab[(p / ppi) * coef_stride]
     // (instance p / ppi, ppi = planes per instance) instead of coef
@@ -48,6 +49,9 @@ struct AxisFFTArgs {
     // mode 4 multiplier per frequency index j < n: scale * exp(i coef R(j)),
     // R(j) Gamma-centred (launch_phase_table); null: formed per element
     const double2* ptab;
+    // mode 5: the unscaled forward transform of in, as mode 1 with scale 1
+    // and no addend would store it (two-step kernels only)
+    double2* out2;
 };
 
 // Per-stage update rule of the fused commutator kernel.  With f the

@@ -43,13 +43,14 @@ class Tuning(C.Structure):
     """edus_tuning: kernel and tile choices of a context."""
     _fields_ = [("fft_family", C.c_int * 2), ("fft_cfg", C.c_int * 2), ("fft_lines", C.c_int * 2),
                 ("comm_kind", C.c_int), ("comm_tile", C.c_int), ("comm_tile_ab", C.c_int), ("grad_fused", C.c_int),
-                ("tri_fields", C.c_int)]
+                ("tri_fields", C.c_int), ("hf_fused", C.c_int)]
 
     def as_dict(self):
         return {"fft_family": list(self.fft_family), "fft_cfg": list(self.fft_cfg),
                 "fft_lines": list(self.fft_lines), "comm_kind": int(self.comm_kind),
                 "comm_tile": int(self.comm_tile), "comm_tile_ab": int(self.comm_tile_ab),
-                "grad_fused": int(self.grad_fused), "tri_fields": int(self.tri_fields)}
+                "grad_fused": int(self.grad_fused), "tri_fields": int(self.tri_fields),
+                "hf_fused": int(self.hf_fused)}
 
 
 class _Laser(C.Structure):

@@ -226,6 +226,12 @@ int edus_timing_get(edus_ctx* ctx, const char* kernel_class, double* total_ms, l
  *                  exchange or the Peierls gauge when H0(k) and every r_d(k)
  *                  are Hermitian to 16 ulp of their largest element; for
  *                  exactly Hermitian fields both give identical results.
+ *   hf_fused       1 (default): with exchange, the forward row transform of
+ *                  the exchange term comes out of the gradient's row pass
+ *                  (same loads; two-step row kernels, two-pass gradient,
+ *                  field along grid axis 1); 0: a pass of its own.  The two
+ *                  agree to rounding (the compiler contracts multiply-adds
+ *                  per kernel).
  * Within one family, FFT tiles give bit-identical results, and so do the
  * commutator tiles; the two FP64 commutator engines differ in summation order
  * only, and the two gradient forms in rounding only (the compiler contracts
@@ -240,6 +246,7 @@ typedef struct {
     int comm_tile_ab;
     int grad_fused;
     int tri_fields;
+    int hf_fused;
 } edus_tuning;
 int edus_get_tuning(edus_ctx* ctx, edus_tuning* out);
 

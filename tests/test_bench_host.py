@@ -29,6 +29,10 @@ def test_comm_bytes_per_stage_model():
     assert bench.kernel_bytes("comm", npl, nk, 14, 1, tri_fields=True) == pytest.approx((6.5 - 2 * 91 / 196) * s)
     assert bench.kernel_bytes("comm", npl, nk, 14, 1, f32_fields=True, tri_fields=True) == \
         pytest.approx((6.5 - 2 * (1 - 105 / 392)) * s)
+    # exchange with the forward row pass taken from the gradient's: grad_row writes it too
+    assert bench.kernel_bytes("grad_row", 64, 8100, 8, 2, "AB", 4, True, hf_fused=True) == \
+        pytest.approx(4 * 16.0 * 64 * 8100)
+    assert bench.kernel_bytes("grad_row", npl, nk, 14, 1, hf_fused=True) == pytest.approx(3 * s)
     # the Peierls gauge reads S_A[H] in place of H0 and xi whatever their storage
     assert bench.kernel_bytes("comm", npl, nk, 14, 1, peierls=True, tri_fields=True) == pytest.approx(4.5 * s)
 

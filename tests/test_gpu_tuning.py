@@ -75,6 +75,7 @@ def test_fft_tiles_give_identical_results(model_dir, name, grid, layout):
         t.comm_tile = base.comm_tile
         t.comm_tile_ab = base.comm_tile_ab
         t.tri_fields = base.tri_fields
+        t.hf_fused = base.hf_fused
         try:
             eng.set_tuning(t)
         except edus.EdusError:
@@ -261,3 +262,54 @@ def test_set_tuning_after_graph_replay(model_dir):
         eng.rk4_steps(0.05, E[5:])
         states.append(eng.get_state())
     assert np.array_equal(states[0], states[1])
+
+
+@pytest.mark.parametrize("grid,layout", [([9, 11], FULL), ([9, 30], FULL), ([9, 15], HERM), ([10, 24], FULL)])
+def test_exchange_row_pass_fused_identical(model_dir, grid, layout):
+    """With exchange, the gradient's row pass can also store the exchange
+    term's forward row transform (hf_fused 1, the default): right-hand sides
+    and RK4 / AB4 states equal to the separate pass's to rounding (the same
+    DFT arithmetic in another kernel; the compiler contracts multiply-adds per
+    kernel, and most shapes come out bit-identical), with the field along
+    both grid axes, along axis 1 only and along axis 0 only (no row pass:
+    nothing to fuse); row lengths with one and with two register steps."""
+    p = _model(model_dir, "hsex8_hex90", grid)
+    o = oracle_py.Oracle(p)
+    eng = _engine(o, layout, p["filledbands"], coulomb=True)
+    t = eng.get_tuning()
+    assert t.hf_fused == 1 and t.fft_family[1] == 1
+    rho = o.rho + 0.05 * _rho(o.Nk, o.nb, 4, hermitian=True)
+    A = o.lattice()
+    # c_j = a_j . E: a field perpendicular to a_1 (a_0) has no axis-1 (axis-0) term
+    z = np.array([0.0, 0.0, 1.0])
+    e0, e1 = np.cross(A[:, 1], z), np.cross(A[:, 0], z)
+    fields = {"both": [3e-3, -2e-3, 0.0], "axis 0 only": list(3e-3 * e0 / np.linalg.norm(e0)),
+              "axis 1 only": list(3e-3 * e1 / np.linalg.norm(e1))}
+    for what, E in fields.items():
+        f = {}
+        for on in (1, 0):
+            t = eng.get_tuning()
+            t.hf_fused = on
+            eng.set_tuning(t)
+            f[on] = eng.rhs(E, rho)
+        _say(grid, layout, what, "fused vs separate: max|df|", np.abs(f[1] - f[0]).max())
+        assert np.abs(f[1] - f[0]).max() <= 1e-15 * np.abs(f[0]).max()
+        if what == "both":
+            ref = o.rhs_E(E, rho)
+            err = np.abs(f[1] - ref).max() if layout == FULL else np.abs(_upper(f[1]) - _upper(ref)).max()
+            assert err <= 1e-11 * max(1.0, np.abs(ref).max())
+    rng = np.random.default_rng(2)
+    Es = rng.uniform(-4e-3, 4e-3, size=(6, 3, 3)) * np.array([1.0, 1.0, 0.0])
+    Eab = rng.uniform(-4e-3, 4e-3, size=(6, 3)) * np.array([1.0, 1.0, 0.0])
+    states = {}
+    for on in (1, 0):
+        t = eng.get_tuning()
+        t.hf_fused = on
+        eng.set_tuning(t)
+        eng.set_state(o.rho)
+        eng.ab_reset()
+        eng.rk4_steps(0.05, Es)
+        eng.ab_steps(0.05, 4, Eab)
+        states[on] = eng.get_state()
+    _say(grid, layout, "6 RK4 + 6 AB4 steps, fused vs separate: max|d rho|", np.abs(states[1] - states[0]).max())
+    assert np.abs(states[1] - states[0]).max() <= 1e-14
