@@ -321,6 +321,7 @@ void Batch::rk4_steps(double dt, int nsteps, const double* E)
             }
         join_to(bs);
     }
+    for (auto& g : groups) g.e->state_dirty = true;  // replays wrote states no host call saw
     // the pinned staging is rewritten by the next call
     sync_all();
     HIP_CHECK(hipGetLastError());
@@ -335,6 +336,7 @@ void Batch::copy_state(int i, bool to_member)
     double2* slot = g.e->y + size_t(where[i].second) * g.e->inst_elems();
     sync_all();
     HIP_CHECK(hipStreamSynchronize(m.s));
+    (to_member ? m : *g.e).check_final_state();  // the state being replaced is checked first
     if (to_member)
         HIP_CHECK(hipMemcpyAsync(m.y, slot, bytes, hipMemcpyDeviceToDevice, g.e->s));
     else

@@ -131,12 +131,15 @@ constexpr int h_tile_slots()
     return (np * plane_stride<KIND, TK>() + 63) / 64 * 64;
 }
 
+// the monitor's per-wave maxima (trace, Hermiticity) of up to 8 waves
+constexpr int kMonRedSlots = 8;
+
 template <int NB, int LAYOUT, int KIND, int TILE, bool TRI>
 constexpr size_t comm_lds_bytes()
 {
-    // rho and H tiles plus the Hartree diagonal; the monitor's block
-    // reduction reuses the H tile (2 doubles per wave)
-    return (size_t(tile_slots<NB, LAYOUT, KIND, TILE>()) + h_tile_slots<NB, LAYOUT, KIND, TILE, TRI>() + kMaxNb) *
+    // rho and H tiles, the Hartree diagonal and the monitor's reduction slots
+    return (size_t(tile_slots<NB, LAYOUT, KIND, TILE>()) + h_tile_slots<NB, LAYOUT, KIND, TILE, TRI>() + kMaxNb +
+            kMonRedSlots) *
            sizeof(double2);
 }
 
@@ -263,6 +266,8 @@ __global__ void __launch_bounds__((comm_threads<KIND, TILE>())) comm_kernel(Comm
     double2* sR = lds;
     double2* sH = lds + NSLOT;
     double2* sHd = sH + h_tile_slots<NB, LAYOUT, KIND, TILE, TRI>();
+    double2* sMon = sHd + kMaxNb;  // (x, y) = (trace, Hermiticity) maximum of wave w
+    static_assert(NT / 64 <= kMonRedSlots, "one monitor slot per wave");
     const int Nk = a.Nk;
     // each XCD takes a contiguous range of tiles, so that tiles sharing a
     // cache line (TK = 4) meet in the same L2; t covers the tiles of instance
@@ -522,6 +527,50 @@ __global__ void __launch_bounds__((comm_threads<KIND, TILE>())) comm_kernel(Comm
                     if (ok[j]) ex[i - 1][j] = u.hist[i][off[j]];
             }
     }
+    if (u.mon) {
+        // Trace and Hermiticity of the stage input (the state the step starts
+        // from: RK4 stage 1, every AB step) over the tile's TK k-points, read
+        // from the rho tile while the update operands are in flight; each
+        // wave leaves its maxima in sMon, which thread 0 folds into the
+        // monitor slots after a later barrier
+        double mt = 0., mh = 0.;
+        // trace and diagonal imaginary part: one thread per k-point of the tile
+        for (int kk = threadIdx.x; kk < TK; kk += NT) {
+            const int k = k0 + kk;
+            if (k >= Nk) continue;
+            double2 tr = make_double2(-u.nf, 0.);
+            for (int n = 0; n < NB; ++n) {
+                bool c;
+                const double2 v = sR[slot<NB, LAYOUT>(n, n, c) * PS + kk];
+                tr = cadd(tr, v);
+                mh = mon_max(mh, 2.0 * fabs(v.y));
+            }
+            const double tt = sqrt(tr.x * tr.x + tr.y * tr.y);
+            mt = mon_max(mt, tt);
+        }
+        // |rho_nm - conj(rho_mn)| over all pairs n < m, by the owner of (n, m)
+        if constexpr (LAYOUT == kFull) {
+#pragma unroll
+            for (int j = 0; j < PER; ++j) {
+                const int idx = threadIdx.x + j * NT;
+                const int p = idx / TK;
+                const int kk = idx - p * TK;
+                const int n = p / NB, m = p - (p / NB) * NB;
+                if (!ok[j] || n >= m) continue;
+                const double2 x = sR[slot_of[j]];
+                const double2 z = sR[(m * NB + n) * PS + kk];
+                const double d = sqrt((x.x - z.x) * (x.x - z.x) + (x.y + z.y) * (x.y + z.y));
+                mh = mon_max(mh, d);
+            }
+        }
+        // across the wave by shuffles (no NaN is left: mon_max made it +inf)
+#pragma unroll
+        for (int w = 32; w > 0; w >>= 1) {
+            mt = fmax(mt, __shfl_xor(mt, w));
+            mh = fmax(mh, __shfl_xor(mh, w));
+        }
+        if ((threadIdx.x & 63) == 0) sMon[threadIdx.x >> 6] = make_double2(mt, mh);
+    }
     __syncthreads();
 
     if constexpr (KIND != kCommScalar) {
@@ -619,63 +668,12 @@ __global__ void __launch_bounds__((comm_threads<KIND, TILE>())) comm_kernel(Comm
     }
 
     if (u.mon) {
-        // Trace and Hermiticity of the state this stage wrote (AB: y, RK: out1)
-        // over the tile's TK k-points, from the values the threads still hold,
-        // put back into the rho tile (no longer read by anyone after the barrier).
-        __syncthreads();
-#pragma unroll
-        for (int j = 0; j < PER; ++j)
-            if (ok[j]) sR[slot_of[j]] = x1v[j];
-        __syncthreads();
-        double mt = 0., mh = 0.;
-        // trace and diagonal imaginary part: one thread per k-point of the tile
-        for (int kk = threadIdx.x; kk < TK; kk += NT) {
-            const int k = k0 + kk;
-            if (k >= Nk) continue;
-            double2 tr = make_double2(-u.nf, 0.);
-            for (int n = 0; n < NB; ++n) {
-                bool c;
-                const double2 v = sR[slot<NB, LAYOUT>(n, n, c) * PS + kk];
-                tr = cadd(tr, v);
-                mh = mon_max(mh, 2.0 * fabs(v.y));
-            }
-            const double t = sqrt(tr.x * tr.x + tr.y * tr.y);
-            mt = mon_max(mt, t);
-        }
-        // |rho_nm - conj(rho_mn)| over all pairs n < m: the owner of (n, m)
-        // holds it in a register and reads its mirror from the tile
-        if constexpr (LAYOUT == kFull) {
-#pragma unroll
-            for (int j = 0; j < PER; ++j) {
-                const int idx = threadIdx.x + j * NT;
-                const int p = idx / TK;
-                const int kk = idx - p * TK;
-                const int n = p / NB, m = p - (p / NB) * NB;
-                if (!ok[j] || n >= m) continue;
-                const double2 x = x1v[j];
-                const double2 z = sR[(m * NB + n) * PS + kk];
-                const double d = sqrt((x.x - z.x) * (x.x - z.x) + (x.y + z.y) * (x.y + z.y));
-                mh = mon_max(mh, d);
-            }
-        }
-        // block maximum: across the wave by shuffles (no NaN is left: mon_max
-        // made it +inf), then across the waves in LDS
-#pragma unroll
-        for (int w = 32; w > 0; w >>= 1) {
-            mt = fmax(mt, __shfl_xor(mt, w));
-            mh = fmax(mh, __shfl_xor(mh, w));
-        }
-        constexpr int NW = NT / 64;
-        double* sred = reinterpret_cast<double*>(sH);  // the H tile is no longer read
-        if ((threadIdx.x & 63) == 0) {
-            sred[threadIdx.x >> 6] = mt;
-            sred[NW + (threadIdx.x >> 6)] = mh;
-        }
-        __syncthreads();
+        // every wave wrote its maxima before the barrier ahead of the products
         if (threadIdx.x == 0) {
-            for (int w = 1; w < NW; ++w) {
-                mt = fmax(mt, sred[w]);
-                mh = fmax(mh, sred[NW + w]);
+            double mt = 0., mh = 0.;
+            for (int w = 0; w < NT / 64; ++w) {
+                mt = fmax(mt, sMon[w].x);
+                mh = fmax(mh, sMon[w].y);
             }
             mon_slot(u.mon, blockIdx.x, mt, mh);
         }

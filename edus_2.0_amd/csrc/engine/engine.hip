@@ -996,7 +996,9 @@ void Engine::evaluate(const double2* in, const double E[3], const StageUpdate& u
 // The reference adds dt/3 k2 to acc in stage 2; here stage 3 recovers it as
 // 2/3 (aux_in - y) = dt/3 k2 from the two arrays it reads anyway, so stage 2
 // neither reads nor writes acc (a step moves 26 plane sets instead of 28).
-// Stage 4 carries the per-step trace / Hermiticity monitor of the new y.
+// Stage 1 carries the per-step trace / Hermiticity monitor of the y the step
+// starts from (the y it writes is checked by the next step, or by
+// check_final_state).
 void Engine::rk4_stages(double dt, StageUpdate u[4]) const
 {
     for (int i = 0; i < 4; ++i) u[i] = StageUpdate{};
@@ -1006,11 +1008,12 @@ void Engine::rk4_stages(double dt, StageUpdate u[4]) const
     u[2].x1 = acc; u[2].out1 = acc; u[2].b1 = dt / 3.; u[2].g1 = 2. / 3.;
     u[2].x2 = y; u[2].out2 = aux; u[2].b2 = dt;
     u[3].x1 = acc; u[3].out1 = y; u[3].b1 = dt / 6.;
-    u[3].mon = mon; u[3].nf = double(nf);
+    u[0].mon = mon; u[0].nf = double(nf);
 }
 
 void Engine::rk4_step(double dt, const double* E3x3, int record_order)
 {
+    state_dirty = true;  // the state it writes is checked by the next step or check_final_state
     StageUpdate u[4];
     rk4_stages(dt, u);
     if (record_order > 0) {
@@ -1190,6 +1193,7 @@ void Engine::evaluate_tab(const double2* in, int ts, int j, const bool use_axis[
 
 void Engine::rk4_step_tab(double dt, int ts, const bool use_axis[2], const bool use_xi[3])
 {
+    state_dirty = true;  // the state it writes is checked by the next step or check_final_state
     StageUpdate u[4];
     rk4_stages(dt, u);
     evaluate_tab(y, ts, 0, use_axis, use_xi, u[0]);
@@ -1262,6 +1266,7 @@ void Engine::evaluate_peierls(const double2* in, const double E[3], const double
 
 void Engine::rk4_step_peierls(double dt, const double* E3x3, const double* A3x3)
 {
+    state_dirty = true;  // the state it writes is checked by the next step or check_final_state
     StageUpdate u[4];
     rk4_stages(dt, u);
     evaluate_peierls(y, E3x3 + 0, A3x3 + 0, u[0]);
@@ -1272,6 +1277,7 @@ void Engine::rk4_step_peierls(double dt, const double* E3x3, const double* A3x3)
 
 void Engine::ab_step_peierls(double dt, int order, const double* E3, const double* A3)
 {
+    state_dirty = true;  // the state it writes is checked by the next step or check_final_state
     ensure_hist();
     static const double b4[5] = {55. / 24., -59. / 24., 37. / 24., -3. / 8., 0.};
     static const double b5[5] = {1901. / 720., -2774. / 720., 2616. / 720., -1274. / 720., 251. / 720.};
@@ -1299,6 +1305,7 @@ void Engine::ensure_hist()
 
 void Engine::ab_step(double dt, int order, const double* E3)
 {
+    state_dirty = true;  // the state it writes is checked by the next step or check_final_state
     ensure_hist();
     // DESolver.hpp:110-126 coefficients, 202-220 update
     static const double b4[5] = {55. / 24., -59. / 24., 37. / 24., -3. / 8., 0.};
@@ -1319,8 +1326,19 @@ void Engine::ab_step(double dt, int order, const double* E3)
     abidx[0] = tmp;
 }
 
+// The step kernels check the state each step starts from (RK4 stage 1, every
+// AB step); the state the last step wrote is checked here, once, before the
+// monitors are read.
+void Engine::check_final_state()
+{
+    if (!state_dirty) return;
+    launch_state_monitor(y, nb, Nk, layout, npl, B, double(nf), mon, s);
+    state_dirty = false;
+}
+
 void Engine::peek_monitor(double out[2])
 {
+    check_final_state();
     std::vector<unsigned long long> m(kMonWords);
     HIP_CHECK(hipMemcpyAsync(m.data(), mon, kMonWords * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipStreamSynchronize(s));
@@ -1351,6 +1369,7 @@ void Engine::enforce_monitor_limits()
 // maxima over every monitor slot since the last read; the slots are cleared
 void Engine::read_monitor(double out[2])
 {
+    check_final_state();
     std::vector<unsigned long long> m(kMonWords);
     HIP_CHECK(hipMemcpyAsync(m.data(), mon, kMonWords * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipMemsetAsync(mon, 0, kMonWords * sizeof(unsigned long long), s));
@@ -1406,7 +1425,10 @@ int edus_destroy(edus_ctx* ctx)
 int edus_set_state(edus_ctx* ctx, const double* rho_k)
 {
     EDUS_REQUIRE(ctx && rho_k);
-    return guarded([&] { ctx->e->upload(rho_k, ctx->e->y); });
+    return guarded([&] {
+        ctx->e->check_final_state();  // the state being replaced is checked first
+        ctx->e->upload(rho_k, ctx->e->y);
+    });
 }
 
 int edus_get_state(edus_ctx* ctx, double* rho_k)

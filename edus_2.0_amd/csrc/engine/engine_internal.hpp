@@ -124,6 +124,9 @@ class Engine {
     int comm_tile_ab = 0;                 // tile of the Adams-Bashforth kernel
     bool hf_fuse = true;  // exchange: its forward row pass taken from the gradient's row pass
     void read_monitor(double out[2]);
+    // steps since the last check_final_state: the state they wrote is unchecked
+    bool state_dirty = false;
+    void check_final_state();
     // the monitors' maxima since the last read, left in place
     void peek_monitor(double out[2]);
     // > 0: step calls check the monitors against these limits (edus_monitor_limits)

@@ -221,7 +221,63 @@ __global__ void add_diag_kernel(double2* __restrict__ H, const double2* __restri
 
 inline unsigned grid1d(long n, int bs) { return unsigned((n + bs - 1) / bs); }
 
+// Trace and Hermiticity monitors of B stacked states ([B][P][Nk] planes in
+// `layout`), one thread per (instance, k): max |Tr rho - nf|, and the largest
+// |rho_nm - conj(rho_mn)| (FULL) or 2 |Im rho_nn| (HERMITIAN), into the
+// monitor slots as the commutator kernel folds its own checks
+__global__ void __launch_bounds__(kObsBlock) state_monitor_kernel(const double2* __restrict__ rho, int nb, int Nk,
+                                                                  int layout, int P, int B, double nf,
+                                                                  unsigned long long* __restrict__ mon)
+{
+    const long i = long(blockIdx.x) * kObsBlock + threadIdx.x;
+    double mt = 0., mh = 0.;
+    if (i < long(B) * Nk) {
+        const int inst = int(i / Nk);
+        const int k = int(i - long(inst) * Nk);
+        const double2* r = rho + long(inst) * P * Nk + k;
+        double2 tr = make_double2(-nf, 0.);
+        for (int n = 0; n < nb; ++n) {
+            const double2 v = r[long(diag_plane(n, nb, layout)) * Nk];
+            tr = cadd(tr, v);
+            if (layout != kFull) mh = mon_max(mh, 2.0 * fabs(v.y));
+        }
+        mt = mon_max(mt, sqrt(tr.x * tr.x + tr.y * tr.y));
+        if (layout == kFull)
+            for (int n = 0; n < nb; ++n)
+                for (int m = n; m < nb; ++m) {
+                    const double2 x = r[long(n * nb + m) * Nk], z = r[long(m * nb + n) * Nk];
+                    mh = mon_max(mh, sqrt((x.x - z.x) * (x.x - z.x) + (x.y + z.y) * (x.y + z.y)));
+                }
+    }
+#pragma unroll
+    for (int w = 32; w > 0; w >>= 1) {
+        mt = fmax(mt, __shfl_xor(mt, w));
+        mh = fmax(mh, __shfl_xor(mh, w));
+    }
+    __shared__ double sm[2 * (kObsBlock / 64)];
+    if ((threadIdx.x & 63) == 0) {
+        sm[threadIdx.x >> 6] = mt;
+        sm[kObsBlock / 64 + (threadIdx.x >> 6)] = mh;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < kObsBlock / 64; ++w) {
+            mt = fmax(mt, sm[w]);
+            mh = fmax(mh, sm[kObsBlock / 64 + w]);
+        }
+        mon_slot(mon, blockIdx.x, mt, mh);
+    }
+}
+
 }  // namespace
+
+void launch_state_monitor(const double2* rho, int nb, int Nk, int layout, int P, int B, double nf,
+                          unsigned long long* mon, hipStream_t s)
+{
+    const long n = long(B) * Nk;
+    hipLaunchKernelGGL(state_monitor_kernel, dim3(unsigned((n + kObsBlock - 1) / kObsBlock)), dim3(kObsBlock), 0, s,
+                       rho, nb, Nk, layout, P, B, nf, mon);
+}
 
 void launch_to_planes(const double2* full, double2* planes, int nb, int Nk, int layout, hipStream_t s)
 {

@@ -28,5 +28,8 @@ void launch_axpy(double2* y, const double2* x, double a, long n, hipStream_t s);
 // out = in rounded to FP32 (nearest), n complex values
 void launch_to_f32(const double2* in, float2* out, long n, hipStream_t s);
 void launch_add_diag(double2* H, const double2* d, double scale, int nb, int Nk, int layout, hipStream_t s);
+// trace / Hermiticity monitors of B stacked states into the monitor slots
+void launch_state_monitor(const double2* rho, int nb, int Nk, int layout, int P, int B, double nf,
+                          unsigned long long* mon, hipStream_t s);
 
 }  // namespace edus

@@ -168,10 +168,13 @@ int edus_ab_seed(edus_ctx* ctx, double dt, int order, const double* E);
  * (Re vx, Im vx, Re vy, Im vy, Re vz, Im vz).  Any pointer may be NULL. */
 int edus_observables(edus_ctx* ctx, double* pop_bloch, double* pop_wannier, double* vel);
 
-/* Invariant monitors accumulated over the steps since the last call (and
- * reset by it): max over steps and k of |Tr rho(k) - filledbands| and of the
- * Hermiticity defect max_{n<=m} |rho_nm - conj(rho_mn)| (HERMITIAN layout:
- * twice the largest |Im rho_nn|, the only part its storage leaves free). */
+/* Invariant monitors accumulated since the last call (and reset by it): the
+ * maximum, over every state the step calls since then started from or wrote
+ * and over k, of |Tr rho(k) - filledbands| and of the Hermiticity defect
+ * max_{n<=m} |rho_nm - conj(rho_mn)| (HERMITIAN layout: twice the largest
+ * |Im rho_nn|, the only part its storage leaves free).  The step kernels check
+ * the state each step starts from; the state the last step wrote is checked
+ * when the monitors are read, or before edus_set_state replaces it. */
 int edus_check(edus_ctx* ctx, double* trace_dev, double* herm_dev);
 /* Limits on those monitors (no counterpart in the reference, which checks
  * nothing while it propagates).  With a limit > 0 set, edus_rk4_steps,
