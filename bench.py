@@ -165,7 +165,7 @@ def main():
     ap.add_argument("--graph", type=int, default=20,
                     help="RK4 steps per replayed hipGraph in the timed region (0: direct launches)")
     ap.add_argument("--cpu-steps", type=int, default=0, help="CPU baseline steps (0: about 15 s worth)")
-    ap.add_argument("--coarse-steps", type=int, default=10, help="parareal: coarse RK4 steps per slice")
+    ap.add_argument("--coarse-steps", type=int, default=20, help="parareal: coarse RK4 steps per slice")
     ap.add_argument("--train-runs", type=int, default=64, help="parareal: random-pulse training runs")
     ap.add_argument("--k-samples", type=int, default=256, help="parareal: k-points sampled per run and slice")
     ap.add_argument("--epochs", type=int, default=20, help="parareal: training epochs")
