@@ -379,7 +379,9 @@ __global__ void __launch_bounds__((comm_threads<KIND, TILE>())) comm_kernel(Comm
                 for (int j = 0; j < PER; ++j)
                     if (h_ok[j]) fb[j] = a.addf[1][off[j] + dh];
             }
-            if (a.hdiag && threadIdx.x < NB) sHd[threadIdx.x] = a.hdiag[inst * NB + threadIdx.x];
+            if (a.hsum && threadIdx.x < NB)
+                sHd[threadIdx.x] = hartree_shift(a.hsum + long(inst) * NPL, a.hart + long(inst) * NB * NB, threadIdx.x,
+                                                 NB, LAYOUT, a.inv_nk);
             __builtin_amdgcn_s_waitcnt(0);  // this wave's direct loads are in LDS
             __syncthreads();                // and every other wave's
 #pragma unroll
@@ -433,7 +435,9 @@ __global__ void __launch_bounds__((comm_threads<KIND, TILE>())) comm_kernel(Comm
                 for (int j = 0; j < PER; ++j)
                     if (h_ok[j]) xb[j] = a.add[1][off[j] + dh];
             }
-            if (a.hdiag && threadIdx.x < NB) sHd[threadIdx.x] = a.hdiag[inst * NB + threadIdx.x];
+            if (a.hsum && threadIdx.x < NB)
+                sHd[threadIdx.x] = hartree_shift(a.hsum + long(inst) * NPL, a.hart + long(inst) * NB * NB, threadIdx.x,
+                                                 NB, LAYOUT, a.inv_nk);
             __builtin_amdgcn_s_waitcnt(0);  // this wave's direct loads are in LDS
             __syncthreads();                // and every other wave's
             // H = H0 + sum of the addends (in the order xi_0, xi_1, xi_2, Sigma)
@@ -487,7 +491,7 @@ __global__ void __launch_bounds__((comm_threads<KIND, TILE>())) comm_kernel(Comm
                 }
             }
         }
-        if (!TRI && a.hdiag) {  // (exchange only, never with upper-triangle fields)
+        if (!TRI && a.hsum) {  // (exchange only, never with upper-triangle fields)
 #pragma unroll
             for (int j = 0; j < PER; ++j) {
                 if (!ok[j]) continue;
@@ -761,8 +765,9 @@ void launch_comm(int nb, int layout, int kind, int tile, const CommArgs& args, h
         a.addf[r] = nullptr;
         a.addf_dd[r] = -1;
     }
-    if (args.tri && (layout != kFull || args.sigma || args.hdiag || kind == kCommScalar))
+    if (args.tri && (layout != kFull || args.sigma || args.hsum || kind == kCommScalar))
         throw std::runtime_error("upper-triangle H0 and xi: FULL layout without exchange, MFMA engines only");
+    if (args.hsum && !args.hart) throw std::runtime_error("the Hartree shift needs V with the plane sums");
     if (kind == kCommMfma32 && args.H0f) {
         if (args.sigma) throw std::runtime_error("FP32 H0 with exchange: H0 holds -Sigma[rho0] and stays FP64");
         a.addf[a.n_addf++] = args.H0f;

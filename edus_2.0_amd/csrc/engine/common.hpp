@@ -78,6 +78,18 @@ __device__ inline int xcd_contiguous_block()
 // monitor instead of being dropped by fmax
 __device__ inline double mon_max(double m, double v) { return (v != v) ? __builtin_huge_val() : fmax(m, v); }
 
+// Hartree shift of band n (Coulomb.cpp:150-159): sum_m V_nm rho_mm(R = 0),
+// rho(R = 0) being the k-mean of the diagonal planes.  hsum holds every plane's
+// plain sum over k (the zero-frequency value of its forward transform, stored
+// by the exchange convolution pass), V = hart [nb][nb] (real).
+__device__ inline double2 hartree_shift(const double2* hsum, const double* hart, int n, int nb, int layout,
+                                        double inv_nk)
+{
+    double2 h = make_double2(0., 0.);
+    for (int m = 0; m < nb; ++m) h = cadd(h, cscale(cscale(hsum[diag_plane(m, nb, layout)], inv_nk), hart[n * nb + m]));
+    return h;
+}
+
 __device__ inline void mon_slot(unsigned long long* mon, unsigned block, double trace_dev, double herm_dev)
 {
     const int o = int(block % kMonSlots) * kMonStride;

@@ -270,7 +270,7 @@ Engine::Engine(const edus_desc& d, const EngineOptions& opt)
         for (int i = 0; i < nb * nb; ++i) hr[i] = d.hartree[2 * i];  // V is real (ModelCoulomb.cpp:168-176)
         alloc(reinterpret_cast<void**>(&hart), nb * nb * sizeof(double));
         alloc(reinterpret_cast<void**>(&hdiag), nb * sizeof(double2));
-        alloc(reinterpret_cast<void**>(&diag_partial), size_t(nb) * diag_blocks(Nk) * sizeof(double2));
+        alloc(reinterpret_cast<void**>(&hsum), size_t(npl) * sizeof(double2));
         HIP_CHECK(hipMemcpyAsync(hart, hr.data(), nb * nb * sizeof(double), hipMemcpyHostToDevice, s));
         // H_ee = -W o (rho - rho0) + Hartree(rho - rho0) (Coulomb.cpp:150-171).  The rho0
         // part does not change in time: fold it into H0 once, H0 <- H0 - HF[rho0].
@@ -278,7 +278,8 @@ Engine::Engine(const edus_desc& d, const EngineOptions& opt)
             upload(d.rho0k, aux);
         else  // the reference state is the initial condition computed above
             HIP_CHECK(hipMemcpyAsync(aux, y, pb, hipMemcpyDeviceToDevice, s));
-        fock(aux);  // T = Sigma[rho0], hdiag = Hartree[rho0]
+        fock(aux);  // T = Sigma[rho0], hsum = the plane sums of rho0
+        launch_hartree_sums(hsum, hart, nb, layout, npl, 1, 1.0 / double(Nk), hdiag, s);  // Hartree[rho0]
         launch_axpy(H0, T, -1.0, long(npl) * Nk, s);
         launch_add_diag(H0, hdiag, -1.0, nb, Nk, layout, s);
         HIP_CHECK(hipMemsetAsync(aux, 0, pb, s));
@@ -546,12 +547,14 @@ AxisFFTArgs Engine::axis_args(int axis, const FFTChoice& c, int mode, const doub
 }
 
 void Engine::axis_fft(int axis, int mode, const double2* in, const double2* addend, double2* out, double coef,
-                      double scale, const double* mult, const char* cls, const double* coef_tab, double2* out2)
+                      double scale, const double* mult, const char* cls, const double* coef_tab, double2* out2,
+                      double2* r0)
 {
     if (mode == 5 && (!fft_choice[axis].two_step || !out2))
         throw EngineError(EDUS_ERR_STATE, "mode 5 needs the two-step kernels and a second output");
     AxisFFTArgs a = axis_args(axis, fft_choice[axis], mode, in, addend, out, coef, scale, mult);
     a.out2 = out2;
+    a.r0 = r0;
     if (coef_tab) {
         a.coef_tab = coef_tab;
         a.coef_stride = 2;
@@ -935,13 +938,15 @@ const double2* Engine::kgradient(const double2* in, const double E[3], const dou
     return D;
 }
 
-// T = F^-1[ -W o (F rho)/N ]  (screened exchange in k),  hdiag = Hartree shift
+// T = F^-1[ -W o (F rho)/N ]  (screened exchange in k).  The column pass also
+// stores each plane's zero-frequency value, its sum over k, in hsum: the
+// commutator forms the Hartree shift from it (hartree_shift), so the Hartree
+// term costs no pass of its own.
 void Engine::fock(const double2* in, bool row_done)
 {
     if (!row_done) axis_fft(1, 1, in, nullptr, T, 0.0, 1.0, nullptr, "hf_row");
-    axis_fft(0, 3, T, nullptr, T, 0.0, -1.0 / double(Nk), W, "hf_col");
+    axis_fft(0, 3, T, nullptr, T, 0.0, -1.0 / double(Nk), W, "hf_col", nullptr, nullptr, hsum);
     axis_fft(1, 2, T, nullptr, T, 0.0, 1.0, nullptr, "hf_row");
-    launch_hartree(in, nb, Nk, layout, npl, B, hart, diag_partial, hdiag, s);
 }
 
 void Engine::transform_to_R(const double2* in, double2* out)
@@ -978,7 +983,9 @@ void Engine::evaluate(const double2* in, const double E[3], const StageUpdate& u
     field_args(c, use_xi, comm_kind);
     for (int d = 0; d < 3; ++d) c.E[d] = E[d];
     c.sigma = coulomb ? T : nullptr;
-    c.hdiag = coulomb ? hdiag : nullptr;
+    c.hsum = coulomb ? hsum : nullptr;
+    c.hart = hart;
+    c.inv_nk = 1.0 / double(Nk);
     c.up = up;
     TimedLaunch t;
     tbegin("comm", t);
@@ -1125,8 +1132,7 @@ Engine::Engine(const std::vector<const Engine*>& members)
         alloc(reinterpret_cast<void**>(&T), pb);
         alloc(reinterpret_cast<void**>(&W), size_t(B) * npl * Nk * sizeof(double));
         alloc(reinterpret_cast<void**>(&hart), size_t(B) * nb * nb * sizeof(double));
-        alloc(reinterpret_cast<void**>(&hdiag), size_t(B) * nb * sizeof(double2));
-        alloc(reinterpret_cast<void**>(&diag_partial), size_t(B) * nb * diag_blocks(Nk) * sizeof(double2));
+        alloc(reinterpret_cast<void**>(&hsum), size_t(B) * npl * sizeof(double2));
         for (int i = 0; i < B; ++i) {
             HIP_CHECK(hipMemcpyAsync(W + size_t(i) * npl * Nk, members[i]->W, size_t(npl) * Nk * sizeof(double),
                                      hipMemcpyDeviceToDevice, s));
@@ -1183,7 +1189,9 @@ void Engine::evaluate_tab(const double2* in, int ts, int j, const bool use_axis[
     field_args(c, use_xi, comm_kind);
     c.E_tab = E_dev + row * B * 3;
     c.sigma = coulomb ? T : nullptr;
-    c.hdiag = coulomb ? hdiag : nullptr;
+    c.hsum = coulomb ? hsum : nullptr;
+    c.hart = hart;
+    c.inv_nk = 1.0 / double(Nk);
     c.up = up;
     TimedLaunch t;
     tbegin("comm", t);
@@ -1256,7 +1264,9 @@ void Engine::evaluate_peierls(const double2* in, const double E[3], const double
     c.D = nullptr;
     c.H0 = Hp;
     c.sigma = coulomb ? T : nullptr;
-    c.hdiag = coulomb ? hdiag : nullptr;
+    c.hsum = coulomb ? hsum : nullptr;
+    c.hart = hart;
+    c.inv_nk = 1.0 / double(Nk);
     c.up = up;
     TimedLaunch t;
     tbegin("comm", t);

@@ -107,7 +107,7 @@ class Engine {
     void prime_launch_attributes();
     double2 *U = nullptr, *T = nullptr, *stage = nullptr;
     double *W = nullptr, *hart = nullptr;
-    double2 *hdiag = nullptr, *diag_partial = nullptr;
+    double2 *hdiag = nullptr, *hsum = nullptr;  // hdiag: set-up only
     int force_kind = -1;  // >= 0: the autotuner keeps this commutator engine (batch groups)
     double *obs_partial = nullptr, *obs_out = nullptr;
     unsigned long long* mon = nullptr;
@@ -190,7 +190,7 @@ class Engine {
     void alloc(void** p, size_t bytes);
     void axis_fft(int axis, int mode, const double2* in, const double2* addend, double2* out, double coef,
                   double scale, const double* mult, const char* cls, const double* coef_tab = nullptr,
-                  double2* out2 = nullptr);
+                  double2* out2 = nullptr, double2* r0 = nullptr);
     // D = E.grad rho with c[j] = c_j / N_j, or per instance from ctab [B][2]
     // fwd1 (may be null): also store the forward row transform of in there
     // from the row pass (mode 5) where it can; *fwd1_done says whether it did

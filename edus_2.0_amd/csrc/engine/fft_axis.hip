@@ -200,6 +200,7 @@ __global__ void __launch_bounds__(NT) axis_fft_kernel(AxisFFTArgs a)
     const int tb = xcd_contiguous_block();  // neighbouring line groups side by side in one XCD
     long base, cstride, rowstride;
     long row0 = 0, plane0 = 0;  // first global row (axis 1), plane (axis 0)
+    int col0 = 0;               // first column (axis 0)
     int nlines;
     if (a.axis == 1) {
         row0 = long(tb) * L;
@@ -211,6 +212,7 @@ __global__ void __launch_bounds__(NT) axis_fft_kernel(AxisFFTArgs a)
         const int nblk = (a.N1 + L - 1) / L;
         plane0 = tb / nblk;
         const int first = int(tb - plane0 * nblk) * L;
+        col0 = first;
         nlines = min(L, a.N1 - first);
         base = plane0 * Nk + first;
         cstride = 1;
@@ -303,7 +305,10 @@ __global__ void __launch_bounds__(NT) axis_fft_kernel(AxisFFTArgs a)
                 const int idx = threadIdx.x + e * NT;
                 const int c = a.axis == 1 ? idx / n : idx % L;
                 const int j = a.axis == 1 ? idx - c * n : idx / L;
-                if (idx < span && c < nlines) x[c * n + j] = cscale(x[c * n + j], a.scale * mul[e]);
+                if (idx < span && c < nlines) {
+                    if (a.r0 && a.axis == 0 && j == 0 && col0 + c == 0) a.r0[plane0] = x[c * n + j];
+                    x[c * n + j] = cscale(x[c * n + j], a.scale * mul[e]);
+                }
             }
         }
         __syncthreads();

@@ -99,6 +99,7 @@ __global__ void __launch_bounds__(NT) two_step_kernel(AxisFFTArgs a)
     const int tb = xcd_contiguous_block();  // neighbouring line groups side by side in one XCD
     long base, cstride, rowstride;
     long row0 = 0, plane0 = 0;  // first global row (axis 1), plane (axis 0)
+    int col0 = 0;               // first column (axis 0)
     int nlines;
     if (a.axis == 1) {
         row0 = long(tb) * L;
@@ -110,6 +111,7 @@ __global__ void __launch_bounds__(NT) two_step_kernel(AxisFFTArgs a)
         const int nblk = (a.N1 + L - 1) / L;
         plane0 = tb / nblk;
         const int first = int(tb - plane0 * nblk) * L;
+        col0 = first;
         nlines = min(L, a.N1 - first);
         base = plane0 * Nk + first;
         cstride = 1;
@@ -167,6 +169,7 @@ __global__ void __launch_bounds__(NT) two_step_kernel(AxisFFTArgs a)
                     v[k] = cmul_i(v[k], coef * double(gamma_centred(k, RA)));
                 });
             } else if constexpr (mode == 3) {
+                if (a.r0 && a.axis == 0 && col0 + lineA == 0) a.r0[plane0] = v[0];
                 sfor<RA>([&](auto k_) {
                     constexpr int k = decltype(k_)::value;
                     v[k] = cscale(v[k], a.scale * mul[k]);
@@ -289,6 +292,8 @@ __global__ void __launch_bounds__(NT) two_step_kernel(AxisFFTArgs a)
                             u[k] = cmul(u[k], phase(a, ka + RA * k, n));
                         });
                     } else {
+                        // frequency ka + RA k: zero at ka = 0, k = 0
+                        if (a.r0 && a.axis == 0 && ka == 0 && col0 + lineB == 0) a.r0[plane0] = u[0];
                         sfor<RB>([&](auto k_) {
                             constexpr int k = decltype(k_)::value;
                             u[k] = cscale(u[k], a.scale * mul[k]);

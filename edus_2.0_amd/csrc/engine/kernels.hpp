@@ -52,6 +52,10 @@ struct AxisFFTArgs {
     // mode 5: the unscaled forward transform of in, as mode 1 with scale 1
     // and no addend would store it (two-step kernels only)
     double2* out2;
+    // mode 3 along columns (axis 0): each plane's zero-frequency value of the
+    // forward transform on column 0, r0[plane] (null: not stored); with the
+    // unscaled forward row transform as input it is the plane's sum over k
+    double2* r0;
 };
 
 // Per-stage update rule of the fused commutator kernel.  With f the
@@ -89,7 +93,12 @@ struct CommArgs {
     double E[3];
     const double* E_tab;      // per-instance field [B][3] on device, replaces E when set
     const double2* sigma;     // Hartree-Fock self energy in k (may be null)
-    const double2* hdiag;     // Hartree diagonal shift, [B][nb] complex on device (may be null)
+    // Hartree diagonal shift (may be null): every plane's sum over k, [B][P]
+    // (AxisFFTArgs::r0 of the exchange pass), V [B][nb][nb] and 1 / Nk;
+    // the kernel forms the shift with hartree_shift
+    const double2* hsum;
+    const double* hart;
+    double inv_nk;
     StageUpdate up;
     // filled by launch_comm from xi and sigma: the addends of H in the order
     // xi_0, xi_1, xi_2, Sigma (present ones only); add_dd is the field

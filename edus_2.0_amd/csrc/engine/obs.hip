@@ -130,69 +130,16 @@ __global__ void reduce_partials_kernel(const double* __restrict__ partial, int n
     out[v] = s * scale;
 }
 
-// Per-block partial sums of the diagonal planes (for the Hartree k-mean).
-__global__ void __launch_bounds__(kObsBlock) diag_sum_kernel(const double2* __restrict__ rho, int nb, int Nk,
-                                                             int layout, int P, double2* __restrict__ partial)
+// hdiag = Hartree shift of B stacked instances from their plane sums over k
+// (set-up: the shift of the reference state, folded into H0); the step
+// kernels form the same shift with the same function.
+__global__ void hartree_sums_kernel(const double2* __restrict__ hsum, const double* __restrict__ hart, int nb,
+                                    int layout, int P, double inv_nk, double2* __restrict__ hdiag)
 {
-    __shared__ double2 sh[kObsBlock];
-    const int n = blockIdx.y;
-    const int inst = blockIdx.z;  // stacked instances [B][P][Nk]
-    const double2* plane = rho + (long(inst) * P + diag_plane(n, nb, layout)) * Nk;
-    double2 s = make_double2(0., 0.);
-    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < Nk; k += gridDim.x * blockDim.x) s = cadd(s, plane[k]);
-    sh[threadIdx.x] = s;
-    __syncthreads();
-    for (int st = kObsBlock / 2; st > 0; st >>= 1) {
-        if (threadIdx.x < st) sh[threadIdx.x] = cadd(sh[threadIdx.x], sh[threadIdx.x + st]);
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) partial[(long(inst) * nb + n) * gridDim.x + blockIdx.x] = sh[0];
-}
-
-// hdiag_n = sum_m Hartree_nm * rho_mm(R=0), rho_mm(R=0) = mean_k rho_mm(k)
-// (Coulomb.cpp:150-159; the rho0 part is folded into H0 at set-up).
-// One block per instance; partials [B][nb][nblk] (nblk <= 64), hart
-// [B][nb][nb], hdiag [B][nb].  All partials are fetched at once (one memory
-// round trip), then each diagonal element adds its nblk partials in block
-// order.
-constexpr int kMaxDiagBlocks = 64;
-constexpr int kHartreeLoads = (kMaxNb * kMaxDiagBlocks + kObsBlock - 1) / kObsBlock;
-
-__global__ void __launch_bounds__(kObsBlock) hartree_kernel(const double2* __restrict__ partial, int nblk,
-                                                            const double* __restrict__ hart, int nb, double invNk,
-                                                            double2* __restrict__ hdiag)
-{
-    __shared__ double2 items[kMaxNb * kMaxDiagBlocks];
-    __shared__ double2 mean[kMaxNb];
-    const int t = threadIdx.x;
-    const int inst = blockIdx.x;
-    partial += long(inst) * nb * nblk;
-    const int total = nb * nblk;
-    double2 v[kHartreeLoads];
-#pragma unroll
-    for (int r = 0; r < kHartreeLoads; ++r) {
-        const int i = t + r * kObsBlock;
-        v[r] = make_double2(0., 0.);
-        if (i < total) v[r] = partial[i];
-    }
-#pragma unroll
-    for (int r = 0; r < kHartreeLoads; ++r) {
-        const int i = t + r * kObsBlock;
-        if (i < total) items[i] = v[r];
-    }
-    __syncthreads();
-    if (t < nb) {
-        double2 s = make_double2(0., 0.);
-        for (int b = 0; b < nblk; ++b) s = cadd(s, items[t * nblk + b]);
-        mean[t] = cscale(s, invNk);
-    }
-    __syncthreads();
-    if (t < nb) {
-        const double* hrow = hart + (long(inst) * nb + t) * nb;
-        double2 h = make_double2(0., 0.);
-        for (int m = 0; m < nb; ++m) h = cadd(h, cscale(mean[m], hrow[m]));
-        hdiag[long(inst) * nb + t] = h;
-    }
+    const int inst = blockIdx.x, n = threadIdx.x;
+    if (n < nb)
+        hdiag[long(inst) * nb + n] = hartree_shift(hsum + long(inst) * P, hart + long(inst) * nb * nb, n, nb, layout,
+                                                   inv_nk);
 }
 
 __global__ void axpy_planes_kernel(double2* __restrict__ y, const double2* __restrict__ x, double a, long n)
@@ -309,14 +256,10 @@ void launch_observables(const ObsArgs& a, double* out, hipStream_t s)
     hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(64), 0, s, a.partial, blocks, nv, 1.0 / double(a.Nk),
                        out);
 }
-int diag_blocks(int Nk) { return int(std::min<long>(grid1d(Nk, kObsBlock), kMaxDiagBlocks)); }
-void launch_hartree(const double2* rho, int nb, int Nk, int layout, int P, int B, const double* hart,
-                    double2* partial, double2* hdiag, hipStream_t s)
+void launch_hartree_sums(const double2* hsum, const double* hart, int nb, int layout, int P, int B, double inv_nk,
+                         double2* hdiag, hipStream_t s)
 {
-    const int nblk = diag_blocks(Nk);
-    hipLaunchKernelGGL(diag_sum_kernel, dim3(nblk, nb, B), dim3(kObsBlock), 0, s, rho, nb, Nk, layout, P, partial);
-    hipLaunchKernelGGL(hartree_kernel, dim3(B), dim3(kObsBlock), 0, s, partial, nblk, hart, nb, 1.0 / double(Nk),
-                       hdiag);
+    hipLaunchKernelGGL(hartree_sums_kernel, dim3(B), dim3(64), 0, s, hsum, hart, nb, layout, P, inv_nk, hdiag);
 }
 void launch_axpy(double2* y, const double2* x, double a, long n, hipStream_t s)
 {

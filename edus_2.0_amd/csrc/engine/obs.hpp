@@ -20,10 +20,10 @@ void launch_to_planes_real(const double2* full, double* planes, int nb, int Nk, 
 void launch_full_to_planes(const double2* full, double2* planes, int nb, int Nk, hipStream_t s);
 int observables_blocks(int Nk);
 void launch_observables(const ObsArgs& a, double* out, hipStream_t s);
-int diag_blocks(int Nk);
-// Hartree shift of B stacked instances ([B][P][Nk]); partial [B][nb][diag_blocks]
-void launch_hartree(const double2* rho, int nb, int Nk, int layout, int P, int B, const double* hart,
-                    double2* partial, double2* hdiag, hipStream_t s);
+// Hartree shift hdiag [B][nb] of B stacked instances from their plane sums
+// over k, hsum [B][P] (hartree_shift)
+void launch_hartree_sums(const double2* hsum, const double* hart, int nb, int layout, int P, int B, double inv_nk,
+                         double2* hdiag, hipStream_t s);
 void launch_axpy(double2* y, const double2* x, double a, long n, hipStream_t s);
 // out = in rounded to FP32 (nearest), n complex values
 void launch_to_f32(const double2* in, float2* out, long n, hipStream_t s);
