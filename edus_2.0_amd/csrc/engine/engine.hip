@@ -1255,7 +1255,16 @@ void Engine::evaluate_peierls(const double2* in, const double E[3], const double
     // cancel; H0 already holds the rho0 terms, inside the phase as the
     // reference has them)
     const double2* add[3] = {xi[0], xi[1], xi[2]};
-    peierls_sandwich(H0, add, E, Av, +1, Hp);
+    // S_A[H0 + E.xi] depends on E and A only: RK4 stages 2 and 3 share them
+    // (t + dt/2), and so do a step's stage 4 and the next step's stage 1 when
+    // the host's times agree to the bit.  Hp is then reused; the sandwich
+    // would recompute the same values bit for bit.
+    const double key[6] = {E[0], E[1], E[2], Av[0], Av[1], Av[2]};
+    if (!hp_valid || std::memcmp(key, hp_key, sizeof(key)) != 0) {
+        peierls_sandwich(H0, add, E, Av, +1, Hp);
+        std::memcpy(hp_key, key, sizeof(key));
+        hp_valid = true;
+    }
     if (coulomb) fock(in);
     CommArgs c{};
     c.Nk = Nk;
@@ -1695,6 +1704,7 @@ int edus_set_tuning(edus_ctx* ctx, const edus_tuning* in)
     EDUS_REQUIRE(ctx && in);
     return guarded([&] {
         Engine& e = *ctx->e;
+        e.hp_valid = false;  // the phased H is formed again with the new kernels
         edus::FFTChoice next[2];
         for (int a = 0; a < 2; ++a) {
             const int n = a == 0 ? e.N0 : e.N1;

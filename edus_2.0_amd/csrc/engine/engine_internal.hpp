@@ -70,6 +70,9 @@ class Engine {
     // E.grad_k in by the stencil into D (E or, for tables, E_tab on the device)
     const double2* kgradient(const double2* in, const double E[3], const double* E_tab);
     double2 *Hp = nullptr, *Tq = nullptr;  // Peierls: phased H(k) and its axis-0 half
+    // E and A that the phased H in Hp was formed with (hp_valid: it was)
+    bool hp_valid = false;
+    double hp_key[6] = {};
     double2* ptab[2] = {nullptr, nullptr};   // Peierls: per-axis phase tables (N0, N1 entries)
     double A[3][3];
     std::vector<std::array<double, 9>> Ainst;  // lattice of each stacked instance

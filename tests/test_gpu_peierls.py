@@ -167,3 +167,22 @@ def test_rhs_peierls_mixed_precision(model_dir, name, grid):
     rel = np.abs(f - f_ref).max() / np.abs(f_ref).max()
     _say(name, grid, "Peierls mixed precision: max|df| / max|f| =", rel)
     assert 1e-12 < rel <= 2e-6
+
+
+def test_phased_h_reuse_is_bit_identical(model_dir):
+    """S_A[H0 + E.xi] is formed once per distinct (E, A): RK4 stages 2 and 3
+    share it, and a step's stage 1 reuses the previous stage 4's when the
+    host times agree to the bit.  A run that forms it again at every step
+    (a tuning write drops the kept one) ends bit for bit where the other does."""
+    p = _model("ipa14_hex120", model_dir, [8, 10])
+    p["peierls"] = True
+    p["lasers"][0]["t0"] = -2.0
+    a = edus.Simulation(p)
+    b = edus.Simulation(p)
+    a.steps(12)
+    for _ in range(12):
+        b.engine.set_tuning(b.engine.get_tuning())
+        b.steps(1)
+    ra, rb = a.engine.get_state(), b.engine.get_state()
+    _say("max|rho_reuse - rho_fresh| =", np.abs(ra - rb).max(), "steps", 12)
+    assert np.array_equal(ra, rb)
