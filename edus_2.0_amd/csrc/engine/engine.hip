@@ -1221,13 +1221,13 @@ void Engine::peierls_sandwich(const double2* in, const double2* const add[3], co
     for (int j = 0; j < 2; ++j) th[j] = sign * (Av[0] * A[0][j] + Av[1] * A[1][j] + Av[2] * A[2][j]);
     const double2* src = in;
     const bool two = N0 > 1 && N1 > 1;
+    // the phase of each frequency once per sandwich, not once per element
+    launch_phase_tables(ptab[0], N0 > 1 ? N0 : 0, th[0], 1.0 / N0, ptab[1], N1 > 1 ? N1 : 0, th[1], 1.0 / N1, s);
     for (int axis = 0; axis < 2; ++axis) {
         const int n = axis == 0 ? N0 : N1;
         if (n < 2) continue;
         AxisFFTArgs a = axis_args(axis, fft_choice[axis], 4, src, nullptr, (two && axis == 0) ? Tq : out, th[axis],
                                   1.0 / n, nullptr);
-        // the phase of each frequency once per launch, not once per element
-        launch_phase_table(ptab[axis], n, th[axis], 1.0 / n, s);
         a.ptab = ptab[axis];
         if (src == in)
             for (int d = 0; d < 3; ++d) {  // a zero field component reads nothing

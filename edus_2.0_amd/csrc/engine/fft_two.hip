@@ -400,18 +400,34 @@ void launch_two(const AxisFFTArgs& a, int variant, hipStream_t s)
 
 }  // namespace
 
-__global__ void phase_table_kernel(double2* __restrict__ tab, int n, double coef, double scale)
+// both axes' tables in one launch: index j < n0 fills tab0, the next n1 tab1
+__global__ void phase_table_kernel(double2* __restrict__ tab0, int n0, double coef0, double scale0,
+                                   double2* __restrict__ tab1, int n1, double coef1, double scale1)
 {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    double2* tab = tab0;
+    int n = n0;
+    double coef = coef0, scale = scale0;
+    if (j >= n0) {
+        j -= n0;
+        tab = tab1;
+        n = n1;
+        coef = coef1;
+        scale = scale1;
+    }
     if (j >= n) return;
     double sn, cs;
     sincos(coef * double(gamma_centred(j, n)), &sn, &cs);
     tab[j] = make_double2(scale * cs, scale * sn);
 }
 
-void launch_phase_table(double2* tab, int n, double coef, double scale, hipStream_t s)
+void launch_phase_tables(double2* tab0, int n0, double coef0, double scale0, double2* tab1, int n1, double coef1,
+                         double scale1, hipStream_t s)
 {
-    hipLaunchKernelGGL(phase_table_kernel, dim3((n + 255) / 256), dim3(256), 0, s, tab, n, coef, scale);
+    const int n = n0 + n1;
+    if (n == 0) return;
+    hipLaunchKernelGGL(phase_table_kernel, dim3((n + 255) / 256), dim3(256), 0, s, tab0, n0, coef0, scale0, tab1, n1,
+                       coef1, scale1);
 }
 
 int two_step_max_lines(int n, int variant)

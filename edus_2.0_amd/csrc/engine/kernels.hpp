@@ -47,7 +47,7 @@ struct AxisFFTArgs {
     const double2* in_add[3];
     double in_add_c[3];
     // mode 4 multiplier per frequency index j < n: scale * exp(i coef R(j)),
-    // R(j) Gamma-centred (launch_phase_table); null: formed per element
+    // R(j) Gamma-centred (launch_phase_tables); null: formed per element
     const double2* ptab;
     // mode 5: the unscaled forward transform of in, as mode 1 with scale 1
     // and no addend would store it (two-step kernels only)
@@ -155,8 +155,10 @@ struct FFTChoice {
     int L = 1;         // lines per work-group
 };
 void launch_fft(const AxisFFTArgs& a, const FFTChoice& c, hipStream_t s);
-// tab[j] = scale * exp(i coef R(j)), j < n, R(j) the Gamma-centred index
-void launch_phase_table(double2* tab, int n, double coef, double scale, hipStream_t s);
+// tab[j] = scale * exp(i coef R(j)), j < n, R(j) the Gamma-centred index, for
+// two tables in one launch (n0 or n1 may be 0: that table is left alone)
+void launch_phase_tables(double2* tab0, int n0, double coef0, double scale0, double2* tab1, int n1, double coef1,
+                         double scale1, hipStream_t s);
 // Engines for [H, rho] in the commutator kernel: FP64 FMAs, FP64 MFMA, or
 // FP32 MFMA with FP32 accumulation (mixed precision; everything else FP64).
 enum CommKind { kCommScalar = 0, kCommMfma64 = 1, kCommMfma32 = 2 };
