@@ -183,7 +183,7 @@ def main():
 
     if args.gpus != 1:
         print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported "
-              "(multi-GPU sharding is described in DESIGN.md, not built)", file=sys.stderr)
+              "(the k-grid is not sharded across GPUs)", file=sys.stderr)
         sys.exit(2)
     if args.workload == "hetero_batch64":
         run_batch(args)
